@@ -1,0 +1,129 @@
+/* C ABI of the gfx950 Ros2 column-step library (libvulcan_hip.so).
+ *
+ * Every compute entry point takes caller-owned DEVICE buffers (plain
+ * pointers), an explicit hipStream_t (passed as void*), runs asynchronously
+ * on that stream and returns an int status: VK_OK or an error code whose
+ * text is available from vk_last_error() (thread-local).  Numerical trouble
+ * (zero pivot, non-finite state) never aborts a batch: it sets bits in a
+ * per-column int32 status word instead.
+ *
+ * Batches.  A vk_batch describes ncols columns packed into flat buffers.
+ * Column c has nz[c] layers, ni[c] species and S[c] rate rows, and uses
+ * network net[c].  Per-column arrays sit at these offsets (in doubles):
+ *   state-shaped (nz, ni)      : off_s[c]   (y, sol, delta, rhs, sup/sub...)
+ *   natural blocks (nz, ni, ni): off_s[c]*ni[c]
+ *   rate table (S, nz)         : off_k[c]   (the reference's k_arr layout)
+ *   atmosphere record          : off_a[c]   (layout: vk_atm_record_layout)
+ *   block stacks               : off_b[c]   (nz * vk_block_doubles(ni))
+ * vk_batch_uniform() builds the table for B identical columns stacked
+ * along a leading axis, which is exactly jax.vmap's layout.
+ *
+ * Reference interfaces replaced (file:line in imalsky/jax-vulcan):
+ *   vk_network_create    chem.to_jax / NetworkArrays           chem.py:28-80
+ *   vk_chem_rhs_f64      chem.chem_rhs                         chem.py:83-131
+ *   vk_chem_jac_f64      chem.chem_jac_analytical              chem.py:144-214
+ *   vk_bt_factor_f64     solver.factor_block_thomas_diag_offdiag solver.py:33-64
+ *   vk_bt_solve_f64      solver.solve_block_thomas_diag_offdiag  solver.py:67-112
+ *   vk_ros2_step_f64     jax_step.jax_ros2_step                jax_step.py:249-315
+ *   vk_residual_f64      steady_state_grad.steady_state_residual steady_state_grad.py:116-150
+ *   vk_jacobian_blocks_f64 steady_state_grad._build_jacobian_blocks
+ *                                                               steady_state_grad.py:153-175
+ */
+#ifndef VULCAN_ROS2_H
+#define VULCAN_ROS2_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  VK_OK = 0,
+  VK_ERR_ARG = 1,         /* bad argument (shape, null pointer, range) */
+  VK_ERR_HIP = 2,         /* HIP runtime error */
+  VK_ERR_NOMEM = 3,       /* device or host allocation failed */
+  VK_ERR_UNSUPPORTED = 4  /* size outside what the kernels support */
+};
+
+typedef struct vk_network_s* vk_network;
+typedef struct vk_batch_s* vk_batch;
+
+/* Last error text of the calling thread ("" when none). */
+const char* vk_last_error(void);
+/* Library ABI version (major*100 + minor). */
+int vk_version(void);
+/* Largest ni handled by the block kernels (16 * 7 = 112). */
+int vk_max_species(void);
+
+/* Upload one reaction network given in the reference's padded table layout
+ * (network.py:366-451): rows 1..nr (row 0 unused), T = max_terms slots per
+ * row, PAD = ni marks an empty slot, host pointers.  Builds the species-major
+ * production/loss lists and the entry-major Jacobian lists on the host.
+ * Replaces chem.to_jax (chem.py:70-80). */
+int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
+                      const int64_t* product_idx, const double* reactant_stoich,
+                      const double* product_stoich, const uint8_t* is_three_body,
+                      vk_network* out);
+int vk_network_destroy(vk_network net);
+
+/* Batch tables (host arrays of length ncols, copied to the device). */
+int vk_batch_create(int ncols, const int32_t* nz, const int32_t* ni, const int32_t* S,
+                    const int32_t* net, const int64_t* off_s, const int64_t* off_k,
+                    const int64_t* off_a, const int64_t* off_b, const vk_network* nets,
+                    int nnets, vk_batch* out);
+int vk_batch_uniform(int ncols, int nz, vk_network net, vk_batch* out);
+int vk_batch_destroy(vk_batch b);
+/* Doubles needed by a block stack of one layer with ni species. */
+int64_t vk_block_doubles(int ni);
+/* Total doubles of the packed block stacks of a batch. */
+int64_t vk_batch_block_doubles(vk_batch b);
+/* Length (doubles) of one atmosphere record and the offsets of its fields,
+ * in this order: Kzz dzi vz Hpi Ti (nz-1 each), Tco g M (nz), ms alpha
+ * top_flux bot_flux bot_vdep gas_mask (ni), Dzz (nz-1,ni), vm (nz,ni),
+ * vs (nz-1,ni), flags[4] = use_vm_mol use_settling use_topflux use_botflux.
+ * Field set and meaning: jax_step.AtmStatic (jax_step.py:26-49). */
+int64_t vk_atm_record_layout(int nz, int ni, int64_t* offsets18);
+
+/* out (off_s) = chemistry rhs at y.  M: per-layer (ncols, nz) array for a
+ * uniform batch, or NULL to read M from the atmosphere records. */
+int vk_chem_rhs_f64(vk_batch b, const double* y, const double* M, const double* atm,
+                    const double* k, double* out, void* stream);
+/* jac (natural blocks, off_s*ni) = analytical chemistry Jacobian at y. */
+int vk_chem_jac_f64(vk_batch b, const double* y, const double* M, const double* atm,
+                    const double* k, double* jac, void* stream);
+
+/* Block-Thomas factor.  diag: natural blocks (off_s*ni); sup_d, sub_d:
+ * (nz-1, ni) at off_s.  transpose != 0 factors the system whose blocks are
+ * diag[j]^T (the adjoint's transposed solve; pass sup/sub swapped as the
+ * reference does, steady_state_grad.py:316-317).  inv receives the stack of
+ * inverses inv(A'_j) in tile layout (off_b).  status: int32 per column. */
+int vk_bt_factor_f64(vk_batch b, const double* diag, const double* sup_d, const double* sub_d,
+                     int transpose, double* inv, int32_t* status, void* stream);
+/* x = solve with a factored stack; rhs and x at off_s; scratch: state-sized. */
+int vk_bt_solve_f64(vk_batch b, const double* inv, const double* sup_d, const double* sub_d,
+                    const double* rhs, double* x, double* scratch, void* stream);
+/* Copy a tile-layout inverse stack into natural (nz, ni, ni) blocks. */
+int vk_bt_inverse_natural_f64(vk_batch b, const double* inv, double* nat, void* stream);
+
+/* One Ros2 step for every column: sol, delta at off_s.  dt: per-column
+ * device array.  fix_mask: optional uint8 state-shaped mask (NULL for none).
+ * ws: device workspace of vk_ros2_workspace_bytes() bytes. */
+size_t vk_ros2_workspace_bytes(vk_batch b);
+int vk_ros2_step_f64(vk_batch b, const double* y, const double* k, const double* atm,
+                     const double* dt, const uint8_t* fix_mask, double* sol, double* delta,
+                     void* ws, size_t ws_bytes, int32_t* status, void* stream);
+
+/* f(y) = chemistry + diffusion at y (off_s). */
+int vk_residual_f64(vk_batch b, const double* y, const double* k, const double* atm,
+                    double* out, void* stream);
+/* J_y blocks without c0: diag (natural, off_s*ni), sup_d = B+Bi, sub_d = C+Ci. */
+int vk_jacobian_blocks_f64(vk_batch b, const double* y, const double* k, const double* atm,
+                           double* diag, double* sup_d, double* sub_d, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VULCAN_ROS2_H */

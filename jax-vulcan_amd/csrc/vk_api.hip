@@ -1,0 +1,474 @@
+// Host side of the C ABI: error text, network upload, batch tables and the
+// entry points that sequence the kernels on the caller's stream.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <set>
+#include <vector>
+
+#include "vk_common.h"
+
+namespace {
+thread_local char g_err[512] = "";
+}
+
+void vk_set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag, int diag_tile,
+                        int transpose, const double* sup_d, const double* sub_d, double* inv,
+                        const double* rhs, double* rmod, int* status, const int* active,
+                        hipStream_t stream);
+int vk_launch_bt_solve(int sl, const VkCol* cols, int ncols, const double* inv,
+                       const double* sup_d, const double* sub_d, const double* rhs,
+                       double* rscratch, double* x, int mode, SolveEpi epi, const int* active,
+                       hipStream_t stream);
+int vk_launch_tile_to_nat(int sl, const VkCol* cols, int ncols, int max_nz, const double* tile,
+                          double* nat, hipStream_t stream);
+int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max_nz, int max_S,
+                         int max_T, const VkNetDev* nets, const double* y, const double* M,
+                         const double* atm, const double* k, const double* dt, const uint8_t* fix,
+                         const double* k1, double* out0, double* out_blk, double* out_sup,
+                         double* out_sub, const int* active, hipStream_t stream);
+
+struct vk_network_s {
+  VkNetDev dev;      // device pointers into buf
+  void* buf = nullptr;
+  int ni = 0, nr = 0, T = 0;
+};
+
+struct vk_batch_s {
+  int ncols = 0;
+  VkCol* d_cols = nullptr;
+  VkNetDev* d_nets = nullptr;
+  std::vector<VkCol> cols;
+  std::vector<int> slots;   // distinct SL buckets
+  int max_nz = 0, max_S = 0, max_T = 0;
+  int64_t state_doubles = 0, block_doubles = 0;
+  bool uniform = false;
+};
+
+namespace {
+
+enum { M_CHEM_RHS = 1, M_CHEM_JAC = 2, M_LHS = 3, M_RHS2 = 4, M_FTOT = 5, M_JY = 6 };
+
+template <class T>
+size_t align_up(size_t n) { return (n * sizeof(T) + 255) & ~size_t(255); }
+
+int check_batch(vk_batch b) {
+  if (!b) { vk_set_error("null batch"); return VK_ERR_ARG; }
+  return VK_OK;
+}
+
+double ros2_gamma() { return 1.0 + 1.0 / sqrt(2.0); }
+
+}  // namespace
+
+extern "C" {
+
+const char* vk_last_error(void) { return g_err; }
+int vk_version(void) { return 100; }
+int vk_max_species(void) { return VK_G * 7; }
+
+int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
+                      const int64_t* product_idx, const double* reactant_stoich,
+                      const double* product_stoich, const uint8_t* is_three_body,
+                      vk_network* out) {
+  g_err[0] = 0;
+  if (!out || !reactant_idx || !product_idx || !reactant_stoich || !product_stoich) {
+    vk_set_error("vk_network_create: null pointer");
+    return VK_ERR_ARG;
+  }
+  if (ni < 1 || ni > vk_max_species() || nr < 0 || T < 1 || T > 4) {
+    vk_set_error("vk_network_create: unsupported sizes ni=%d nr=%d T=%d (ni<=%d, T<=4)", ni, nr,
+                 T, vk_max_species());
+    return VK_ERR_UNSUPPORTED;
+  }
+  const int S = nr + 1;
+  std::vector<int> r_idx(S * T);
+  std::vector<double> r_st(S * T);
+  std::vector<int> is3(S, 0);
+  for (int r = 0; r < S; ++r) {
+    for (int s = 0; s < T; ++s) {
+      const int64_t ri = reactant_idx[r * T + s], pi = product_idx[r * T + s];
+      if (ri < 0 || ri > ni || pi < 0 || pi > ni) {
+        vk_set_error("vk_network_create: species index out of range at row %d slot %d", r, s);
+        return VK_ERR_ARG;
+      }
+      const double st = reactant_stoich[r * T + s];
+      if (st < 0 || st != floor(st) || st > 8 || product_stoich[r * T + s] < 0) {
+        vk_set_error("vk_network_create: stoichiometry must be a non-negative integer <= 8 "
+                     "(row %d slot %d)", r, s);
+        return VK_ERR_UNSUPPORTED;
+      }
+      r_idx[r * T + s] = (int)ri;
+      r_st[r * T + s] = st;
+    }
+    is3[r] = is_three_body ? (is_three_body[r] != 0) : 0;
+  }
+  // species-major loss / production lists in ascending (row, slot) order
+  std::vector<int> lp(ni + 1, 0), pp(ni + 1, 0);
+  std::vector<std::vector<std::pair<int, double>>> L(ni), P(ni);
+  for (int r = 0; r < S; ++r)
+    for (int s = 0; s < T; ++s) {
+      const int ri = (int)reactant_idx[r * T + s], pi = (int)product_idx[r * T + s];
+      if (ri < ni) L[ri].push_back({r * T + s, reactant_stoich[r * T + s]});
+      if (pi < ni) P[pi].push_back({r * T + s, product_stoich[r * T + s]});
+    }
+  std::vector<int> li, pi_;
+  std::vector<double> lc, pc;
+  for (int i = 0; i < ni; ++i) {
+    lp[i] = (int)li.size();
+    for (auto& e : L[i]) { li.push_back(e.first); lc.push_back(e.second); }
+    pp[i] = (int)pi_.size();
+    for (auto& e : P[i]) { pi_.push_back(e.first); pc.push_back(e.second); }
+  }
+  lp[ni] = (int)li.size();
+  pp[ni] = (int)pi_.size();
+  // entry-major Jacobian lists in ascending (row, out slot, in slot) order
+  std::vector<std::vector<std::pair<int, double>>> JL((size_t)ni * ni);
+  for (int r = 0; r < S; ++r)
+    for (int so = 0; so < 2 * T; ++so) {
+      const int oi = so < T ? (int)reactant_idx[r * T + so] : (int)product_idx[r * T + so - T];
+      const double oc = so < T ? -reactant_stoich[r * T + so] : product_stoich[r * T + so - T];
+      if (oi >= ni) continue;
+      for (int sj = 0; sj < T; ++sj) {
+        const int ii = (int)reactant_idx[r * T + sj];
+        if (ii >= ni) continue;
+        JL[(size_t)oi * ni + ii].push_back({r * T + sj, oc});
+      }
+    }
+  std::vector<int> jp((size_t)ni * ni + 1), ji;
+  std::vector<double> jc;
+  for (size_t e = 0; e < JL.size(); ++e) {
+    jp[e] = (int)ji.size();
+    for (auto& x : JL[e]) { ji.push_back(x.first); jc.push_back(x.second); }
+  }
+  jp[JL.size()] = (int)ji.size();
+  // one device allocation for every table
+  size_t off = 0;
+  auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+  const size_t o_ridx = take(r_idx.size() * 4), o_rst = take(r_st.size() * 8),
+               o_is3 = take(is3.size() * 4), o_lp = take(lp.size() * 4),
+               o_li = take(std::max<size_t>(li.size(), 1) * 4),
+               o_lc = take(std::max<size_t>(lc.size(), 1) * 8), o_pp = take(pp.size() * 4),
+               o_pi = take(std::max<size_t>(pi_.size(), 1) * 4),
+               o_pc = take(std::max<size_t>(pc.size(), 1) * 8), o_jp = take(jp.size() * 4),
+               o_ji = take(std::max<size_t>(ji.size(), 1) * 4),
+               o_jc = take(std::max<size_t>(jc.size(), 1) * 8);
+  std::vector<char> host(off, 0);
+  auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) memcpy(&host[o], src, bytes); };
+  put(o_ridx, r_idx.data(), r_idx.size() * 4);
+  put(o_rst, r_st.data(), r_st.size() * 8);
+  put(o_is3, is3.data(), is3.size() * 4);
+  put(o_lp, lp.data(), lp.size() * 4);
+  put(o_li, li.data(), li.size() * 4);
+  put(o_lc, lc.data(), lc.size() * 8);
+  put(o_pp, pp.data(), pp.size() * 4);
+  put(o_pi, pi_.data(), pi_.size() * 4);
+  put(o_pc, pc.data(), pc.size() * 8);
+  put(o_jp, jp.data(), jp.size() * 4);
+  put(o_ji, ji.data(), ji.size() * 4);
+  put(o_jc, jc.data(), jc.size() * 8);
+  vk_network_s* n = new vk_network_s();
+  if (hipMalloc(&n->buf, off) != hipSuccess) {
+    delete n;
+    vk_set_error("vk_network_create: hipMalloc(%zu) failed", off);
+    return VK_ERR_NOMEM;
+  }
+  if (hipMemcpy(n->buf, host.data(), off, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(n->buf);
+    delete n;
+    vk_set_error("vk_network_create: upload failed");
+    return VK_ERR_HIP;
+  }
+  char* b = (char*)n->buf;
+  n->dev.ni = ni; n->dev.S = S; n->dev.T = T; n->dev.pad = 0;
+  n->dev.r_idx = (const int*)(b + o_ridx);
+  n->dev.r_st = (const double*)(b + o_rst);
+  n->dev.is3 = (const int*)(b + o_is3);
+  n->dev.loss_ptr = (const int*)(b + o_lp);
+  n->dev.loss_item = (const int*)(b + o_li);
+  n->dev.loss_c = (const double*)(b + o_lc);
+  n->dev.prod_ptr = (const int*)(b + o_pp);
+  n->dev.prod_item = (const int*)(b + o_pi);
+  n->dev.prod_c = (const double*)(b + o_pc);
+  n->dev.jac_ptr = (const int*)(b + o_jp);
+  n->dev.jac_item = (const int*)(b + o_ji);
+  n->dev.jac_c = (const double*)(b + o_jc);
+  n->ni = ni; n->nr = nr; n->T = T;
+  *out = n;
+  return VK_OK;
+}
+
+int vk_network_destroy(vk_network net) {
+  if (!net) return VK_OK;
+  if (net->buf) (void)hipFree(net->buf);
+  delete net;
+  return VK_OK;
+}
+
+int64_t vk_block_doubles(int ni) {
+  const int sl = vk_slots(ni);
+  return (int64_t)sl * sl * VK_NT;
+}
+
+int vk_batch_create(int ncols, const int32_t* nz, const int32_t* ni, const int32_t* S,
+                    const int32_t* net, const int64_t* off_s, const int64_t* off_k,
+                    const int64_t* off_a, const int64_t* off_b, const vk_network* nets,
+                    int nnets, vk_batch* out) {
+  g_err[0] = 0;
+  if (!out || ncols < 1 || !nz || !ni || !S || !net || !off_s || !off_k || !off_a || !off_b ||
+      !nets || nnets < 1) {
+    vk_set_error("vk_batch_create: bad arguments");
+    return VK_ERR_ARG;
+  }
+  vk_batch_s* b = new vk_batch_s();
+  b->ncols = ncols;
+  b->cols.resize(ncols);
+  std::set<int> sls;
+  for (int c = 0; c < ncols; ++c) {
+    if (net[c] < 0 || net[c] >= nnets || !nets[net[c]]) {
+      delete b;
+      vk_set_error("vk_batch_create: column %d names network %d of %d", c, net[c], nnets);
+      return VK_ERR_ARG;
+    }
+    const vk_network_s* nw = nets[net[c]];
+    if (nz[c] < 2 || ni[c] != nw->ni || S[c] != nw->nr + 1 || ni[c] > vk_max_species()) {
+      delete b;
+      vk_set_error("vk_batch_create: column %d has nz=%d ni=%d S=%d but its network has "
+                   "ni=%d S=%d (nz >= 2, ni <= %d)", c, nz[c], ni[c], S[c], nw->ni, nw->nr + 1,
+                   vk_max_species());
+      return VK_ERR_ARG;
+    }
+    VkCol& cl = b->cols[c];
+    cl.nz = nz[c]; cl.ni = ni[c]; cl.S = S[c]; cl.net = net[c];
+    cl.off_s = off_s[c]; cl.off_k = off_k[c]; cl.off_a = off_a[c]; cl.off_b = off_b[c];
+    sls.insert(vk_slots(ni[c]));
+    b->max_nz = std::max(b->max_nz, (int)nz[c]);
+    b->max_S = std::max(b->max_S, (int)S[c]);
+    b->max_T = std::max(b->max_T, nw->T);
+    b->state_doubles = std::max<int64_t>(b->state_doubles, off_s[c] + (int64_t)nz[c] * ni[c]);
+    b->block_doubles =
+        std::max<int64_t>(b->block_doubles, off_b[c] + (int64_t)nz[c] * vk_block_doubles(ni[c]));
+  }
+  b->slots.assign(sls.begin(), sls.end());
+  std::vector<VkNetDev> hn(nnets);
+  for (int i = 0; i < nnets; ++i) hn[i] = nets[i] ? nets[i]->dev : VkNetDev{};
+  if (hipMalloc(&b->d_cols, sizeof(VkCol) * ncols) != hipSuccess ||
+      hipMalloc(&b->d_nets, sizeof(VkNetDev) * nnets) != hipSuccess) {
+    vk_batch_destroy(b);
+    vk_set_error("vk_batch_create: hipMalloc failed");
+    return VK_ERR_NOMEM;
+  }
+  if (hipMemcpy(b->d_cols, b->cols.data(), sizeof(VkCol) * ncols, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->d_nets, hn.data(), sizeof(VkNetDev) * nnets, hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    vk_batch_destroy(b);
+    vk_set_error("vk_batch_create: upload failed");
+    return VK_ERR_HIP;
+  }
+  *out = b;
+  return VK_OK;
+}
+
+int vk_batch_uniform(int ncols, int nz, vk_network net, vk_batch* out) {
+  if (!net) { vk_set_error("vk_batch_uniform: null network"); return VK_ERR_ARG; }
+  std::vector<int32_t> vnz(ncols, nz), vni(ncols, net->ni), vS(ncols, net->nr + 1),
+      vnet(ncols, 0);
+  std::vector<int64_t> os(ncols), ok(ncols), oa(ncols), ob(ncols);
+  const int64_t alen = vk_atm_offsets(nz, net->ni).total;
+  for (int c = 0; c < ncols; ++c) {
+    os[c] = (int64_t)c * nz * net->ni;
+    ok[c] = (int64_t)c * (net->nr + 1) * nz;
+    oa[c] = (int64_t)c * alen;
+    ob[c] = (int64_t)c * nz * vk_block_doubles(net->ni);
+  }
+  vk_network nets[1] = {net};
+  int rc = vk_batch_create(ncols, vnz.data(), vni.data(), vS.data(), vnet.data(), os.data(),
+                           ok.data(), oa.data(), ob.data(), nets, 1, out);
+  if (rc == VK_OK) (*out)->uniform = true;
+  return rc;
+}
+
+int vk_batch_destroy(vk_batch b) {
+  if (!b) return VK_OK;
+  if (b->d_cols) (void)hipFree(b->d_cols);
+  if (b->d_nets) (void)hipFree(b->d_nets);
+  delete b;
+  return VK_OK;
+}
+
+int64_t vk_batch_block_doubles(vk_batch b) { return b ? b->block_doubles : 0; }
+
+int64_t vk_atm_record_layout(int nz, int ni, int64_t* off) {
+  const VkAtmOff o = vk_atm_offsets(nz, ni);
+  if (off) {
+    const int64_t v[18] = {o.Kzz, o.dzi, o.vz, o.Hpi, o.Ti, o.Tco, o.g, o.M, o.ms,
+                           o.alpha, o.top_flux, o.bot_flux, o.bot_vdep, o.gas, o.Dzz, o.vm,
+                           o.vs, o.flags};
+    memcpy(off, v, sizeof(v));
+  }
+  return o.total;
+}
+
+static int chem_call(vk_batch b, int mode, const double* y, const double* M, const double* atm,
+                     const double* k, const double* dt, const uint8_t* fix, const double* k1,
+                     double* out0, double* out_blk, double* sup, double* sub, const int* active,
+                     hipStream_t st) {
+  for (int sl : b->slots) {
+    int rc = vk_launch_chem_layer(sl, mode, b->d_cols, b->ncols, b->max_nz, b->max_S, b->max_T,
+                                  b->d_nets, y, M, atm, k, dt, fix, k1, out0, out_blk, sup, sub,
+                                  active, st);
+    if (rc) return rc;
+  }
+  return VK_OK;
+}
+
+int vk_chem_rhs_f64(vk_batch b, const double* y, const double* M, const double* atm,
+                    const double* k, double* out, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  if (!y || !k || !out || (!M && !atm)) { vk_set_error("vk_chem_rhs_f64: null pointer"); return VK_ERR_ARG; }
+  if (M && !b->uniform) { vk_set_error("vk_chem_rhs_f64: explicit M needs a uniform batch"); return VK_ERR_ARG; }
+  return chem_call(b, M_CHEM_RHS, y, M, atm, k, nullptr, nullptr, nullptr, out, nullptr, nullptr,
+                   nullptr, nullptr, (hipStream_t)stream);
+}
+
+int vk_chem_jac_f64(vk_batch b, const double* y, const double* M, const double* atm,
+                    const double* k, double* jac, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  if (!y || !k || !jac || (!M && !atm)) { vk_set_error("vk_chem_jac_f64: null pointer"); return VK_ERR_ARG; }
+  if (M && !b->uniform) { vk_set_error("vk_chem_jac_f64: explicit M needs a uniform batch"); return VK_ERR_ARG; }
+  return chem_call(b, M_CHEM_JAC, y, M, atm, k, nullptr, nullptr, nullptr, jac, nullptr, nullptr,
+                   nullptr, nullptr, (hipStream_t)stream);
+}
+
+int vk_residual_f64(vk_batch b, const double* y, const double* k, const double* atm, double* out,
+                    void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  if (!y || !k || !atm || !out) { vk_set_error("vk_residual_f64: null pointer"); return VK_ERR_ARG; }
+  return chem_call(b, M_FTOT, y, nullptr, atm, k, nullptr, nullptr, nullptr, out, nullptr, nullptr,
+                   nullptr, nullptr, (hipStream_t)stream);
+}
+
+int vk_jacobian_blocks_f64(vk_batch b, const double* y, const double* k, const double* atm,
+                           double* diag, double* sup_d, double* sub_d, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  if (!y || !k || !atm || !diag || !sup_d || !sub_d) {
+    vk_set_error("vk_jacobian_blocks_f64: null pointer");
+    return VK_ERR_ARG;
+  }
+  return chem_call(b, M_JY, y, nullptr, atm, k, nullptr, nullptr, nullptr, diag, nullptr, sup_d,
+                   sub_d, nullptr, (hipStream_t)stream);
+}
+
+int vk_bt_factor_f64(vk_batch b, const double* diag, const double* sup_d, const double* sub_d,
+                     int transpose, double* inv, int32_t* status, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  if (!diag || !sup_d || !sub_d || !inv) { vk_set_error("vk_bt_factor_f64: null pointer"); return VK_ERR_ARG; }
+  for (int sl : b->slots) {
+    int rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, diag, 0, transpose, sup_d, sub_d, inv,
+                                 nullptr, nullptr, (int*)status, nullptr, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return VK_OK;
+}
+
+int vk_bt_solve_f64(vk_batch b, const double* inv, const double* sup_d, const double* sub_d,
+                    const double* rhs, double* x, double* scratch, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  if (!inv || !sup_d || !sub_d || !rhs || !x || !scratch) {
+    vk_set_error("vk_bt_solve_f64: null pointer");
+    return VK_ERR_ARG;
+  }
+  SolveEpi epi{};
+  for (int sl : b->slots) {
+    int rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, inv, sup_d, sub_d, rhs, scratch, x, 0,
+                                epi, nullptr, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return VK_OK;
+}
+
+int vk_bt_inverse_natural_f64(vk_batch b, const double* inv, double* nat, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  for (int sl : b->slots) {
+    int rc = vk_launch_tile_to_nat(sl, b->d_cols, b->ncols, b->max_nz, inv, nat,
+                                   (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return VK_OK;
+}
+
+size_t vk_ros2_workspace_bytes(vk_batch b) {
+  if (!b) return 0;
+  return align_up<double>(b->block_doubles) + 8 * align_up<double>(b->state_doubles);
+}
+
+int vk_ros2_step_f64(vk_batch b, const double* y, const double* k, const double* atm,
+                     const double* dt, const uint8_t* fix_mask, double* sol, double* delta,
+                     void* ws, size_t ws_bytes, int32_t* status, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  if (!y || !k || !atm || !dt || !sol || !delta || !ws) {
+    vk_set_error("vk_ros2_step_f64: null pointer");
+    return VK_ERR_ARG;
+  }
+  if (ws_bytes < vk_ros2_workspace_bytes(b)) {
+    vk_set_error("vk_ros2_step_f64: workspace %zu < %zu bytes", ws_bytes, vk_ros2_workspace_bytes(b));
+    return VK_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  char* w = (char*)ws;
+  double* blk = (double*)w; w += align_up<double>(b->block_doubles);
+  const size_t sv = align_up<double>(b->state_doubles);
+  double* rhs_y = (double*)w; w += sv;
+  double* rmod = (double*)w; w += sv;
+  double* k1 = (double*)w; w += sv;
+  double* yk2 = (double*)w; w += sv;
+  double* rhs2 = (double*)w; w += sv;
+  double* k2 = (double*)w; w += sv;
+  double* sup = (double*)w; w += sv;
+  double* sub = (double*)w; w += sv;
+  const double r = ros2_gamma();
+  for (int sl : b->slots) {
+    int rc;
+    if ((rc = vk_launch_chem_layer(sl, M_LHS, b->d_cols, b->ncols, b->max_nz, b->max_S, b->max_T,
+                                   b->d_nets, y, nullptr, atm, k, dt, fix_mask, nullptr, rhs_y, blk,
+                                   sup, sub, nullptr, st)))
+      return rc;
+    if ((rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, blk, 1, 0, sup, sub, blk, rhs_y, rmod,
+                                  (int*)status, nullptr, st)))
+      return rc;
+    SolveEpi e1{1, r, 0.0, y, nullptr, nullptr, yk2, nullptr};
+    if ((rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, rmod, rmod, k1, 1, e1,
+                                 nullptr, st)))
+      return rc;
+    if ((rc = vk_launch_chem_layer(sl, M_RHS2, b->d_cols, b->ncols, b->max_nz, b->max_S, b->max_T,
+                                   b->d_nets, yk2, nullptr, atm, k, dt, fix_mask, k1, rhs2, nullptr,
+                                   nullptr, nullptr, nullptr, st)))
+      return rc;
+    SolveEpi e2{2, 3.0 / (2.0 * r), 1.0 / (2.0 * r), y, k1, yk2, sol, delta};
+    if ((rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, rhs2, rmod, k2, 0, e2,
+                                 nullptr, st)))
+      return rc;
+  }
+  return VK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,454 @@
+// Block-tridiagonal Thomas factor and solves for diagonal off-diagonal blocks.
+//
+// Restates solver.factor_block_thomas_diag_offdiag (solver.py:33-64) and
+// solver.solve_block_thomas_diag_offdiag (solver.py:67-112):
+//   A'_0 = D_0,  A'_j = D_j - (c_j b_{j-1}^T) o inv(A'_{j-1})
+//   r'_0 = r_0,  r'_j = r_j - c_j o inv(A'_{j-1}) r'_{j-1}
+//   x_last = inv(A'_last) r'_last,  x_j = inv(A'_j) (r'_j - b_j o x_{j+1})
+// with b = sup_d, c = sub_d.  The reference keeps LU factors and calls
+// lu_solve(., I) for the inverse every layer; here each A'_j is inverted
+// once, in registers, by Gauss-Jordan elimination with partial pivoting (the
+// same pivot sequence as getrf's), and the stack of inverses replaces the
+// LU stack, so both sweeps become matrix-vector products.
+//
+// One workgroup of 256 threads owns one column.  Thread (tr, tc) = (tid/16,
+// tid%16) holds the SL x SL register tile a[s][t] = A[16s+tr][16t+tc].
+#include "vk_common.h"
+
+namespace {
+
+constexpr int NW = VK_NT / 64;  // waves per workgroup
+
+template <int SL>
+struct BtShared {
+  double cand_row[2][NW][VK_G * SL];  // each wave's best pivot row
+  double cand_val[2][NW];
+  int cand_idx[2][NW];
+  double xbuf[VK_G * VK_G * SL];      // permutation staging (16 x N)
+  double vec[2][VK_G * SL];           // sweep vector, double-buffered by layer
+  int perm[VK_G * SL];
+  int iperm[VK_G * SL];
+};
+
+// Invert the register-resident matrix a (n x n inside the padded N x N tile)
+// in place.  On return a holds inv(A) in natural order.  Returns nonzero
+// in every thread when a zero or non-finite pivot was met.
+template <int SL>
+__device__ int gj_invert(double (&a)[SL][SL], int n, BtShared<SL>& sh) {
+  const int tid = threadIdx.x;
+  const int tr = tid >> 4, tc = tid & 15;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  unsigned pivmask = 0;  // bit s: row 16s+tr already used as a pivot
+  int bad = 0;
+
+#pragma unroll
+  for (int kb = 0; kb < SL; ++kb) {
+    for (int kc = 0; kc < VK_G; ++kc) {
+      const int k = kb * VK_G + kc;
+      if (k >= n) break;
+      const int buf = k & 1;
+      // Column k of this thread's rows, broadcast from lane (tr, kc).
+      double colv[SL];
+#pragma unroll
+      for (int s = 0; s < SL; ++s) colv[s] = __shfl(a[s][kb], kc, 16);
+      // Best unpivoted row of this 16-lane group (first max: smallest row).
+      // A NaN entry ranks below every number but above "no candidate".
+      double bval = -1.0;
+      int brow = 0x7fffffff;
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        const int i = s * VK_G + tr;
+        const bool ok = (i < n) && !((pivmask >> s) & 1u);
+        double av = fabs(colv[s]);
+        if (!(av >= 0.0)) av = -0.5;
+        if (ok && av > bval) { bval = av; brow = i; }
+      }
+      // Reduce over the four groups of this wave.
+#pragma unroll
+      for (int off = 16; off < 64; off <<= 1) {
+        const double ov = __shfl_xor(bval, off, 64);
+        const int orow = __shfl_xor(brow, off, 64);
+        if (ov > bval || (ov == bval && orow < brow)) { bval = ov; brow = orow; }
+      }
+      brow = __builtin_amdgcn_readfirstlane(brow);
+      // The owning group publishes the whole candidate row.
+      if (brow != 0x7fffffff) {
+        const int ws = brow >> 4;
+        const bool own = (tr == (brow & 15));
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          if (s == ws && own) {
+#pragma unroll
+            for (int t = 0; t < SL; ++t) sh.cand_row[buf][wave][t * VK_G + tc] = a[s][t];
+          }
+        }
+      }
+      if (lane == 0) { sh.cand_val[buf][wave] = bval; sh.cand_idx[buf][wave] = brow; }
+      __syncthreads();
+      // Global pivot: best of the wave candidates.
+      double pval = sh.cand_val[buf][0];
+      int p = sh.cand_idx[buf][0];
+      int pw = 0;
+#pragma unroll
+      for (int w = 1; w < NW; ++w) {
+        const double v = sh.cand_val[buf][w];
+        const int r = sh.cand_idx[buf][w];
+        if (v > pval || (v == pval && r < p)) { pval = v; p = r; pw = w; }
+      }
+      p = __builtin_amdgcn_readfirstlane(p);
+      pw = __builtin_amdgcn_readfirstlane(pw);
+      if ((unsigned)p >= (unsigned)n) {  // cannot happen while k < n; keeps indices in range
+        p = k;
+        bad = 1;
+      }
+      double u[SL];
+#pragma unroll
+      for (int t = 0; t < SL; ++t) u[t] = sh.cand_row[buf][pw][t * VK_G + tc];
+      const double piv = sh.cand_row[buf][pw][k];
+      if (!(piv != 0.0) || !isfinite(piv)) bad = 1;
+      const double rpiv = 1.0 / piv;
+      if (tid == 0) sh.perm[k] = p;
+      // Rank-1 update of every row; column k and row p are rewritten below.
+      double l[SL];
+#pragma unroll
+      for (int s = 0; s < SL; ++s) l[s] = colv[s] * rpiv;
+#pragma unroll
+      for (int s = 0; s < SL; ++s)
+#pragma unroll
+        for (int t = 0; t < SL; ++t) a[s][t] = fma(-l[s], u[t], a[s][t]);
+      if (tc == kc) {
+#pragma unroll
+        for (int s = 0; s < SL; ++s) a[s][kb] = -l[s];
+      }
+      const int ps = p >> 4;
+      if (tr == (p & 15)) {
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          if (s == ps) {
+#pragma unroll
+            for (int t = 0; t < SL; ++t) a[s][t] = u[t] * rpiv;
+            if (tc == kc) a[s][kb] = rpiv;
+          }
+        }
+        pivmask |= 1u << ps;
+      }
+    }
+  }
+  // Padding indices map to themselves.
+  for (int i = n + tid; i < VK_G * SL; i += VK_NT) sh.perm[i] = i;
+  for (int i = tid; i < VK_G * SL; i += VK_NT) sh.iperm[i] = i;
+  __syncthreads();
+  for (int i = tid; i < VK_G * SL; i += VK_NT) sh.iperm[sh.perm[i]] = i;
+  __syncthreads();
+  // R -> X with X[a][b] = R[pi(a)][pi^-1(b)]: first the columns
+  // (R'[r][pi(s)] = R[r][s]), then the rows (X[pi^-1(r)][.] = R'[r][.]).
+  constexpr int N = VK_G * SL;
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+#pragma unroll
+    for (int t = 0; t < SL; ++t) sh.xbuf[tr * N + sh.perm[t * VK_G + tc]] = a[s][t];
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < SL; ++t) a[s][t] = sh.xbuf[tr * N + t * VK_G + tc];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int t = 0; t < SL; ++t) {
+#pragma unroll
+    for (int s = 0; s < SL; ++s) sh.xbuf[sh.iperm[s * VK_G + tr] * VK_G + tc] = a[s][t];
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SL; ++s) a[s][t] = sh.xbuf[(s * VK_G + tr) * VK_G + tc];
+    __syncthreads();
+  }
+  return bad;
+}
+
+// y_i = sum_q a[i][q] v[q] for this thread's rows, v in shared memory.
+template <int SL>
+__device__ inline void tile_matvec(const double (&a)[SL][SL], const double* v, double (&y)[SL]) {
+  const int tc = threadIdx.x & 15;
+  double vv[SL];
+#pragma unroll
+  for (int t = 0; t < SL; ++t) vv[t] = v[t * VK_G + tc];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    double acc = 0.0;
+#pragma unroll
+    for (int t = 0; t < SL; ++t) acc = fma(a[s][t], vv[t], acc);
+    y[s] = vk_rowsum16(acc);
+  }
+}
+
+template <int SL>
+__device__ inline void load_tile(double (&a)[SL][SL], const double* src) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int s = 0; s < SL; ++s)
+#pragma unroll
+    for (int t = 0; t < SL; ++t) a[s][t] = src[(s * SL + t) * VK_NT + tid];
+}
+
+template <int SL>
+__device__ inline void store_tile(const double (&a)[SL][SL], double* dst) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int s = 0; s < SL; ++s)
+#pragma unroll
+    for (int t = 0; t < SL; ++t) dst[(s * SL + t) * VK_NT + tid] = a[s][t];
+}
+
+// Element (i, q) of block j of a natural-layout (nz, ni, ni) stack, or of
+// its transpose; zero in the padding.
+__device__ inline double nat_elem(const double* blk, int ni, int i, int q, int transpose) {
+  if (i >= ni || q >= ni) return 0.0;
+  return transpose ? blk[(int64_t)q * ni + i] : blk[(int64_t)i * ni + q];
+}
+
+}  // namespace
+
+// Factor one column per workgroup.  diag is either natural (nz, ni, ni)
+// blocks at off_s*ni (diag_tile = 0) or tile-layout blocks at off_b
+// (diag_tile = 1, which may alias inv).  When rhs is non-null the forward
+// sweep r' is fused and written to rmod.
+template <int SL>
+__global__ __launch_bounds__(VK_NT) void k_bt_factor(
+    const VkCol* __restrict__ cols, int ncols, const double* diag, int diag_tile,
+    int transpose, const double* __restrict__ sup_d, const double* __restrict__ sub_d,
+    double* inv, const double* __restrict__ rhs, double* __restrict__ rmod,
+    int* __restrict__ status, const int* __restrict__ active) {
+  __shared__ BtShared<SL> sh;
+  const int c = blockIdx.x;
+  if (c >= ncols) return;
+  if (active && !(active[c])) return;
+  const VkCol col = cols[c];
+  if (vk_slots(col.ni) != SL) return;
+  const int nz = col.nz, ni = col.ni;
+  const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  constexpr int BLK = vk_blk<SL>();
+  const double* bsup = sup_d + col.off_s;  // (nz-1, ni)
+  const double* bsub = sub_d + col.off_s;
+  double a[SL][SL];
+  int bad = 0;
+  const bool sweep = (rhs != nullptr);
+  for (int j = 0; j < nz; ++j) {
+    const double* Dj_nat = diag + col.off_s * ni + (int64_t)j * ni * ni;
+    const double* Dj_tile = diag + col.off_b + (int64_t)j * BLK;
+    if (j == 0) {
+      if (diag_tile) {
+        load_tile<SL>(a, Dj_tile);
+      } else {
+#pragma unroll
+        for (int s = 0; s < SL; ++s)
+#pragma unroll
+          for (int t = 0; t < SL; ++t)
+            a[s][t] = nat_elem(Dj_nat, ni, s * VK_G + tr, t * VK_G + tc, transpose);
+      }
+      if (sweep) {
+        for (int i = tid; i < VK_G * SL; i += VK_NT) {
+          const double v = (i < ni) ? rhs[col.off_s + i] : 0.0;
+          sh.vec[0][i] = v;
+          if (i < ni) rmod[col.off_s + i] = v;
+        }
+      }
+    } else {
+      double cv[SL], bv[SL];
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        const int i = s * VK_G + tr;
+        cv[s] = (i < ni) ? bsub[(int64_t)(j - 1) * ni + i] : 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < SL; ++t) {
+        const int q = t * VK_G + tc;
+        bv[t] = (q < ni) ? bsup[(int64_t)(j - 1) * ni + q] : 0.0;
+      }
+      if (sweep) {
+        // r'_j = r_j - c_j o (X_{j-1} r'_{j-1}) while a still holds X_{j-1}.
+        double w[SL];
+        tile_matvec<SL>(a, sh.vec[(j - 1) & 1], w);
+        if (tc == 0) {
+#pragma unroll
+          for (int s = 0; s < SL; ++s) {
+            const int i = s * VK_G + tr;
+            double r = 0.0;
+            if (i < ni) {
+              r = rhs[col.off_s + (int64_t)j * ni + i] - cv[s] * w[s];
+              rmod[col.off_s + (int64_t)j * ni + i] = r;
+            }
+            sh.vec[j & 1][i] = r;
+          }
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < SL; ++s)
+#pragma unroll
+        for (int t = 0; t < SL; ++t) {
+          const double d = diag_tile
+              ? Dj_tile[(s * SL + t) * VK_NT + tid]
+              : nat_elem(Dj_nat, ni, s * VK_G + tr, t * VK_G + tc, transpose);
+          a[s][t] = d - (cv[s] * bv[t]) * a[s][t];
+        }
+    }
+    bad |= gj_invert<SL>(a, ni, sh);
+    store_tile<SL>(a, inv + col.off_b + (int64_t)j * BLK);
+  }
+  if (bad && tid == 0 && status) atomicOr(&status[c], VK_ST_SINGULAR);
+}
+
+// Sweeps with the inverse stack.  mode 0: forward + backward from rhs;
+// mode 1: backward only, rhs already holds r'.  Optional epilogues per
+// element of x_j (all pointers at off_s):
+//   epi 1: out1 = y + x / e0                          (Ros2 stage-1 yk2)
+//   epi 2: out1 = y + e0*k1 + e1*x, out2 = |out1 - yk2|  (Ros2 sol, delta)
+
+template <int SL>
+__global__ __launch_bounds__(VK_NT) void k_bt_solve(
+    const VkCol* __restrict__ cols, int ncols, const double* __restrict__ inv,
+    const double* __restrict__ sup_d, const double* __restrict__ sub_d,
+    const double* __restrict__ rhs, double* __restrict__ rscratch, double* __restrict__ x,
+    int mode, SolveEpi epi, const int* __restrict__ active) {
+  __shared__ double vec[2][VK_G * SL];
+  const int c = blockIdx.x;
+  if (c >= ncols) return;
+  if (active && !(active[c])) return;
+  const VkCol col = cols[c];
+  if (vk_slots(col.ni) != SL) return;
+  const int nz = col.nz, ni = col.ni;
+  const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  constexpr int BLK = vk_blk<SL>();
+  const double* bsup = sup_d + col.off_s;
+  const double* bsub = sub_d + col.off_s;
+  const double* stk = inv + col.off_b;
+  double a[SL][SL];
+  double w[SL];
+  const double* rm = rhs;  // r' source for the backward sweep
+  if (mode == 0) {
+    for (int i = tid; i < VK_G * SL; i += VK_NT) {
+      const double v = (i < ni) ? rhs[col.off_s + i] : 0.0;
+      vec[0][i] = v;
+      if (i < ni) rscratch[col.off_s + i] = v;
+    }
+    __syncthreads();
+    for (int j = 1; j < nz; ++j) {
+      load_tile<SL>(a, stk + (int64_t)(j - 1) * BLK);
+      tile_matvec<SL>(a, vec[(j - 1) & 1], w);
+      if (tc == 0) {
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          const int i = s * VK_G + tr;
+          double r = 0.0;
+          if (i < ni) {
+            r = rhs[col.off_s + (int64_t)j * ni + i] - bsub[(int64_t)(j - 1) * ni + i] * w[s];
+            rscratch[col.off_s + (int64_t)j * ni + i] = r;
+          }
+          vec[j & 1][i] = r;
+        }
+      }
+      __syncthreads();
+    }
+    rm = rscratch;
+    __syncthreads();
+  }
+  // Backward sweep.  vec holds t_j = r'_j - b_j o x_{j+1}.
+  for (int i = tid; i < VK_G * SL; i += VK_NT)
+    vec[(nz - 1) & 1][i] = (i < ni) ? rm[col.off_s + (int64_t)(nz - 1) * ni + i] : 0.0;
+  __syncthreads();
+  for (int j = nz - 1; j >= 0; --j) {
+    load_tile<SL>(a, stk + (int64_t)j * BLK);
+    tile_matvec<SL>(a, vec[j & 1], w);
+    if (tc == 0) {
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        const int i = s * VK_G + tr;
+        if (i < ni) {
+          const int64_t e = col.off_s + (int64_t)j * ni + i;
+          const double xv = w[s];
+          x[e] = xv;
+          if (epi.kind == 1) {
+            epi.out1[e] = epi.y[e] + xv / epi.e0;
+          } else if (epi.kind == 2) {
+#pragma clang fp contract(off)
+            const double sol = epi.y[e] + epi.e0 * epi.k1[e] + epi.e1 * xv;
+            epi.out1[e] = sol;
+            epi.out2[e] = fabs(sol - epi.yk2[e]);
+          }
+          if (j > 0) {
+            const int64_t e2 = col.off_s + (int64_t)(j - 1) * ni + i;
+            vec[(j - 1) & 1][i] = rm[e2] - bsup[(int64_t)(j - 1) * ni + i] * xv;
+          }
+        } else if (j > 0) {
+          vec[(j - 1) & 1][i] = 0.0;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Copy a tile-layout inverse stack into natural (nz, ni, ni) blocks.
+template <int SL>
+__global__ __launch_bounds__(VK_NT) void k_tile_to_nat(const VkCol* __restrict__ cols, int ncols,
+                                                       const double* __restrict__ tile,
+                                                       double* __restrict__ nat) {
+  const int c = blockIdx.y;
+  const int j = blockIdx.x;
+  if (c >= ncols) return;
+  const VkCol col = cols[c];
+  if (vk_slots(col.ni) != SL || j >= col.nz) return;
+  const int ni = col.ni, tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  const double* src = tile + col.off_b + (int64_t)j * vk_blk<SL>();
+  double* dst = nat + col.off_s * ni + (int64_t)j * ni * ni;
+  for (int s = 0; s < SL; ++s)
+    for (int t = 0; t < SL; ++t) {
+      const int i = s * VK_G + tr, q = t * VK_G + tc;
+      if (i < ni && q < ni) dst[(int64_t)i * ni + q] = src[(s * SL + t) * VK_NT + tid];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers (dispatch on the slot count of the batch bucket)
+// ---------------------------------------------------------------------------
+
+#define VK_DISPATCH_SL(sl, STMT)                 \
+  switch (sl) {                                  \
+    case 1: { constexpr int SL_ = 1; STMT; } break; \
+    case 2: { constexpr int SL_ = 2; STMT; } break; \
+    case 3: { constexpr int SL_ = 3; STMT; } break; \
+    case 4: { constexpr int SL_ = 4; STMT; } break; \
+    case 5: { constexpr int SL_ = 5; STMT; } break; \
+    case 6: { constexpr int SL_ = 6; STMT; } break; \
+    case 7: { constexpr int SL_ = 7; STMT; } break; \
+    default: return VK_ERR_UNSUPPORTED;          \
+  }
+
+int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag, int diag_tile,
+                        int transpose, const double* sup_d, const double* sub_d, double* inv,
+                        const double* rhs, double* rmod, int* status, const int* active,
+                        hipStream_t stream) {
+  VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_bt_factor<SL_>, dim3(ncols), dim3(VK_NT), 0, stream,
+                                        cols, ncols, diag, diag_tile, transpose, sup_d, sub_d,
+                                        inv, rhs, rmod, status, active));
+  VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}
+
+int vk_launch_bt_solve(int sl, const VkCol* cols, int ncols, const double* inv,
+                       const double* sup_d, const double* sub_d, const double* rhs,
+                       double* rscratch, double* x, int mode, SolveEpi epi, const int* active,
+                       hipStream_t stream) {
+  VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_bt_solve<SL_>, dim3(ncols), dim3(VK_NT), 0, stream,
+                                        cols, ncols, inv, sup_d, sub_d, rhs, rscratch, x, mode,
+                                        epi, active));
+  VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}
+
+int vk_launch_tile_to_nat(int sl, const VkCol* cols, int ncols, int max_nz, const double* tile,
+                          double* nat, hipStream_t stream) {
+  VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_tile_to_nat<SL_>, dim3(max_nz, ncols), dim3(VK_NT), 0,
+                                        stream, cols, ncols, tile, nat));
+  VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}

@@ -1,0 +1,400 @@
+// Chemistry right-hand side, analytical Jacobian and eddy/molecular
+// diffusion for one (layer, column) per workgroup.
+//
+// Restates chem.chem_rhs_per_layer (chem.py:83-125),
+// chem.chem_jac_analytical_per_layer (chem.py:144-208) and the diffusion
+// operator of jax_step.py:67-246.  Sums run over precomputed CSR lists in a
+// fixed order, with FP contraction off, so they repeat bit for bit from run
+// to run and from batch to batch (no atomics).
+#include "vk_common.h"
+
+namespace {
+
+// Kernel modes.
+enum {
+  M_CHEM_RHS = 1,  // out0 = chem rhs (natural nz x ni)
+  M_CHEM_JAC = 2,  // out0 = chem Jacobian (natural nz x ni x ni)
+  M_LHS = 3,       // Ros2 system at y: D tiles, rhs_y, -sup, -sub
+  M_RHS2 = 4,      // rhs2 = f(yk2) - 2/(gamma dt) k1
+  M_FTOT = 5,      // out0 = chem + diffusion
+  M_JY = 6,        // J_y = chem_J + diag(A+Ai) blocks (natural), sup, sub
+};
+
+struct LayerArgs {
+  const VkCol* cols;
+  int ncols;
+  const VkNetDev* nets;
+  const double* y;       // state the operator is evaluated at (off_s)
+  const double* M;       // per-layer M at off_s/ni*... or null: atm record
+  const double* atm;     // packed atmosphere records (off_a)
+  const double* k;       // (S, nz) rate tables (off_k)
+  const double* dt;      // per column
+  const uint8_t* fix;    // optional fix mask (off_s)
+  const double* k1;      // M_RHS2
+  double* out0;          // rhs-like output (off_s) or natural blocks
+  double* out_blk;       // M_LHS: tile blocks at off_b
+  double* out_sup;       // (nz-1, ni) at off_s
+  double* out_sub;
+  const int* active;
+  int mode;
+};
+
+constexpr double GAMMA = 1.7071067811865475;  // 1 + 1/sqrt(2)
+
+__device__ inline double posp(double v) { return v > 0.0 ? v : 0.0; }
+__device__ inline double negp(double v) { return v < 0.0 ? v : 0.0; }
+
+// Diffusion coefficients of layer z: eddy (A, B, C) and, for species i, the
+// molecular (Ai, Bi, Ci), following jax_step.py:67-220 term by term.
+struct DiffCoef {
+  double A, B, C;
+};
+
+__device__ inline DiffCoef eddy_coef(int z, int nz, const double* Kzz, const double* dzi,
+                                     const double* vz, double ysm, double ys0, double ysp) {
+#pragma clang fp contract(off)
+  DiffCoef d;
+  if (z == 0) {
+    d.A = -1.0 / dzi[0] * (Kzz[0] / dzi[0]) * (ysp + ys0) / 2.0 / ys0 - posp(vz[0]) / dzi[0];
+    d.B = 1.0 / dzi[0] * (Kzz[0] / dzi[0]) * (ysp + ys0) / 2.0 / ysp - negp(vz[0]) / dzi[0];
+    d.C = 0.0;
+  } else if (z == nz - 1) {
+    const int t = nz - 2;
+    d.A = -1.0 / dzi[t] * (Kzz[t] / dzi[t]) * (ys0 + ysm) / 2.0 / ys0 + negp(vz[t]) / dzi[t];
+    d.C = 1.0 / dzi[t] * (Kzz[t] / dzi[t]) * (ys0 + ysm) / 2.0 / ysm + posp(vz[t]) / dzi[t];
+    d.B = 0.0;
+  } else {
+    const double dza = 0.5 * (dzi[z - 1] + dzi[z]);
+    d.A = -1.0 / dza *
+              (Kzz[z] / dzi[z] * (ysp + ys0) / 2.0 + Kzz[z - 1] / dzi[z - 1] * (ys0 + ysm) / 2.0) /
+              ys0 -
+          (posp(vz[z]) - negp(vz[z - 1])) / dza;
+    d.B = 1.0 / dza * Kzz[z] / dzi[z] * (ysp + ys0) / 2.0 / ysp - negp(vz[z]) / dza;
+    d.C = 1.0 / dza * Kzz[z - 1] / dzi[z - 1] * (ys0 + ysm) / 2.0 / ysm + posp(vz[z - 1]) / dza;
+  }
+  return d;
+}
+
+struct AtmView {
+  const double *Kzz, *dzi, *vz, *Hpi, *Ti, *Tco, *g, *M;
+  const double *ms, *alpha, *top_flux, *bot_flux, *bot_vdep, *gas;
+  const double *Dzz, *vm, *vs, *flags;
+};
+
+__device__ inline AtmView atm_view(const double* base, int nz, int ni) {
+  const VkAtmOff o = vk_atm_offsets(nz, ni);
+  AtmView v;
+  v.Kzz = base + o.Kzz; v.dzi = base + o.dzi; v.vz = base + o.vz; v.Hpi = base + o.Hpi;
+  v.Ti = base + o.Ti; v.Tco = base + o.Tco; v.g = base + o.g; v.M = base + o.M;
+  v.ms = base + o.ms; v.alpha = base + o.alpha; v.top_flux = base + o.top_flux;
+  v.bot_flux = base + o.bot_flux; v.bot_vdep = base + o.bot_vdep; v.gas = base + o.gas;
+  v.Dzz = base + o.Dzz; v.vm = base + o.vm; v.vs = base + o.vs; v.flags = base + o.flags;
+  return v;
+}
+
+// gravity/thermal-diffusion factor of compute_diff_grav (jax_step.py:80-87):
+// -1/Hpi[a] + ms g[b] / (Navo kb Ti[a]) + alpha/Ti[a] (Tco[a+1]-Tco[a])/dzi[a]
+__device__ inline double grav_fac(const AtmView& v, int i, int a, int b) {
+#pragma clang fp contract(off)
+  const double NAKB = 6.02214086e23 * 1.38064852e-16;
+  return -1.0 / v.Hpi[a] + v.ms[i] * v.g[b] / (NAKB * v.Ti[a]) +
+         v.alpha[i] / v.Ti[a] * (v.Tco[a + 1] - v.Tco[a]) / v.dzi[a];
+}
+
+// Molecular coefficients of species i at layer z (jax_step.py:80-147, 195-218).
+__device__ inline DiffCoef mol_coef(int z, int nz, int ni, int i, const AtmView& v, double ysm,
+                                    double ys0, double ysp) {
+#pragma clang fp contract(off)
+  const double use_vm = v.flags[0] != 0.0 ? 1.0 : 0.0;
+  const double use_set = v.flags[1] != 0.0 ? 1.0 : 0.0;
+  const double* dzi = v.dzi;
+  DiffCoef d;
+  if (z == 0) {
+    const double D0 = v.Dzz[i];
+    const double g0 = 1.0 / dzi[0] * D0 / 2.0 * grav_fac(v, i, 0, 0);
+    const double vm0 = v.vm[i], vs0 = v.vs[i];
+    const double bA = (1.0 - use_vm) * g0 + use_vm * (-posp(vm0) / dzi[0]) + use_set * (-posp(vs0) / dzi[0]);
+    const double bB = (1.0 - use_vm) * g0 + use_vm * (-negp(vm0) / dzi[0]) + use_set * (-negp(vs0) / dzi[0]);
+    d.A = -1.0 / dzi[0] * (D0 / dzi[0]) * (ysp + ys0) / 2.0 / ys0 + bA;
+    d.B = 1.0 / dzi[0] * (D0 / dzi[0]) * (ysp + ys0) / 2.0 / ysp + bB;
+    d.C = 0.0;
+  } else if (z == nz - 1) {
+    const int t = nz - 2;
+    const double Dt = v.Dzz[(int64_t)t * ni + i];
+    const double gt = -1.0 / dzi[t] * Dt / 2.0 *
+                      (-1.0 / v.Hpi[t] + v.ms[i] * v.g[nz - 1] / (6.02214086e23 * 1.38064852e-16 * v.Ti[t]) +
+                       v.alpha[i] / v.Ti[t] * (v.Tco[nz - 1] - v.Tco[nz - 2]) / dzi[t]);
+    const double vmt = v.vm[(int64_t)(nz - 1) * ni + i], vst = v.vs[(int64_t)t * ni + i];
+    const double bA = (1.0 - use_vm) * gt + use_vm * (negp(vmt) / dzi[t]) + use_set * (negp(vst) / dzi[t]);
+    const double bC = (1.0 - use_vm) * gt + use_vm * (posp(vmt) / dzi[t]) + use_set * (posp(vst) / dzi[t]);
+    d.A = -1.0 / dzi[t] * (Dt / dzi[t]) * (ys0 + ysm) / 2.0 / ys0 + bA;
+    d.C = 1.0 / dzi[t] * (Dt / dzi[t]) * (ys0 + ysm) / 2.0 / ysm + bC;
+    d.B = 0.0;
+  } else {
+    const double dza = 0.5 * (dzi[z - 1] + dzi[z]);
+    const double Dj = v.Dzz[(int64_t)z * ni + i], Djm = v.Dzz[(int64_t)(z - 1) * ni + i];
+    const double inv2 = 1.0 / (2.0 * dza);
+    const double gA = inv2 * (Dj * grav_fac(v, i, z, z) - Djm * grav_fac(v, i, z - 1, z));
+    const double gB = inv2 * Dj * grav_fac(v, i, z, z + 1);
+    const double gC = -inv2 * Djm * grav_fac(v, i, z - 1, z - 1);
+    const double vmj = v.vm[(int64_t)z * ni + i], vmm = v.vm[(int64_t)(z - 1) * ni + i];
+    const double vsj = v.vs[(int64_t)z * ni + i], vsm = v.vs[(int64_t)(z - 1) * ni + i];
+    const double Avm = -(posp(vmj) - negp(vmm)) / dza, Bvm = -negp(vmj) / dza, Cvm = posp(vmm) / dza;
+    const double Avs = -(posp(vsj) - negp(vsm)) / dza, Bvs = -negp(vsj) / dza, Cvs = posp(vsm) / dza;
+    const double Ax = ((1.0 - use_vm) * gA + use_vm * Avm) + use_set * Avs;
+    const double Bx = ((1.0 - use_vm) * gB + use_vm * Bvm) + use_set * Bvs;
+    const double Cx = ((1.0 - use_vm) * gC + use_vm * Cvm) + use_set * Cvs;
+    d.A = -1.0 / dza *
+              (Dj / dzi[z] * (ysp + ys0) / 2.0 + Djm / dzi[z - 1] * (ys0 + ysm) / 2.0) / ys0 +
+          Ax;
+    d.B = 1.0 / dza * Dj / dzi[z] * (ysp + ys0) / 2.0 / ysp + Bx;
+    d.C = 1.0 / dza * Djm / dzi[z - 1] * (ys0 + ysm) / 2.0 / ysm + Cx;
+  }
+  return d;
+}
+
+__device__ inline double block_sum(double v, double* red) {
+  // sum over the 256 threads; result valid in every thread
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+}  // namespace
+
+// Dynamic shared memory: rate[S] + drate[S*T] (Jacobian modes) + 3 y rows.
+template <int SL>
+__global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
+  extern __shared__ double smem[];
+  __shared__ double ysum3[3];
+  __shared__ double red[4];
+  __shared__ double dd_s[VK_G * SL];  // A+Ai (+ bottom vdep for M_LHS) per species
+  __shared__ double bt_s[VK_G * SL];  // bottom vdep term, kept apart for M_JY
+  const int c = blockIdx.y;
+  const int z = blockIdx.x;
+  if (c >= L.ncols) return;
+  if (L.active && !L.active[c]) return;
+  const VkCol col = L.cols[c];
+  if (vk_slots(col.ni) != SL) return;  // columns of other buckets
+  const int nz = col.nz, ni = col.ni, tid = threadIdx.x;
+  if (z >= nz) return;
+  const VkNetDev net = L.nets[col.net];
+  const int S = net.S, T = net.T;
+  const bool want_jac = (L.mode == M_CHEM_JAC || L.mode == M_LHS || L.mode == M_JY);
+  const bool want_diff = (L.mode == M_LHS || L.mode == M_RHS2 || L.mode == M_FTOT || L.mode == M_JY);
+  const bool want_rhs = (L.mode != M_CHEM_JAC && L.mode != M_JY);
+  double* rate = smem;
+  double* drate = smem + S;
+  double* yrow = smem + S + (want_jac ? (int64_t)S * T : 0);  // [3][ni+1]
+  const double* y = L.y + col.off_s;
+  const double* atmb = L.atm ? L.atm + col.off_a : nullptr;
+  AtmView av;
+  if (atmb) av = atm_view(atmb, nz, ni);
+  const double Mz = L.M ? L.M[col.off_s / ni + z] : av.M[z];
+  const double* kz = L.k + col.off_k;  // k[r * nz + z]
+
+  // y rows z-1, z, z+1 with a trailing 1.0 pad (PAD index = ni)
+  for (int e = tid; e < 3 * (ni + 1); e += VK_NT) {
+    const int rr = e / (ni + 1), i = e % (ni + 1);
+    const int zz = z - 1 + rr;
+    double v = 1.0;
+    if (i < ni) v = (zz >= 0 && zz < nz) ? y[(int64_t)zz * ni + i] : 0.0;
+    yrow[e] = v;
+  }
+  __syncthreads();
+  const double* y0 = yrow + (ni + 1);
+
+  // reaction rates and rate derivatives (chem.py:92-99, 162-183)
+  for (int r = tid; r < S; r += VK_NT) {
+#pragma clang fp contract(off)
+    const double kr = kz[(int64_t)r * nz + z];
+    double fac[4];
+    double prodr = 1.0;
+    for (int s = 0; s < T; ++s) {
+      const int sp = net.r_idx[r * T + s];
+      const double st = net.r_st[r * T + s];
+      const double f = (st > 0.0) ? vk_ipow(y0[sp], st) : 1.0;
+      if (s < 4) fac[s] = f;
+      prodr = prodr * f;
+    }
+    double rt = kr * prodr;
+    const bool tb = net.is3[r] != 0;
+    if (tb) rt = rt * Mz;
+    rate[r] = rt;
+    if (want_jac) {
+      for (int s = 0; s < T; ++s) {
+        const int sp = net.r_idx[r * T + s];
+        const double st = net.r_st[r * T + s];
+        double leave = 1.0;
+        for (int l2 = 0; l2 < T; ++l2)
+          if (l2 != s) leave = leave * (l2 < 4 ? fac[l2] : 1.0);
+        const double pm1 = (st > 0.0) ? vk_ipow(y0[sp], st - 1.0) : 0.0;
+        double dr = st * pm1 * leave * kr;
+        if (tb) dr = dr * Mz;
+        drate[r * T + s] = dr;
+      }
+    }
+  }
+  // column sums of y for the eddy terms (gas species only)
+  if (want_diff) {
+    for (int rr = 0; rr < 3; ++rr) {
+      const int zz = z - 1 + rr;
+      double v = 0.0;
+      if (zz >= 0 && zz < nz) {
+        // sequential order within a thread keeps the sum reproducible
+        if (tid == 0) {
+          for (int i = 0; i < ni; ++i)
+            if (av.gas[i] != 0.0) v += yrow[rr * (ni + 1) + i];
+          ysum3[rr] = v;
+        }
+      } else if (tid == 0) {
+        ysum3[rr] = 1.0;
+      }
+    }
+  }
+  __syncthreads();
+
+  const double ysm = want_diff ? ysum3[0] : 0.0;
+  const double ys0 = want_diff ? ysum3[1] : 0.0;
+  const double ysp = want_diff ? ysum3[2] : 0.0;
+  DiffCoef ed = {0.0, 0.0, 0.0};
+  if (want_diff) ed = eddy_coef(z, nz, av.Kzz, av.dzi, av.vz, ysm, ys0, ysp);
+  const bool use_top = want_diff && av.flags[2] != 0.0;
+  const bool use_bot = want_diff && av.flags[3] != 0.0;
+  const double dtc = L.dt ? L.dt[c] : 1.0;
+  const double c0 = 1.0 / (GAMMA * dtc);
+
+  // per-species outputs: chem rhs (+ diffusion), diagonal/off-diagonal terms
+  for (int i = tid; i < ni; i += VK_NT) {
+#pragma clang fp contract(off)
+    double f = 0.0;
+    if (want_rhs) {
+      double loss = 0.0, prod = 0.0;
+      for (int e = net.loss_ptr[i]; e < net.loss_ptr[i + 1]; ++e)
+        loss = loss + net.loss_c[e] * rate[net.loss_item[e] / T];
+      for (int e = net.prod_ptr[i]; e < net.prod_ptr[i + 1]; ++e)
+        prod = prod + net.prod_c[e] * rate[net.prod_item[e] / T];
+      f = prod - loss;
+    }
+    const int64_t e_out = col.off_s + (int64_t)z * ni + i;
+    if (want_diff) {
+      const DiffCoef md = mol_coef(z, nz, ni, i, av, ysm, ys0, ysp);
+      const double At = ed.A + md.A, Bt = ed.B + md.B, Ct = ed.C + md.C;
+      const double yi = y0[i];
+      double diff;
+      if (z == 0)
+        diff = At * yi + Bt * yrow[2 * (ni + 1) + i];
+      else if (z == nz - 1)
+        diff = At * yi + Ct * yrow[i];
+      else
+        diff = At * yi + Bt * yrow[2 * (ni + 1) + i] + Ct * yrow[i];
+      if (z == nz - 1 && use_top) diff = diff + av.top_flux[i] / av.dzi[nz - 2];
+      if (z == 0 && use_bot) diff = diff + (av.bot_flux[i] - yi * av.bot_vdep[i]) / av.dzi[0];
+      const double bterm = (z == 0 && use_bot) ? (-av.bot_vdep[i] / av.dzi[0]) : 0.0;
+      if (want_jac) {
+        dd_s[i] = (L.mode == M_LHS && z == 0 && use_bot) ? At + bterm : At;
+        bt_s[i] = bterm;
+      }
+      const bool fixed = (L.fix != nullptr) && L.fix[e_out];
+      if (L.mode == M_LHS) {
+        const double tot = f + diff;
+        L.out0[e_out] = fixed ? 0.0 : tot;
+        if (z < nz - 1) {
+          const bool fx = fixed;
+          L.out_sup[col.off_s + (int64_t)z * ni + i] = fx ? 0.0 : -Bt;
+        }
+        if (z > 0) L.out_sub[col.off_s + (int64_t)(z - 1) * ni + i] = fixed ? 0.0 : -Ct;
+      } else if (L.mode == M_JY) {
+        if (z < nz - 1) L.out_sup[col.off_s + (int64_t)z * ni + i] = Bt;
+        if (z > 0) L.out_sub[col.off_s + (int64_t)(z - 1) * ni + i] = Ct;
+      } else if (L.mode == M_RHS2) {
+        const double tot = fixed ? 0.0 : (f + diff);
+        L.out0[e_out] = tot - (2.0 / (GAMMA * dtc)) * L.k1[e_out];
+      } else {  // M_FTOT
+        L.out0[e_out] = f + diff;
+      }
+    } else if (L.mode == M_CHEM_RHS) {
+      L.out0[e_out] = f;
+    }
+  }
+  if (!want_jac) return;
+  __syncthreads();
+  // Jacobian entries from the entry-major CSR (chem.py:187-208)
+  if (L.mode == M_LHS) {
+    // tile-layout D = c0 I - J - diag(dd), fix rows -> c0 e_i
+    constexpr int BLK = vk_blk<SL>();
+    double* dst = L.out_blk + col.off_b + (int64_t)z * BLK;
+    const int tr = tid >> 4, tc = tid & 15;
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+#pragma unroll
+      for (int t = 0; t < SL; ++t) {
+#pragma clang fp contract(off)
+        const int i = s * VK_G + tr, q = t * VK_G + tc;
+        double v = 0.0;
+        if (i < ni && q < ni) {
+          const int e = i * ni + q;
+          double J = 0.0;
+          for (int p = net.jac_ptr[e]; p < net.jac_ptr[e + 1]; ++p)
+            J = J + net.jac_c[p] * drate[net.jac_item[p]];
+          v = (i == q ? c0 : 0.0) - J;
+          if (i == q) v = v + (-dd_s[i]);
+          if (L.fix && L.fix[col.off_s + (int64_t)z * ni + i]) v = (i == q) ? c0 : 0.0;
+        }
+        dst[(s * SL + t) * VK_NT + tid] = v;
+      }
+    }
+  } else {
+    double* dst = L.out0 + col.off_s * ni + (int64_t)z * ni * ni;
+    for (int e = tid; e < ni * ni; e += VK_NT) {
+#pragma clang fp contract(off)
+      double J = 0.0;
+      for (int p = net.jac_ptr[e]; p < net.jac_ptr[e + 1]; ++p)
+        J = J + net.jac_c[p] * drate[net.jac_item[p]];
+      if (L.mode == M_JY) {
+        const int i = e / ni, q = e % ni;
+        if (i == q) {
+          J = J + dd_s[i];
+          if (z == 0 && use_bot) J = J + bt_s[i];
+        }
+      }
+      dst[e] = J;
+    }
+  }
+}
+
+int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max_nz, int max_S,
+                         int max_T, const VkNetDev* nets, const double* y, const double* M,
+                         const double* atm, const double* k, const double* dt, const uint8_t* fix,
+                         const double* k1, double* out0, double* out_blk, double* out_sup,
+                         double* out_sub, const int* active, hipStream_t stream) {
+  LayerArgs L{cols, ncols, nets, y, M, atm, k, dt, fix, k1, out0, out_blk, out_sup, out_sub,
+              active, mode};
+  const bool want_jac = (mode == M_CHEM_JAC || mode == M_LHS || mode == M_JY);
+  // ni+1 <= 16*sl+1 for the y rows
+  const size_t shmem =
+      sizeof(double) * ((size_t)max_S + (want_jac ? (size_t)max_S * max_T : 0) +
+                        3 * (size_t)(VK_G * (sl > 0 ? sl : 1) + 1));
+  if (shmem > 160 * 1024) {
+    vk_set_error("chem layer needs %zu bytes of LDS (S=%d, T=%d)", shmem, max_S, max_T);
+    return VK_ERR_UNSUPPORTED;
+  }
+  dim3 grid(max_nz, ncols);
+  switch (sl) {
+#define VK_CASE(n)                                                                        \
+  case n:                                                                                 \
+    hipFuncSetAttribute((const void*)k_chem_layer<n>,                                     \
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);          \
+    hipLaunchKernelGGL(k_chem_layer<n>, grid, dim3(VK_NT), shmem, stream, L);             \
+    break;
+    VK_CASE(1) VK_CASE(2) VK_CASE(3) VK_CASE(4) VK_CASE(5) VK_CASE(6) VK_CASE(7)
+#undef VK_CASE
+    default:
+      return VK_ERR_UNSUPPORTED;
+  }
+  VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}

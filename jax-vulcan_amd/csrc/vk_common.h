@@ -1,0 +1,148 @@
+// Shared definitions for the gfx950 Ros2 column kernels.
+//
+// Layout conventions (all FP64 unless noted, row-major):
+//   * a "column" is one atmosphere: nz layers x ni species;
+//   * per-column arrays live in packed batch buffers at offsets given by a
+//     VkCol descriptor, so one launch can carry columns of different nz;
+//   * dense ni x ni blocks (Jacobian blocks, block inverses) are stored in
+//     the "tile layout" of the 16x16 thread grid that factors them: element
+//     (i, j) with i = 16a + tr, j = 16b + tc lives at ((a*SL + b)*256 +
+//     tr*16 + tc), so every load/store of a register tile is a contiguous
+//     512-byte wave access.  SL = ceil(ni/16) is a template parameter.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vulcan_ros2.h"
+
+#define VK_NT 256          // threads per column workgroup (16 x 16 grid)
+#define VK_G 16            // grid edge
+
+// Status bits written per column (never a batch abort).
+#define VK_ST_SINGULAR 1   // zero pivot met in a block inverse
+#define VK_ST_NONFINITE 2  // non-finite value in the state after a step
+#define VK_ST_CONVERGED 4  // steady-state criterion met
+#define VK_ST_MAXSTEPS 8   // step cap reached
+#define VK_ST_DONE (VK_ST_CONVERGED | VK_ST_MAXSTEPS)
+
+// Offsets (in doubles) of the fields of one packed atmosphere record.
+struct VkAtmOff {
+  int64_t Kzz, dzi, vz, Hpi, Ti, Tco, g, M;
+  int64_t ms, alpha, top_flux, bot_flux, bot_vdep, gas;
+  int64_t Dzz, vm, vs, flags, total;
+};
+
+__host__ __device__ inline VkAtmOff vk_atm_offsets(int nz, int ni) {
+  VkAtmOff o;
+  int64_t p = 0;
+  const int64_t nzm = nz - 1;
+  o.Kzz = p; p += nzm;
+  o.dzi = p; p += nzm;
+  o.vz = p; p += nzm;
+  o.Hpi = p; p += nzm;
+  o.Ti = p; p += nzm;
+  o.Tco = p; p += nz;
+  o.g = p; p += nz;
+  o.M = p; p += nz;
+  o.ms = p; p += ni;
+  o.alpha = p; p += ni;
+  o.top_flux = p; p += ni;
+  o.bot_flux = p; p += ni;
+  o.bot_vdep = p; p += ni;
+  o.gas = p; p += ni;
+  o.Dzz = p; p += nzm * ni;
+  o.vm = p; p += (int64_t)nz * ni;
+  o.vs = p; p += nzm * ni;
+  o.flags = p; p += 4;
+  o.total = p;
+  return o;
+}
+
+// Device copy of one reaction network.  Index tables are int32; the
+// stoichiometric coefficients stay float64 exactly as in the reference
+// tables.  Three CSR structures give deterministic, atomic-free sums:
+//   loss/prod  : species-major lists of (rate row*T + slot) in ascending
+//                (row, slot) order, the order np.add.at / a sequential
+//                segment_sum visit them;
+//   jac        : entry-major lists over (out, in) pairs of
+//                (row*T + in_slot, signed out stoich) in ascending
+//                (row, out slot, in slot) order.
+struct VkNetDev {
+  int ni, S, T, pad;
+  const int* r_idx;      // (S*T) reactant species, PAD = ni
+  const double* r_st;    // (S*T)
+  const int* is3;        // (S) three-body flag
+  const int* loss_ptr;   // (ni+1)
+  const int* loss_item;  // row*T + slot
+  const double* loss_c;  // stoich
+  const int* prod_ptr;
+  const int* prod_item;
+  const double* prod_c;
+  const int* jac_ptr;    // (ni*ni+1), entry e = out*ni + in
+  const int* jac_item;   // row*T + in_slot
+  const double* jac_c;   // signed out stoich
+};
+
+// One column of a batch.
+struct VkCol {
+  int32_t nz, ni, S, net;
+  int64_t off_s;  // (nz, ni) state-shaped arrays
+  int64_t off_k;  // (S, nz) rate table
+  int64_t off_a;  // packed atmosphere record
+  int64_t off_b;  // (nz, SL*SL*256) block stacks
+};
+
+template <int SL>
+__host__ __device__ constexpr int vk_blk() { return SL * SL * VK_NT; }
+
+__host__ __device__ inline int vk_slots(int ni) { return (ni + VK_G - 1) / VK_G; }
+
+// Epilogue applied by the backward sweep to each element of x_j (all
+// pointers at off_s):
+//   kind 1: out1 = y + x / e0                         (Ros2 stage-1 yk2)
+//   kind 2: out1 = y + e0*k1 + e1*x, out2 = |out1 - yk2|  (Ros2 sol, delta)
+struct SolveEpi {
+  int kind;
+  double e0, e1;
+  const double* y;
+  const double* k1;
+  const double* yk2;
+  double* out1;
+  double* out2;
+};
+
+// Thread-local error text set by the host API.
+void vk_set_error(const char* fmt, ...);
+
+#define VK_CHECK_HIP(expr)                                                    \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess) {                                                   \
+      vk_set_error("%s failed: %s", #expr, hipGetErrorString(_e));           \
+      return VK_ERR_HIP;                                                      \
+    }                                                                         \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+
+// Sum over the 16 lanes that share a grid row (lanes tr*16 .. tr*16+15).
+__device__ inline double vk_rowsum16(double v) {
+  v += __shfl_xor(v, 8, 16);
+  v += __shfl_xor(v, 4, 16);
+  v += __shfl_xor(v, 2, 16);
+  v += __shfl_xor(v, 1, 16);
+  return v;
+}
+
+// Integer power for small non-negative integer exponents held as double.
+__device__ inline double vk_ipow(double y, double nu) {
+  int n = (int)nu;
+  double r = y;
+  if (n == 1) return y;
+  if (n == 2) return y * y;
+  if (n <= 0) return 1.0;
+  for (int i = 1; i < n; ++i) r = r * y;
+  return r;
+}

@@ -1,0 +1,87 @@
+"""ctypes binding of ``libvulcan_hip.so`` (the C ABI in include/vulcan_ros2.h).
+
+The library is required: importing a compute wrapper without it raises,
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libvulcan_hip.so"
+
+VK_OK = 0
+_ERRNAMES = {1: "bad argument", 2: "HIP error", 3: "out of memory", 4: "unsupported size"}
+
+p = C.c_void_p
+i32 = C.c_int
+i64 = C.c_int64
+sz = C.c_size_t
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "vk_last_error": (C.c_char_p, []),
+    "vk_version": (i32, []),
+    "vk_max_species": (i32, []),
+    "vk_network_create": (i32, [i32, i32, i32, p, p, p, p, p, C.POINTER(p)]),
+    "vk_network_destroy": (i32, [p]),
+    "vk_batch_create": (i32, [i32, p, p, p, p, p, p, p, p, p, i32, C.POINTER(p)]),
+    "vk_batch_uniform": (i32, [i32, i32, p, C.POINTER(p)]),
+    "vk_batch_destroy": (i32, [p]),
+    "vk_block_doubles": (i64, [i32]),
+    "vk_batch_block_doubles": (i64, [p]),
+    "vk_atm_record_layout": (i64, [i32, i32, p]),
+    "vk_chem_rhs_f64": (i32, [p, p, p, p, p, p, p]),
+    "vk_chem_jac_f64": (i32, [p, p, p, p, p, p, p]),
+    "vk_bt_factor_f64": (i32, [p, p, p, p, i32, p, p, p]),
+    "vk_bt_solve_f64": (i32, [p, p, p, p, p, p, p, p]),
+    "vk_bt_inverse_natural_f64": (i32, [p, p, p, p]),
+    "vk_ros2_workspace_bytes": (sz, [p]),
+    "vk_ros2_step_f64": (i32, [p, p, p, p, p, p, p, p, p, sz, p, p]),
+    "vk_residual_f64": (i32, [p, p, p, p, p, p]),
+    "vk_jacobian_blocks_f64": (i32, [p, p, p, p, p, p, p, p]),
+}
+
+_lib = None
+
+
+class VulcanError(RuntimeError):
+    """Raised when a C entry point returns a non-zero status."""
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def load():
+    """Load the library once; raise if it was never built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+            f"g.build()'` (hipcc, gfx950).  There is no CPU fallback.")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != VK_OK:
+        msg = load().vk_last_error().decode(errors="replace")
+        raise VulcanError(f"{what}: {_ERRNAMES.get(rc, rc)}: {msg}")
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a tensor (None for None)."""
+    return None if t is None else t.data_ptr()
+
+
+def stream_ptr(device=None) -> int:
+    import torch
+    return torch.cuda.current_stream(device).cuda_stream

@@ -1,0 +1,349 @@
+"""Seeded synthetic columns for the five Ros2 workloads.
+
+Every array is float64 / int64 in cgs units and row-major.  The generators
+follow the reference's setup code for geometry and table layout:
+
+* pressure interfaces: ``atm_setup.compute_pico`` (atm_setup.py:23-38);
+* heights, gravity, scale height, ``dzi``: ``compute_mu_dz_g`` with its two
+  sequential scans (atm_setup.py:253-375), reference pressure 1 bar;
+* reaction tables: forward record at an odd row, its reverse on the next even
+  row with reactants and products swapped, PAD = ni, duplicate species
+  collapsed into one slot with stoichiometry 2 (network.py:270-284, 366-419);
+* forward rate ``A * T**b * exp(-E/T)`` (rates.py:36-38) and reverse rate
+  ``k_f / K_eq`` (gibbs.py:147-150).
+
+The workloads (all FP64):
+
+1. one column, nz=100, ni=50, 300 reversible 2->2 reactions, dt=1e-6 s;
+2. one column, nz=150, ni=93, 1200 reversible reactions + 60 photolysis
+   branches, 2500 wavelength bins, two-stream photolysis;
+3. a batch of uneven columns, nz in U{60..200}, ni in {50, 69, 93},
+   13 ni reversible reactions (26 ni rate rows);
+4. gradients at a converged workload-2 state;
+5. a Poisson stream of workload-3-sized requests.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from ._types import AtmStatic, NetworkArrays
+
+# Physical constants (phy_const.py:3-9).
+KB = 1.38064852e-16
+NAVO = 6.02214086e23
+HC = 1.98644582e-9
+AU = 1.49597871e13
+R_SUN = 6.957e10
+R_JUP = 7.1492e9
+
+# Planet / star used for geometry and the stellar flux (vulcan_cfg.py:41-45,81).
+GS = 2140.0
+RP = 1.138 * R_JUP
+MU_MEAN = 2.3
+R_STAR = 0.805
+ORBIT_AU = 0.03142
+SL_ANGLE = 48.0 / 180.0 * 3.14159
+EDD = 0.5
+
+
+@dataclass
+class PhotoSetup:
+    """Cross sections and stellar flux for the photolysis branches.
+
+    Field meanings follow photo.PhotoData / PhotoJData (photo.py:15-32,
+    248-266); there are no temperature-dependent cross sections here.
+    """
+    bins: np.ndarray          # (nbin,) nm
+    dbin1: float
+    dbin2: float
+    din12_indx: int
+    sflux_top: np.ndarray     # (nbin,) erg cm-2 s-1 nm-1 at the top
+    absp_idx: np.ndarray      # (n_abs,) species index
+    absp_cross: np.ndarray    # (n_abs, nbin) cm2
+    scat_idx: np.ndarray      # (n_scat,)
+    scat_cross: np.ndarray    # (n_scat, nbin)
+    cross_J: np.ndarray       # (n_br, nbin)
+    branch_re_idx: np.ndarray  # (n_br,) k row written by each branch
+    branch_active: np.ndarray  # (n_br,) bool
+    mu_zenith: float = float(np.cos(SL_ANGLE))
+    edd: float = EDD
+    ag0: float = 0.0
+    hc: float = HC
+    f_diurnal: float = 1.0
+
+
+@dataclass
+class Column:
+    """One synthetic column: state, rates, atmosphere and network."""
+    name: str
+    nz: int
+    ni: int
+    y: np.ndarray             # (nz, ni) number densities cm-3
+    k: np.ndarray             # (nr+1, nz)
+    atm: AtmStatic
+    net: NetworkArrays
+    dt: float
+    P: np.ndarray             # (nz,) dyn cm-2
+    dz: np.ndarray            # (nz,)
+    photo: Optional[PhotoSetup] = None
+    meta: dict = field(default_factory=dict)
+
+
+# --------------------------------------------------------------------------
+# geometry
+# --------------------------------------------------------------------------
+
+def compute_pico(pco: np.ndarray) -> np.ndarray:
+    """Interface pressures from layer-centre pressures (atm_setup.py:23-38)."""
+    pco = np.asarray(pco, dtype=np.float64)
+    interior = np.sqrt(pco[1:] * pco[:-1])
+    bottom = pco[0] ** 1.5 * pco[1] ** -0.5
+    top = pco[-1] ** 1.5 * pco[-2] ** -0.5
+    return np.concatenate([[bottom], interior, [top]])
+
+
+def compute_geometry(pco: np.ndarray, Tco: np.ndarray, mu: float = MU_MEAN,
+                     gs: float = GS, Rp: float = RP):
+    """Hydrostatic heights, gravity and scale heights (atm_setup.py:253-375).
+
+    The reference layer is the interface closest to 1 bar; the column is
+    integrated upward from it and downward below it.
+    """
+    nz = Tco.shape[0]
+    pico = compute_pico(pco)
+    pref = int(np.argmin(np.abs(np.log10(pico) - 6.0)))
+    pref = min(pref, nz - 1)
+    gz = np.zeros(nz)
+    Hp = np.zeros(nz)
+    dz = np.zeros(nz)
+    zco = np.zeros(nz + 1)
+    z = 0.0
+    for n_i, i in enumerate(range(pref, nz)):
+        g_i = gs if n_i == 0 else gs * (Rp / (Rp + z)) ** 2
+        H_i = KB * Tco[i] / (mu / NAVO * g_i)
+        d_i = H_i * np.log(pico[i] / pico[i + 1])
+        z = z + d_i
+        gz[i], Hp[i], dz[i], zco[i + 1] = g_i, H_i, d_i, z
+    z = 0.0
+    for i in range(pref - 1, -1, -1):
+        g_i = gs * (Rp / (Rp + z)) ** 2
+        H_i = KB * Tco[i] / (mu / NAVO * g_i)
+        d_i = H_i * np.log(pico[i] / pico[i + 1])
+        z = z - d_i
+        gz[i], Hp[i], dz[i], zco[i] = g_i, H_i, d_i, z
+    dzi = 0.5 * (dz[1:] + dz[:-1])
+    Ti = 0.5 * (Tco[1:] + Tco[:-1])
+    Hpi = 0.5 * (Hp[1:] + Hp[:-1])
+    return dict(pico=pico, g=gz, Hp=Hp, dz=dz, dzi=dzi, zco=zco, Ti=Ti, Hpi=Hpi)
+
+
+def smooth_temperature(rng, P: np.ndarray, lo=500.0, hi=2000.0, n_modes=4):
+    """Smooth random T(z): low-order cosine modes in log P mapped to [lo, hi]."""
+    x = (np.log10(P) - np.log10(P).min()) / max(np.ptp(np.log10(P)), 1e-30)
+    prof = np.zeros_like(x)
+    for m in range(n_modes):
+        prof += rng.normal() / (m + 1) * np.cos(np.pi * m * x + rng.uniform(0, 2 * np.pi))
+    prof = (prof - prof.min()) / max(np.ptp(prof), 1e-30)
+    return lo + (hi - lo) * prof
+
+
+# --------------------------------------------------------------------------
+# network
+# --------------------------------------------------------------------------
+
+def _collapse(species):
+    out = {}
+    for sp in species:
+        out[sp] = out.get(sp, 0.0) + 1.0
+    return out
+
+
+def random_network(rng, ni: int, n_rev: int, n_photo: int = 0):
+    """Reversible 2->2 reactions plus optional A -> B + C photolysis rows.
+
+    Returns (net, photo_parent, photo_rows).  Forward reaction q sits at row
+    2q+1, its reverse at 2q+2; photolysis branches follow at rows
+    2*n_rev+1 ... (single rows, no reverse).
+    """
+    nr = 2 * n_rev + n_photo
+    T = 2
+    PAD = ni
+    r_idx = np.full((nr + 1, T), PAD, dtype=np.int64)
+    p_idx = np.full((nr + 1, T), PAD, dtype=np.int64)
+    r_st = np.zeros((nr + 1, T))
+    p_st = np.zeros((nr + 1, T))
+    for q in range(n_rev):
+        reac = _collapse(rng.integers(0, ni, size=2).tolist())
+        prod = _collapse(rng.integers(0, ni, size=2).tolist())
+        i, ir = 2 * q + 1, 2 * q + 2
+        for s, (sp, st) in enumerate(reac.items()):
+            r_idx[i, s], r_st[i, s] = sp, st
+            p_idx[ir, s], p_st[ir, s] = sp, st
+        for s, (sp, st) in enumerate(prod.items()):
+            p_idx[i, s], p_st[i, s] = sp, st
+            r_idx[ir, s], r_st[ir, s] = sp, st
+    parents = np.zeros(n_photo, dtype=np.int64)
+    rows = np.zeros(n_photo, dtype=np.int64)
+    for b in range(n_photo):
+        i = 2 * n_rev + 1 + b
+        a = int(rng.integers(0, ni))
+        prod = _collapse(rng.integers(0, ni, size=2).tolist())
+        r_idx[i, 0], r_st[i, 0] = a, 1.0
+        for s, (sp, st) in enumerate(prod.items()):
+            p_idx[i, s], p_st[i, s] = sp, st
+        parents[b], rows[b] = a, i
+    net = NetworkArrays(ni=ni, nr=nr, reactant_idx=r_idx, product_idx=p_idx,
+                        reactant_stoich=r_st, product_stoich=p_st,
+                        is_three_body=np.zeros(nr + 1, dtype=bool))
+    return net, parents, rows
+
+
+def random_rates(rng, Tco: np.ndarray, n_rev: int, n_photo: int = 0):
+    """k (nr+1, nz): forward A*T^b*exp(-E/T), reverse k_f / 10**U[-5,5]."""
+    nz = Tco.shape[0]
+    nr = 2 * n_rev + n_photo
+    k = np.zeros((nr + 1, nz))
+    A = 10.0 ** rng.uniform(-14.0, -9.0, size=n_rev)
+    b = rng.uniform(-2.0, 2.0, size=n_rev)
+    E = rng.uniform(0.0, 2.0e4, size=n_rev)
+    logK = rng.uniform(-5.0, 5.0, size=n_rev)
+    for q in range(n_rev):
+        kf = A[q] * Tco ** b[q] * np.exp(-E[q] / Tco)
+        k[2 * q + 1] = kf
+        k[2 * q + 2] = kf / 10.0 ** logK[q]
+    return k
+
+
+# --------------------------------------------------------------------------
+# photolysis setup
+# --------------------------------------------------------------------------
+
+def wavelength_grid(nbin: int = 2500, lo: float = 10.0, hi: float = 700.0,
+                    n1: Optional[int] = None):
+    """Two-resolution grid: n1 bins at dbin1 from lo, the rest at dbin2 to hi.
+
+    The J trapezoid is split at din12_indx = n1 as in photo.py:309-330.
+    """
+    if n1 is None:
+        n1 = int(round(0.8 * nbin))
+    dbin1 = 0.2 * (2500.0 / nbin)
+    seg1 = lo + dbin1 * np.arange(n1)
+    start2 = seg1[-1] + dbin1
+    dbin2 = (hi - start2) / (nbin - n1 - 1)
+    seg2 = start2 + dbin2 * np.arange(nbin - n1)
+    return np.concatenate([seg1, seg2]), dbin1, dbin2, n1
+
+
+def smooth_log_cross(rng, n: int, nbin: int, lo=-24.0, hi=-17.0, width=20):
+    """Cross sections log-uniform in [10^lo, 10^hi], Gaussian-smoothed in log."""
+    raw = rng.uniform(lo, hi, size=(n, nbin + 6 * width))
+    x = np.arange(-3 * width, 3 * width + 1)
+    ker = np.exp(-0.5 * (x / width) ** 2)
+    ker /= ker.sum()
+    sm = np.stack([np.convolve(r, ker, mode="valid")[:nbin] for r in raw])
+    # smoothing narrows the spread; stretch back to the requested range
+    mn = sm.min(axis=1, keepdims=True)
+    mx = sm.max(axis=1, keepdims=True)
+    sm = lo + (hi - lo) * (sm - mn) / np.maximum(mx - mn, 1e-30)
+    return 10.0 ** sm
+
+
+def blackbody_flux(bins_nm: np.ndarray, T_star=5800.0, r_star=R_STAR,
+                   orbit_au=ORBIT_AU):
+    """pi * B_lambda(T) diluted by (R*/a)^2, erg cm-2 s-1 nm-1."""
+    h = 6.62607015e-27
+    c = 2.99792458e10
+    lam = bins_nm * 1e-7
+    B = 2 * h * c ** 2 / lam ** 5 / np.expm1(h * c / (lam * KB * T_star))
+    return np.pi * B * 1e-7 * (r_star * R_SUN / (orbit_au * AU)) ** 2
+
+
+def random_photo(rng, ni, parents, rows, nbin=2500):
+    bins, dbin1, dbin2, din12 = wavelength_grid(nbin)
+    n_br = parents.shape[0]
+    cross_J = smooth_log_cross(rng, n_br, nbin)
+    absp_sp = np.unique(parents)
+    absp_cross = np.stack([cross_J[parents == sp].sum(axis=0) for sp in absp_sp])
+    scat_sp = np.array([s for s in range(ni) if s not in set(absp_sp.tolist())][:2],
+                       dtype=np.int64)
+    if scat_sp.size == 0:
+        scat_sp = np.array([0], dtype=np.int64)
+    sig0 = 10.0 ** rng.uniform(-23.0, -21.0, size=scat_sp.size)
+    scat_cross = sig0[:, None] * (bins[None, :] / 100.0) ** -4
+    return PhotoSetup(
+        bins=bins, dbin1=dbin1, dbin2=dbin2, din12_indx=din12,
+        sflux_top=blackbody_flux(bins),
+        absp_idx=absp_sp.astype(np.int64), absp_cross=absp_cross,
+        scat_idx=scat_sp, scat_cross=scat_cross,
+        cross_J=cross_J, branch_re_idx=rows.astype(np.int64),
+        branch_active=np.ones(n_br, dtype=bool),
+    )
+
+
+# --------------------------------------------------------------------------
+# columns
+# --------------------------------------------------------------------------
+
+def make_column(seed: int, nz: int, ni: int, n_rev: int, n_photo: int = 0,
+                nbin: int = 2500, dt: float = 1e-6, name: str = "column",
+                p_bot: float = 1e9, p_top: float = 1e-2) -> Column:
+    """Draw one column with the workload-1 distributions."""
+    rng = np.random.default_rng(seed)
+    P = np.logspace(np.log10(p_bot), np.log10(p_top), nz)
+    Tco = smooth_temperature(rng, P)
+    geo = compute_geometry(P, Tco)
+    M = P / (KB * Tco)
+    ymix = 10.0 ** rng.uniform(-20.0, 0.0, size=(nz, ni))
+    ymix /= ymix.sum(axis=1, keepdims=True)
+    y = M[:, None] * ymix
+    net, parents, rows = random_network(rng, ni, n_rev, n_photo)
+    k = random_rates(rng, Tco, n_rev, n_photo)
+    Kzz = 10.0 ** rng.uniform(5.0, 10.0, size=nz - 1)
+    atm = AtmStatic(
+        Kzz=Kzz, Dzz=np.zeros((nz - 1, ni)), dzi=geo["dzi"], vz=np.zeros(nz - 1),
+        Hpi=geo["Hpi"], Ti=geo["Ti"], Tco=Tco, g=geo["g"],
+        ms=np.full(ni, MU_MEAN), alpha=np.zeros(ni), M=M,
+        vm=np.zeros((nz, ni)), vs=np.zeros((nz - 1, ni)),
+        top_flux=np.zeros(ni), bot_flux=np.zeros(ni), bot_vdep=np.zeros(ni),
+        gas_indx_mask=np.ones(ni, dtype=bool),
+    )
+    photo = random_photo(rng, ni, parents, rows, nbin) if n_photo > 0 else None
+    return Column(name=name, nz=nz, ni=ni, y=y, k=k, atm=atm, net=net, dt=dt,
+                  P=P, dz=geo["dz"], photo=photo,
+                  meta=dict(seed=seed, n_rev=n_rev, n_photo=n_photo))
+
+
+def config1(seed: int = 1001) -> Column:
+    """Plain single Ros2 step: nz=100, ni=50, 300 reversible reactions."""
+    return make_column(seed, nz=100, ni=50, n_rev=300, dt=1e-6, name="cfg1")
+
+
+def config2(seed: int = 1002, nbin: int = 2500) -> Column:
+    """Plain steady state: nz=150, ni=93, 1200 reversible + 60 photo branches."""
+    return make_column(seed, nz=150, ni=93, n_rev=1200, n_photo=60, nbin=nbin,
+                       dt=1e-10, name="cfg2")
+
+
+def config3_sizes(seed: int = 1003, ncols: int = 512):
+    """(nz, ni, n_rev) for the uneven batch: nz~U{60..200}, ni in {50,69,93}."""
+    rng = np.random.default_rng(seed)
+    nz = rng.integers(60, 201, size=ncols)
+    ni = rng.choice(np.array([50, 69, 93]), size=ncols)
+    n_rev = 13 * ni
+    return [(int(a), int(b), int(c)) for a, b, c in zip(nz, ni, n_rev)]
+
+
+def config3(seed: int = 1003, ncols: int = 512, dt: float = 1e-10):
+    """Uneven batch of independently drawn columns."""
+    sizes = config3_sizes(seed, ncols)
+    return [make_column(seed * 100003 + c, nz=a, ni=b, n_rev=r, dt=dt,
+                        name=f"cfg3_{c}") for c, (a, b, r) in enumerate(sizes)]
+
+
+def poisson_arrivals(seed: int = 1005, n: int = 10000, lam: float = 200.0):
+    """Arrival times (s) of a Poisson stream with rate lam."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / lam, size=n))

@@ -1,0 +1,463 @@
+"""Pins of the CPU oracle against closed forms, known answers and dense solves.
+
+Each test is chosen so that a plausible slip in the restatement (a dropped
+term, a wrong sign, a transposed operand, a wrong boundary index) fails it.
+Reference test / file each pin follows is named in the docstring.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import ros2_np as O
+from vulcan_amd import synth
+from vulcan_amd._types import AtmStatic, NetworkArrays
+
+GAMMA = O.GAMMA
+
+
+# ---------------------------------------------------------------------------
+# block Thomas
+# ---------------------------------------------------------------------------
+
+def test_block_thomas_dense_blocks_seed42():
+    """solver.block_thomas vs np.linalg.solve (test_block_thomas.py:22-50)."""
+    rng = np.random.default_rng(42)
+    nz, ni = 8, 5
+    diag = rng.standard_normal((nz, ni, ni)) + 10.0 * np.eye(ni)
+    sup = rng.standard_normal((nz - 1, ni, ni))
+    sub = rng.standard_normal((nz - 1, ni, ni))
+    rhs = rng.standard_normal((nz, ni))
+    x = O.block_thomas_dense(diag, sup, sub, rhs)
+    ref = np.linalg.solve(O.assemble_dense(diag, sup, sub), rhs.reshape(-1)).reshape(nz, ni)
+    assert np.max(np.abs(x - ref) / np.maximum(np.abs(ref), 1e-12)) < 1e-9
+
+
+def test_block_thomas_diag_seed7():
+    """diag-offdiag variant vs dense (test_block_thomas_diag.py:40-67)."""
+    rng = np.random.default_rng(7)
+    nz, ni = 8, 5
+    diag = rng.standard_normal((nz, ni, ni)) + 10.0 * np.eye(ni)
+    sup = rng.standard_normal((nz - 1, ni))
+    sub = rng.standard_normal((nz - 1, ni))
+    rhs = rng.standard_normal((nz, ni))
+    x = O.block_thomas_diag_offdiag(diag, sup, sub, rhs)
+    ref = np.linalg.solve(O.assemble_dense(diag, sup, sub), rhs.reshape(-1)).reshape(nz, ni)
+    assert np.max(np.abs(x - ref) / np.maximum(np.abs(ref), 1e-12)) < 1e-12
+    # the same system through the dense-off-block path
+    x2 = O.block_thomas_dense(diag, np.stack([np.diag(v) for v in sup]),
+                              np.stack([np.diag(v) for v in sub]), rhs)
+    assert np.max(np.abs(x - x2) / np.maximum(np.abs(x2), 1e-12)) < 1e-12
+
+
+def test_block_thomas_transposed_swaps_sup_sub():
+    """The adjoint's transposed solve (steady_state_grad.py:316-317)."""
+    rng = np.random.default_rng(1)
+    nz, ni = 6, 4
+    diag = rng.standard_normal((nz, ni, ni)) + 6.0 * np.eye(ni)
+    sup = rng.standard_normal((nz - 1, ni))
+    sub = rng.standard_normal((nz - 1, ni))
+    v = rng.standard_normal((nz, ni))
+    lam = O.block_thomas_diag_offdiag(np.transpose(diag, (0, 2, 1)), sub, sup, v)
+    A = O.assemble_dense(diag, sup, sub)
+    ref = np.linalg.solve(A.T, v.reshape(-1)).reshape(nz, ni)
+    assert np.max(np.abs(lam - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+# ---------------------------------------------------------------------------
+# chemistry
+# ---------------------------------------------------------------------------
+
+def test_chem_rhs_matches_loop_restatement(cfg1):
+    """Vectorised rhs vs the per-reaction loop of chem_rhs_numpy (chem.py:217-243)."""
+    c = cfg1
+    f = O.chem_rhs(c.y, c.atm.M, c.k, c.net)
+    f2 = O.chem_rhs_loop(c.y, c.atm.M, c.k, c.net)
+    rate, _, _ = O._rates(c.y, c.atm.M, c.k, c.net)
+    scale = np.abs(rate).max(axis=1, keepdims=True)  # cancellation scale per layer
+    assert np.max(np.abs(f - f2) / scale) < 1e-13
+
+
+def test_chem_rhs_conserves_molecules_for_2to2(cfg1):
+    """Every row is 2 -> 2: sum_i rhs[z, i] = 0 up to roundoff of the terms."""
+    c = cfg1
+    f = O.chem_rhs(c.y, c.atm.M, c.k, c.net)
+    rate, _, _ = O._rates(c.y, c.atm.M, c.k, c.net)
+    terms = 4.0 * np.abs(rate).sum(axis=1)
+    assert np.all(np.abs(f.sum(axis=1)) <= 1e-13 * terms)
+
+
+def _tiny_network():
+    """A + B <-> C + C, C -> D (single slot reactant), D + A <-> B + B."""
+    ni, T = 4, 2
+    PAD = ni
+    r_idx = np.full((6, T), PAD, np.int64)
+    p_idx = np.full((6, T), PAD, np.int64)
+    r_st = np.zeros((6, T))
+    p_st = np.zeros((6, T))
+    rows = [([(0, 1), (1, 1)], [(2, 2)]), ([(2, 2)], [(0, 1), (1, 1)]),
+            ([(2, 1)], [(3, 1)]), ([(3, 1)], [(2, 1)]),
+            ([(3, 1), (0, 1)], [(1, 2)])]
+    for r, (re, pr) in enumerate(rows, start=1):
+        for s, (sp, st) in enumerate(re):
+            r_idx[r, s], r_st[r, s] = sp, st
+        for s, (sp, st) in enumerate(pr):
+            p_idx[r, s], p_st[r, s] = sp, st
+    is3 = np.zeros(6, bool)
+    is3[5] = True
+    return NetworkArrays(ni, 5, r_idx, p_idx, r_st, p_st, is3)
+
+
+def test_chem_jac_matches_finite_differences():
+    """Analytical Jacobian (chem.py:144-214) vs central differences of chem_rhs."""
+    net = _tiny_network()
+    rng = np.random.default_rng(3)
+    nz = 3
+    y = rng.uniform(0.5, 2.0, size=(nz, net.ni))
+    M = rng.uniform(1.0, 3.0, size=nz)
+    k = rng.uniform(0.1, 1.0, size=(net.nr + 1, nz))
+    J = O.chem_jac(y, M, k, net)
+    Jfd = O.chem_jac_fd(y, M, k, net, h=1e-6)
+    assert np.max(np.abs(J - Jfd)) < 1e-8 * np.max(np.abs(J))
+
+
+def test_chem_rhs_brute_force_tiny():
+    """Hand-expanded mass-action rhs of the tiny network at one layer."""
+    net = _tiny_network()
+    y = np.array([[1.5, 0.7, 2.0, 0.3]])
+    M = np.array([2.0])
+    k = np.array([[0.0], [0.4], [0.2], [0.9], [0.05], [0.3]])
+    A, B, Cc, D = y[0]
+    r1 = 0.4 * A * B
+    r2 = 0.2 * Cc * Cc
+    r3 = 0.9 * Cc
+    r4 = 0.05 * D
+    r5 = 0.3 * D * A * 2.0
+    want = np.array([-r1 + r2 - r5, -r1 + r2 + 2 * r5, 2 * r1 - 2 * r2 - r3 + r4, r3 - r4 - r5])
+    got = O.chem_rhs(y, M, k, net)[0]
+    np.testing.assert_allclose(got, want, rtol=1e-14)
+
+
+def test_chem_jac_euler_identity(cfg1):
+    """Bimolecular rates are homogeneous of degree 2 in y, so J(y) y = 2 f(y).
+
+    Checked per layer against the rate scale (the sums cancel heavily)."""
+    c = cfg1
+    J = O.chem_jac(c.y, c.atm.M, c.k, c.net)
+    f = O.chem_rhs(c.y, c.atm.M, c.k, c.net)
+    rate, _, _ = O._rates(c.y, c.atm.M, c.k, c.net)
+    scale = 8.0 * np.abs(rate).sum(axis=1, keepdims=True)
+    Jy = np.einsum("zij,zj->zi", J, c.y)
+    assert np.max(np.abs(Jy - 2.0 * f) / scale) < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# diffusion
+# ---------------------------------------------------------------------------
+
+def _col(nz=12, ni=3, seed=0, **kw):
+    rng = np.random.default_rng(seed)
+    P = np.logspace(9, 2, nz)
+    T = np.linspace(1500, 800, nz)
+    geo = synth.compute_geometry(P, T)
+    M = P / (synth.KB * T)
+    atm = AtmStatic(
+        Kzz=10 ** rng.uniform(5, 9, nz - 1), Dzz=kw.get("Dzz", np.zeros((nz - 1, ni))),
+        dzi=geo["dzi"], vz=kw.get("vz", np.zeros(nz - 1)), Hpi=geo["Hpi"], Ti=geo["Ti"], Tco=T,
+        g=geo["g"], ms=rng.uniform(2, 40, ni), alpha=kw.get("alpha", np.zeros(ni)), M=M,
+        vm=np.zeros((nz, ni)), vs=np.zeros((nz - 1, ni)), top_flux=np.zeros(ni),
+        bot_flux=np.zeros(ni), bot_vdep=np.zeros(ni), gas_indx_mask=np.ones(ni, bool))
+    ymix = rng.uniform(0.1, 1.0, (nz, ni))
+    ymix /= ymix.sum(1, keepdims=True)
+    return atm, M[:, None] * ymix, geo
+
+
+def _diff(y, atm):
+    g = O.compute_diff_grav(atm)
+    A, B, C, Ai, Bi, Ci, _ = O.build_diff_coeffs(y, atm, g)
+    return O.apply_diffusion(y, A, B, C, Ai, Bi, Ci, atm)
+
+
+def test_eddy_diffusion_column_integral_is_zero():
+    """Flux form: dzi_0 d_0 + sum dz_ave_j d_j + dzi_top d_top = 0 (zero-flux BCs)."""
+    atm, y, geo = _col()
+    d = _diff(y, atm)
+    dzi = atm.dzi
+    w = np.concatenate([[dzi[0]], 0.5 * (dzi[:-1] + dzi[1:]), [dzi[-1]]])
+    total = np.abs(w[:, None] * d).sum(axis=0)
+    assert np.all(np.abs((w[:, None] * d).sum(axis=0)) <= 1e-12 * total)
+
+
+def test_eddy_diffusion_vanishes_for_well_mixed_column():
+    """y_z = n_z x with uniform mixing ratios x: A + B + C = 0 => diff = 0."""
+    atm, y, geo = _col()
+    x = np.array([0.2, 0.3, 0.5])
+    y = atm.M[:, None] * x[None]
+    d = _diff(y, atm)
+    assert np.max(np.abs(d)) < 1e-12 * np.max(np.abs(y) * np.max(atm.Kzz / atm.dzi ** 2))
+
+
+def test_molecular_diffusion_flux_form_and_sign():
+    """Dzz > 0 with gravity: the column integral vanishes for uniform g (the
+    reference's boundary rows use g[0] / g[-1] for both neighbours, so the
+    sum telescopes exactly only then), and a heavy species in a well-mixed
+    column is pushed down (d[0] > 0 at the bottom)."""
+    nz, ni = 12, 3
+    atm, y, geo = _col(nz, ni, Dzz=np.full((nz - 1, ni), 1e7))
+    atm = atm._replace(g=np.full(nz, 2140.0))
+    d = _diff(y, atm)
+    w = np.concatenate([[atm.dzi[0]], 0.5 * (atm.dzi[:-1] + atm.dzi[1:]), [atm.dzi[-1]]])
+    total = np.abs(w[:, None] * d).sum(axis=0)
+    assert np.all(np.abs((w[:, None] * d).sum(axis=0)) <= 1e-10 * total)
+    x = np.array([0.2, 0.3, 0.5])
+    yw = atm.M[:, None] * x[None]
+    dw = _diff(yw, atm)
+    heavy = int(np.argmax(atm.ms))
+    assert dw[0, heavy] > 0
+
+
+# ---------------------------------------------------------------------------
+# Ros2 step
+# ---------------------------------------------------------------------------
+
+def _linear_column(nz=5, ni=4, seed=2):
+    """First-order isomerisations A_i -> A_j; no transport: f(y) = J y exactly."""
+    rng = np.random.default_rng(seed)
+    pairs = [(i, j) for i in range(ni) for j in range(ni) if i != j][: 2 * ni]
+    nr = len(pairs)
+    r_idx = np.full((nr + 1, 1), ni, np.int64)
+    p_idx = np.full((nr + 1, 1), ni, np.int64)
+    r_st = np.zeros((nr + 1, 1))
+    p_st = np.zeros((nr + 1, 1))
+    for r, (i, j) in enumerate(pairs, start=1):
+        r_idx[r, 0], r_st[r, 0], p_idx[r, 0], p_st[r, 0] = i, 1.0, j, 1.0
+    net = NetworkArrays(ni, nr, r_idx, p_idx, r_st, p_st, np.zeros(nr + 1, bool))
+    k = np.zeros((nr + 1, nz))
+    k[1:] = rng.uniform(0.1, 10.0, size=(nr, nz))
+    atm, _, _ = _col(nz, ni, seed)
+    atm = atm._replace(Kzz=np.zeros(nz - 1))
+    y = rng.uniform(1.0, 2.0, size=(nz, ni))
+    return y, k, atm, net
+
+
+@pytest.mark.parametrize("dt", [1e-3, 0.1, 10.0])
+def test_ros2_matches_stability_function(dt):
+    """sol = (I - g hJ)^-2 (I + (1-2g) hJ) y  for linear f; from R(z) of Ros2."""
+    y, k, atm, net = _linear_column()
+    sol, delta = O.ros2_step(y, k, dt, atm, net)
+    J = O.chem_jac(y, atm.M, k, net)
+    I = np.eye(y.shape[1])
+    for z in range(y.shape[0]):
+        A = I - GAMMA * dt * J[z]
+        want = np.linalg.solve(A, np.linalg.solve(A, (I + (1 - 2 * GAMMA) * dt * J[z]) @ y[z]))
+        np.testing.assert_allclose(sol[z], want, rtol=1e-12)
+        yk2 = np.linalg.solve(A, (I + (1 - GAMMA) * dt * J[z]) @ y[z])
+        np.testing.assert_allclose(delta[z], np.abs(want - yk2), rtol=1e-9, atol=1e-12)
+
+
+def test_ros2_second_order_convergence():
+    """Global error after a fixed span falls ~4x when dt halves (2nd order)."""
+    y, k, atm, net = _linear_column()
+    J = O.chem_jac(y, atm.M, k, net)
+    from scipy.linalg import expm
+    span = 0.05
+    errs = []
+    for n in (32, 64, 128):
+        yy = y
+        for _ in range(n):
+            yy, _ = O.ros2_step(yy, k, span / n, atm, net)
+        exact = np.stack([expm(span * J[z]) @ y[z] for z in range(y.shape[0])])
+        errs.append(np.max(np.abs(yy - exact)))
+    assert 3.5 < errs[0] / errs[1] < 4.5 and 3.5 < errs[1] / errs[2] < 4.5
+
+
+def test_ros2_fix_mask_pins_cells(cfg1):
+    """Fixed cells keep y (jax_step.py:292-308): their rows of W are c0 e_i and
+    their rhs is 0, so k1 = k2 = 0 there.  With dt small enough that c0
+    dominates, the solve reproduces that to roundoff of the layer scale."""
+    c = cfg1
+    fix = np.zeros((c.nz, c.ni), bool)
+    fix[:, 3] = True
+    sol, _ = O.ros2_step(c.y, c.k, 1e-16, c.atm, c.net, fix_mask=fix)
+    scale = np.abs(c.y).max(axis=1)
+    assert np.all(np.abs(sol[:, 3] - c.y[:, 3]) <= 1e-14 * scale)
+    free, _ = O.ros2_step(c.y, c.k, 1e-16, c.atm, c.net)
+    assert np.max(np.abs(free[:, 3] - c.y[:, 3]) / scale) > 1e-12
+
+
+# ---------------------------------------------------------------------------
+# step control
+# ---------------------------------------------------------------------------
+
+def test_aggregate_delta_known_answer():
+    """test_outer_loop_regressions.py:19-35: the mask uses the pre-step ymix."""
+    got = O.aggregate_delta(np.array([[1.0, 1.0]]), np.array([[0.25, 0.50]]),
+                            np.array([[0.10, 0.10]]), mtol=1e-2, atol=0.0)
+    assert got == 0.50
+
+
+def test_step_size_closed_form():
+    """h = clip(0.9 sqrt(rtol/delta), 0.5, 2); delta = 0 -> 0.01 rtol (outer_loop.py:181-193)."""
+    assert O.step_size(1.0, 0.2, 0.2, 0.5, 2.0, 1e-14, 1e9) == pytest.approx(0.9)
+    assert O.step_size(1.0, 0.0, 0.2, 0.5, 2.0, 1e-14, 1e9) == 2.0
+    assert O.step_size(1.0, 100.0, 0.2, 0.5, 2.0, 1e-14, 1e9) == 0.5
+    assert O.step_size(1e-15, 1e-3, 0.2, 0.5, 2.0, 1e-14, 1e9) == 1e-14
+
+
+def test_control_reject_then_force_accept():
+    """delta > rtol rejects and halves dt; dt underflow forces acceptance."""
+    nz, ni = 3, 2
+    y = np.ones((nz, ni))
+    compo = np.ones((ni, 1))
+    s, ini = O.init_control(y, 1e-3, compo)
+    p = O.StepParams(rtol=0.2)
+    sol = y * 1.5
+    delta = np.full((nz, ni), 0.9)
+    s2, d = O.control_update(s, sol, delta, y.sum(1), compo, ini, p)
+    assert not d["do_accept"] and s2.dt == pytest.approx(0.5e-3) and s2.delta_count == 1
+    s3 = s._replace(dt=1.5e-14)
+    s4, d = O.control_update(s3, sol, delta, y.sum(1), compo, ini, p)
+    assert d["force"] and s4.t == p.dt_min
+
+
+# ---------------------------------------------------------------------------
+# photolysis
+# ---------------------------------------------------------------------------
+
+def test_J_known_answer_photo_ion():
+    """test_photo_ion.py:20-41: trapezoid split at din12 vs its _manual_jion."""
+    aflux = np.array([[1.0, 2.0, 4.0, 8.0, 16.0], [0.5, 1.0, 1.5, 2.0, 2.5]])
+    cross = np.array([[1.0, 0.8, 0.6, 0.4, 0.2], [0.3, 0.2, 0.1, 0.05, 0.02],
+                      [0.0, 0.1, 0.2, 0.3, 0.4]])
+    split, d1, d2 = 3, 0.1, 1.5
+
+    def manual(cr):
+        left = np.sum(aflux[:, :split] * cr[:split], axis=1) * d1
+        left -= 0.5 * (aflux[:, 0] * cr[0] + aflux[:, split - 1] * cr[split - 1]) * d1
+        right = np.sum(aflux[:, split:] * cr[split:], axis=1) * d2
+        right -= 0.5 * (aflux[:, split] * cr[split] + aflux[:, -1] * cr[-1]) * d2
+        return left + right
+
+    J = O.compute_J(aflux, cross, split, d1, d2)
+    for b in range(3):
+        np.testing.assert_allclose(J[b], manual(cross[b]), rtol=1e-14)
+    k = O.update_k_with_J(np.zeros((10, 2)), J, np.array([7, 8, 9]), np.ones(3, bool), 1.0)
+    np.testing.assert_array_equal(k[7:10], J)
+
+
+def test_tau_closed_form():
+    """Constant n, sigma, dz: tau_j = (nz - j) n sigma dz, tau_nz = 0 (photo.py:84-123)."""
+    nz, ni, nbin = 7, 3, 5
+    y = np.full((nz, ni), 2.0e10)
+    dz = np.full(nz, 3.0e5)
+    sig = np.full((1, nbin), 1e-20)
+    tau = O.compute_tau(y, dz, np.array([1]), sig, np.zeros(0, int), np.zeros((0, nbin)))
+    j = np.arange(nz + 1)
+    np.testing.assert_allclose(tau[:, 0], (nz - j) * 2.0e10 * 1e-20 * 3.0e5, rtol=1e-14)
+
+
+def test_flux_closed_form_without_scattering():
+    """w0 = 0: diffuse terms vanish; aflux = mean direct beam / (hc/lambda)."""
+    nz, ni, nbin = 6, 2, 4
+    rng = np.random.default_rng(0)
+    y = rng.uniform(1e9, 1e11, (nz, ni))
+    ymix = y / y.sum(1, keepdims=True)
+    dz = np.full(nz, 1e5)
+    sig = rng.uniform(1e-21, 1e-19, (1, nbin))
+    tau = O.compute_tau(y, dz, np.array([0]), sig, np.zeros(0, int), np.zeros((0, nbin)))
+    bins = np.linspace(100, 200, nbin)
+    F = np.full(nbin, 3.0)
+    mu = np.cos(48 / 180 * np.pi)
+    af, sf, dd, du = O.compute_flux(tau, F, ymix, np.array([0]), sig, np.zeros(0, int),
+                                    np.zeros((0, nbin)), bins, mu, 0.5, 0.0, synth.HC,
+                                    np.zeros((nz + 1, nbin)))
+    want = 0.5 * F * (np.exp(-tau[:-1] / mu) + np.exp(-tau[1:] / mu)) / (synth.HC / bins)
+    np.testing.assert_allclose(af, want, rtol=1e-13)
+    assert np.all(dd == 0) and np.all(du == 0)
+
+
+def test_flux_two_stream_single_layer_energy_balance():
+    """Conservative scattering (w0 -> 1) in one layer: what enters leaves.
+
+    For w0 = 1 - 1e-8 nothing is absorbed, so the net downward flux at the
+    bottom (direct + diffuse down - diffuse up) equals the incoming flux."""
+    nz, nbin = 1, 1
+    mu = 0.7
+    tau = np.array([[0.3], [0.0]])
+    ymix = np.array([[1.0]])
+    af, sf, dd, du = O.compute_flux(tau, np.array([1.0]), ymix, np.zeros(0, int),
+                                    np.zeros((0, nbin)), np.array([0]), np.ones((1, nbin)),
+                                    np.array([100.0]), mu, 0.5, 0.0, synth.HC,
+                                    np.zeros((2, nbin)))
+    top_in = 1.0 * mu
+    bottom_net = sf[0, 0] * mu + dd[0, 0]
+    assert np.isfinite(af).all() and bottom_net <= top_in * (1 + 1e-9)
+
+
+# ---------------------------------------------------------------------------
+# adjoint
+# ---------------------------------------------------------------------------
+
+def _open_chain():
+    """test_steady_state_grad.py:44-132: 0 -> A, A -> B, B -> 0 with weak diffusion."""
+    ni, nr, PAD = 2, 6, 2
+    r_idx = np.full((nr + 1, 1), PAD, np.int64)
+    p_idx = np.full((nr + 1, 1), PAD, np.int64)
+    r_st = np.zeros((nr + 1, 1))
+    p_st = np.zeros((nr + 1, 1))
+    p_idx[1, 0], p_st[1, 0] = 0, 1.0
+    r_idx[3, 0], r_st[3, 0], p_idx[3, 0], p_st[3, 0] = 0, 1.0, 1, 1.0
+    r_idx[5, 0], r_st[5, 0] = 1, 1.0
+    net = NetworkArrays(ni, nr, r_idx, p_idx, r_st, p_st, np.zeros(nr + 1, bool))
+    nz = 4
+    atm = AtmStatic(Kzz=np.full(nz - 1, 1e-2), Dzz=np.full((nz - 1, ni), 1e-3),
+                    dzi=np.ones(nz - 1), vz=np.zeros(nz - 1), Hpi=np.ones(nz - 1),
+                    Ti=np.full(nz - 1, 1500.0), Tco=np.full(nz, 1500.0), g=np.ones(nz),
+                    ms=np.ones(ni), alpha=np.zeros(ni), M=np.ones(nz), vm=np.zeros((nz, ni)),
+                    vs=np.zeros((nz - 1, ni)), top_flux=np.zeros(ni), bot_flux=np.zeros(ni),
+                    bot_vdep=np.zeros(ni), gas_indx_mask=np.ones(ni, bool))
+    k = np.zeros((nr + 1, nz))
+    k[1], k[3], k[5] = 1.0, 0.5, 0.3
+    y0 = np.zeros((nz, ni))
+    y0[:, 0], y0[:, 1] = 2.0, 3.3
+    return y0, k, atm, net
+
+
+def test_open_chain_steady_state_closed_form():
+    """Well-mixed limit [A] = k1/k3, [B] = k1/k5 (within the weak-diffusion shift)."""
+    y0, k, atm, net = _open_chain()
+    ys, it, res = O.newton_steady_state(y0, k, atm, net)
+    assert res < 1e-9
+    np.testing.assert_allclose(ys[:, 0], 2.0, rtol=5e-2)
+    np.testing.assert_allclose(ys[:, 1], 1.0 / 0.3, rtol=5e-2)
+
+
+def test_adjoint_gradient_vs_finite_differences():
+    """IFT gradient of sum(y*[:,1]^2) wrt k vs central FD, eps = 1e-6, 5e-4 of scale."""
+    y0, k, atm, net = _open_chain()
+    ys, _, _ = O.newton_steady_state(y0, k, atm, net)
+    v = np.zeros_like(ys)
+    v[:, 1] = 2.0 * ys[:, 1]
+    g, lam = O.adjoint_grad_k(ys, k, atm, net, v)
+    # lambda solves J_y^T lambda = v (dense check)
+    diag, sup, sub = O.jacobian_blocks(ys, k, atm, net)
+    A = O.assemble_dense(diag, sup, sub)
+    np.testing.assert_allclose(lam.reshape(-1), np.linalg.solve(A.T, v.reshape(-1)), rtol=1e-10)
+    fd = np.zeros_like(k)
+    eps = 1e-6
+    for r in (1, 3, 5):
+        for z in range(k.shape[1]):
+            kp, km = k.copy(), k.copy()
+            kp[r, z] += eps
+            km[r, z] -= eps
+            yp, _, _ = O.newton_steady_state(ys, kp, atm, net, tol=1e-12)
+            ym, _, _ = O.newton_steady_state(ys, km, atm, net, tol=1e-12)
+            fd[r, z] = ((yp[:, 1] ** 2).sum() - (ym[:, 1] ** 2).sum()) / (2 * eps)
+    assert np.max(np.abs(g - fd)) / np.max(np.abs(fd)) < 5e-4
+
+
+def test_residual_gate_raises():
+    """validate_steady_state_solution raises ValueError off the steady state."""
+    y0, k, atm, net = _open_chain()
+    with pytest.raises(ValueError):
+        O.validate_steady_state(y0 * 3.0, k, atm, net)
