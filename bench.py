@@ -1,0 +1,199 @@
+#!/usr/bin/env python
+"""Ros2 column-step throughput on one MI355X (FP64, nz=150, ni=93).
+
+One bench "step" is one full Ros2 step (chemistry + Jacobian + block-Thomas
+factor + two block solves + combine) for every column of a batch of
+independent columns of the nz=150 / ni=93 / 2461-rate-row workload, inputs
+resident in HBM.  Prints one JSON line; see the DESIGN notes for the
+roofline and CPU-baseline definitions.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT), str(ROOT / "jax-vulcan_amd")]
+
+NZ, NI, NREV, NPHOTO = 150, 93, 1200, 60
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector = matrix)
+PEAK_HBM_GBS = 8000.0
+
+
+def factor_flops(nz: int, n: int) -> float:
+    """Algorithmic FLOPs of one block-Thomas factor of one column:
+    nz LU's (2/3 n^3) + (nz-1) inverses (4/3 n^3) + Schur updates (3 n^2)."""
+    return nz * (2.0 / 3.0) * n ** 3 + (nz - 1) * ((4.0 / 3.0) * n ** 3 + 3.0 * n ** 2)
+
+
+def step_flops(nz: int, n: int, S: int) -> float:
+    """Algorithmic FLOPs of one Ros2 column-step (factor + 2 solves + chemistry)."""
+    f_sol = 2.0 * ((nz - 1) * (2 * n * n + 2 * n) * 2 + 2 * n * n)
+    return factor_flops(nz, n) + f_sol + 44.0 * nz * S
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=1024)
+    ap.add_argument("--single-steps", type=int, default=5)
+    ap.add_argument("--cpu-sample", type=int, default=64,
+                    help="column-steps timed on the host for the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(columns are independent: multi-GPU runs shard columns with no exchange)",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    from vulcan_amd import _lib, core, synth
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    lib = _lib.load()
+    B = args.batch
+    t_gen = time.time()
+    cols = synth.columns_same_network(1002, B, NZ, NI, NREV, NPHOTO, dt=1e-10, name="bench")
+    net = cols[0].net
+    S = net.nr + 1
+    y = torch.tensor(np.stack([c.y for c in cols]), dtype=torch.float64, device=dev)
+    k = torch.tensor(np.stack([c.k for c in cols]), dtype=torch.float64, device=dev)
+    atm = torch.cat([core.pack_atm_record(c.atm) for c in cols]).to(dev)
+    dt = torch.full((B,), 1e-10, dtype=torch.float64, device=dev)
+    print(f"[bench] generated {B} columns in {time.time() - t_gen:.1f}s", file=sys.stderr, flush=True)
+
+    def make(Bn):
+        b = core.uniform_batch(Bn, NZ, net)
+        ws = b.workspace()
+        sol = torch.empty((Bn, NZ, NI), dtype=torch.float64, device=dev)
+        delta = torch.empty_like(sol)
+        status = torch.zeros(Bn, dtype=torch.int32, device=dev)
+        return b, ws, sol, delta, status
+
+    def stepper(Bn):
+        b, ws, sol, delta, status = make(Bn)
+        st = torch.cuda.current_stream().cuda_stream
+
+        def run():
+            rc = lib.vk_ros2_step_f64(b.handle, y.data_ptr(), k.data_ptr(), atm.data_ptr(),
+                                      dt.data_ptr(), None, sol.data_ptr(), delta.data_ptr(),
+                                      ws.data_ptr(), ws.numel(), status.data_ptr(), st)
+            _lib.check(rc, "vk_ros2_step_f64")
+        return run, sol, status
+
+    # ---- batched throughput ------------------------------------------------
+    run, sol, status = stepper(B)
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    lib.vk_timing_reset()
+    lib.vk_timing_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lib.vk_timing_enable(0)
+    ms = (C.c_double * 6)()
+    cnt = (C.c_longlong * 6)()
+    lib.vk_timing_read(ms, cnt, 6)
+    kern_ms = [ms[i] / max(cnt[i], 1) for i in range(5)]
+    elapsed = t1 - t0
+    value = B * args.steps / elapsed
+    finite = bool(torch.isfinite(sol).all().item())
+    n_singular = int((status != 0).sum().item())
+    print(f"[bench] batch {B}: {elapsed / args.steps * 1e3:.2f} ms/step, "
+          f"{value:.1f} column-steps/s; kernels ms {['%.3f' % v for v in kern_ms]}",
+          file=sys.stderr, flush=True)
+
+    # ---- plain single-column kernel ------------------------------------------
+    run1, _, _ = stepper(1)
+    run1()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.single_steps):
+        run1()
+    torch.cuda.synchronize()
+    single = args.single_steps / (time.perf_counter() - t0)
+    print(f"[bench] single column: {single:.1f} column-steps/s", file=sys.stderr, flush=True)
+
+    # ---- roofline of the dominant kernel (block factor) ---------------------
+    fac_ms = kern_ms[1]
+    fac_flops = B * factor_flops(NZ, NI)
+    achieved = fac_flops / (fac_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+                "traffic": None}
+
+    # ---- CPU baseline: C restatement of the reference step -------------------
+    cpu = None
+    if not args.no_cpu:
+        from oracle import ros2_c
+        ncpu = os.cpu_count() or 1
+        threads = max(1, min(ncpu, 16))
+        n = min(args.cpu_sample, B)
+        ys = y[:n].cpu().numpy()
+        ks = k[:n].cpu().numpy()
+        recs = atm.view(B, -1)[:n].cpu().numpy()
+        ros2_c.ros2_steps(ys[:1], ks[:1], recs[:1], 1e-10, net, nthreads=1)  # warm
+        t0 = time.perf_counter()
+        ros2_c.ros2_steps(ys, ks, recs, 1e-10, net, nthreads=threads)
+        cpu_rate = n / (time.perf_counter() - t0)
+        m1 = max(1, n // 8)
+        t0 = time.perf_counter()
+        ros2_c.ros2_steps(ys[:m1], ks[:m1], recs[:m1], 1e-10, net, nthreads=1)
+        cpu_1 = m1 / (time.perf_counter() - t0)
+        cpu = {"value": round(cpu_rate, 3), "unit": "column-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{n} column-steps (one Ros2 step of {n} columns, nz=150, ni=93), "
+                         f"LU-based C restatement, one column per OpenMP thread",
+               "single_thread": round(cpu_1, 3), "host_cpus": ncpu}
+        print(f"[bench] cpu: {cpu_rate:.2f} column-steps/s on {threads} threads, "
+              f"{cpu_1:.2f} single-thread", file=sys.stderr, flush=True)
+
+    metric = json.loads((ROOT / "BASELINE.json").read_text())["metric"]
+    out = {
+        "metric": metric,
+        "value": round(value, 3),
+        "unit": "column-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded: workload-1 distributions, one shared network)",
+        "config": {"workload": "Ros2 step, nz=150 ni=93 S=2461 (1200 reversible + 60 photo rows), "
+                               f"{B} independent columns per step, dt=1e-10",
+                   "batch_columns": B, "nz": NZ, "ni": NI, "rate_rows": S},
+        "single_column_steps_per_s": round(single, 3),
+        "batched_over_single": round(value / single, 2),
+        "kernel_ms_per_launch": {"chem_lhs": round(kern_ms[0], 4), "bt_factor": round(kern_ms[1], 4),
+                                 "bt_back1": round(kern_ms[2], 4), "rhs2": round(kern_ms[3], 4),
+                                 "bt_solve2": round(kern_ms[4], 4)},
+        "step_tflops": round(B * step_flops(NZ, NI, S) / (elapsed / args.steps) / 1e12, 3),
+        "finite": finite,
+        "singular_columns": n_singular,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

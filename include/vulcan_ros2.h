@@ -122,6 +122,16 @@ int vk_residual_f64(vk_batch b, const double* y, const double* k, const double* 
 int vk_jacobian_blocks_f64(vk_batch b, const double* y, const double* k, const double* atm,
                            double* diag, double* sup_d, double* sub_d, void* stream);
 
+/* Kernel timing with HIP events recorded on the launch stream around each
+ * kernel of vk_ros2_step_f64 (no counterpart in the reference, which has no
+ * device timing).  Kernel ids: 0 chemistry+LHS, 1 block factor, 2 stage-1
+ * back sweep, 3 stage-2 rhs, 4 stage-2 solve.  vk_timing_read synchronises
+ * on the recorded events and returns accumulated milliseconds and launch
+ * counts per id.  Not thread-safe; for benchmarks. */
+int vk_timing_enable(int on);
+int vk_timing_reset(void);
+int vk_timing_read(double* ms, long long* count, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,53 @@ void vk_set_error(const char* fmt, ...) {
   va_end(ap);
 }
 
+// ---------------------------------------------------------------------------
+// optional per-kernel timing with HIP events recorded on the launch stream
+// ---------------------------------------------------------------------------
+namespace {
+enum { VK_NKERN = 6 };
+struct TimedLaunch {
+  int kind;
+  hipEvent_t a, b;
+};
+struct Timing {
+  bool on = false;
+  std::vector<TimedLaunch> pending;
+  std::vector<hipEvent_t> pool;
+  double ms[VK_NKERN] = {0};
+  long long count[VK_NKERN] = {0};
+};
+Timing g_timing;
+
+hipEvent_t take_event() {
+  if (!g_timing.pool.empty()) {
+    hipEvent_t e = g_timing.pool.back();
+    g_timing.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct KernelTimer {
+  int kind;
+  hipStream_t st;
+  hipEvent_t a = nullptr, b = nullptr;
+  KernelTimer(int k, hipStream_t s) : kind(k), st(s) {
+    if (!g_timing.on) return;
+    a = take_event();
+    b = take_event();
+    if (a && b) (void)hipEventRecord(a, st);
+  }
+  ~KernelTimer() {
+    if (!a || !b) return;
+    (void)hipEventRecord(b, st);
+    g_timing.pending.push_back({kind, a, b});
+  }
+};
+}  // namespace
+
 int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag, int diag_tile,
                         int transpose, const double* sup_d, const double* sub_d, double* inv,
                         const double* rhs, double* rmod, int* status, const int* active,
@@ -415,6 +462,39 @@ int vk_bt_inverse_natural_f64(vk_batch b, const double* inv, double* nat, void* 
   return VK_OK;
 }
 
+int vk_timing_enable(int on) {
+  g_timing.on = on != 0;
+  return VK_OK;
+}
+
+int vk_timing_reset(void) {
+  for (auto& p : g_timing.pending) {
+    g_timing.pool.push_back(p.a);
+    g_timing.pool.push_back(p.b);
+  }
+  g_timing.pending.clear();
+  for (int i = 0; i < VK_NKERN; ++i) { g_timing.ms[i] = 0; g_timing.count[i] = 0; }
+  return VK_OK;
+}
+
+int vk_timing_read(double* ms, long long* count, int n) {
+  for (auto& p : g_timing.pending) {
+    float t = 0.f;
+    VK_CHECK_HIP(hipEventSynchronize(p.b));
+    VK_CHECK_HIP(hipEventElapsedTime(&t, p.a, p.b));
+    g_timing.ms[p.kind] += t;
+    g_timing.count[p.kind] += 1;
+    g_timing.pool.push_back(p.a);
+    g_timing.pool.push_back(p.b);
+  }
+  g_timing.pending.clear();
+  for (int i = 0; i < n && i < VK_NKERN; ++i) {
+    if (ms) ms[i] = g_timing.ms[i];
+    if (count) count[i] = g_timing.count[i];
+  }
+  return VK_OK;
+}
+
 size_t vk_ros2_workspace_bytes(vk_batch b) {
   if (!b) return 0;
   return align_up<double>(b->block_doubles) + 8 * align_up<double>(b->state_doubles);
@@ -448,25 +528,40 @@ int vk_ros2_step_f64(vk_batch b, const double* y, const double* k, const double*
   const double r = ros2_gamma();
   for (int sl : b->slots) {
     int rc;
-    if ((rc = vk_launch_chem_layer(sl, M_LHS, b->d_cols, b->ncols, b->max_nz, b->max_S, b->max_T,
-                                   b->d_nets, y, nullptr, atm, k, dt, fix_mask, nullptr, rhs_y, blk,
-                                   sup, sub, nullptr, st)))
-      return rc;
-    if ((rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, blk, 1, 0, sup, sub, blk, rhs_y, rmod,
-                                  (int*)status, nullptr, st)))
-      return rc;
+    {
+      KernelTimer t(0, st);
+      if ((rc = vk_launch_chem_layer(sl, M_LHS, b->d_cols, b->ncols, b->max_nz, b->max_S,
+                                     b->max_T, b->d_nets, y, nullptr, atm, k, dt, fix_mask,
+                                     nullptr, rhs_y, blk, sup, sub, nullptr, st)))
+        return rc;
+    }
+    {
+      KernelTimer t(1, st);
+      if ((rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, blk, 1, 0, sup, sub, blk, rhs_y,
+                                    rmod, (int*)status, nullptr, st)))
+        return rc;
+    }
     SolveEpi e1{1, r, 0.0, y, nullptr, nullptr, yk2, nullptr};
-    if ((rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, rmod, rmod, k1, 1, e1,
-                                 nullptr, st)))
-      return rc;
-    if ((rc = vk_launch_chem_layer(sl, M_RHS2, b->d_cols, b->ncols, b->max_nz, b->max_S, b->max_T,
-                                   b->d_nets, yk2, nullptr, atm, k, dt, fix_mask, k1, rhs2, nullptr,
-                                   nullptr, nullptr, nullptr, st)))
-      return rc;
+    {
+      KernelTimer t(2, st);
+      if ((rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, rmod, rmod, k1, 1, e1,
+                                   nullptr, st)))
+        return rc;
+    }
+    {
+      KernelTimer t(3, st);
+      if ((rc = vk_launch_chem_layer(sl, M_RHS2, b->d_cols, b->ncols, b->max_nz, b->max_S,
+                                     b->max_T, b->d_nets, yk2, nullptr, atm, k, dt, fix_mask, k1,
+                                     rhs2, nullptr, nullptr, nullptr, nullptr, st)))
+        return rc;
+    }
     SolveEpi e2{2, 3.0 / (2.0 * r), 1.0 / (2.0 * r), y, k1, yk2, sol, delta};
-    if ((rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, rhs2, rmod, k2, 0, e2,
-                                 nullptr, st)))
-      return rc;
+    {
+      KernelTimer t(4, st);
+      if ((rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, rhs2, rmod, k2, 0, e2,
+                                   nullptr, st)))
+        return rc;
+    }
   }
   return VK_OK;
 }

@@ -40,6 +40,9 @@ _SIGS = {
     "vk_ros2_step_f64": (i32, [p, p, p, p, p, p, p, p, p, sz, p, p]),
     "vk_residual_f64": (i32, [p, p, p, p, p, p]),
     "vk_jacobian_blocks_f64": (i32, [p, p, p, p, p, p, p, p]),
+    "vk_timing_enable": (i32, [i32]),
+    "vk_timing_reset": (i32, []),
+    "vk_timing_read": (i32, [p, p, i32]),
 }
 
 _lib = None

@@ -201,20 +201,31 @@ def random_network(rng, ni: int, n_rev: int, n_photo: int = 0):
     return net, parents, rows
 
 
-def random_rates(rng, Tco: np.ndarray, n_rev: int, n_photo: int = 0):
-    """k (nr+1, nz): forward A*T^b*exp(-E/T), reverse k_f / 10**U[-5,5]."""
-    nz = Tco.shape[0]
-    nr = 2 * n_rev + n_photo
-    k = np.zeros((nr + 1, nz))
+def random_rate_params(rng, n_rev: int):
+    """(A, b, E, log10 K_eq) per reversible reaction."""
     A = 10.0 ** rng.uniform(-14.0, -9.0, size=n_rev)
     b = rng.uniform(-2.0, 2.0, size=n_rev)
     E = rng.uniform(0.0, 2.0e4, size=n_rev)
     logK = rng.uniform(-5.0, 5.0, size=n_rev)
+    return A, b, E, logK
+
+
+def rates_from_params(params, Tco: np.ndarray, n_photo: int = 0):
+    """k (nr+1, nz): forward A*T^b*exp(-E/T), reverse k_f / 10**logK."""
+    A, b, E, logK = params
+    n_rev = A.shape[0]
+    nz = Tco.shape[0]
+    k = np.zeros((2 * n_rev + n_photo + 1, nz))
     for q in range(n_rev):
         kf = A[q] * Tco ** b[q] * np.exp(-E[q] / Tco)
         k[2 * q + 1] = kf
         k[2 * q + 2] = kf / 10.0 ** logK[q]
     return k
+
+
+def random_rates(rng, Tco: np.ndarray, n_rev: int, n_photo: int = 0):
+    """k (nr+1, nz) with freshly drawn parameters."""
+    return rates_from_params(random_rate_params(rng, n_rev), Tco, n_photo)
 
 
 # --------------------------------------------------------------------------
@@ -314,6 +325,34 @@ def make_column(seed: int, nz: int, ni: int, n_rev: int, n_photo: int = 0,
     return Column(name=name, nz=nz, ni=ni, y=y, k=k, atm=atm, net=net, dt=dt,
                   P=P, dz=geo["dz"], photo=photo,
                   meta=dict(seed=seed, n_rev=n_rev, n_photo=n_photo))
+
+
+def columns_same_network(seed: int, B: int, nz: int, ni: int, n_rev: int,
+                         n_photo: int = 0, dt: float = 1e-6, name: str = "col"):
+    """B columns sharing one network and its rate parameters; each column
+    draws its own T(z), Kzz and mixing ratios (workload-1 distributions)."""
+    rng = np.random.default_rng(seed)
+    net, parents, rows = random_network(rng, ni, n_rev, n_photo)
+    params = random_rate_params(rng, n_rev)
+    cols = []
+    for c in range(B):
+        r = np.random.default_rng([seed, c])
+        P = np.logspace(9.0, -2.0, nz)
+        Tco = smooth_temperature(r, P)
+        geo = compute_geometry(P, Tco)
+        M = P / (KB * Tco)
+        ymix = 10.0 ** r.uniform(-20.0, 0.0, size=(nz, ni))
+        ymix /= ymix.sum(axis=1, keepdims=True)
+        atm = AtmStatic(
+            Kzz=10.0 ** r.uniform(5.0, 10.0, size=nz - 1), Dzz=np.zeros((nz - 1, ni)),
+            dzi=geo["dzi"], vz=np.zeros(nz - 1), Hpi=geo["Hpi"], Ti=geo["Ti"], Tco=Tco,
+            g=geo["g"], ms=np.full(ni, MU_MEAN), alpha=np.zeros(ni), M=M,
+            vm=np.zeros((nz, ni)), vs=np.zeros((nz - 1, ni)), top_flux=np.zeros(ni),
+            bot_flux=np.zeros(ni), bot_vdep=np.zeros(ni), gas_indx_mask=np.ones(ni, dtype=bool))
+        cols.append(Column(name=f"{name}_{c}", nz=nz, ni=ni, y=M[:, None] * ymix,
+                           k=rates_from_params(params, Tco, n_photo), atm=atm, net=net, dt=dt,
+                           P=P, dz=geo["dz"], meta=dict(seed=seed, index=c)))
+    return cols
 
 
 def config1(seed: int = 1001) -> Column:
