@@ -122,6 +122,39 @@ int vk_residual_f64(vk_batch b, const double* y, const double* k, const double* 
 int vk_jacobian_blocks_f64(vk_batch b, const double* y, const double* k, const double* atm,
                            double* diag, double* sup_d, double* sub_d, void* stream);
 
+/* ---- adaptive steady-state runs with the step control on the device ----
+ * Replaces the accept/reject/dt logic of outer_loop.body_fn
+ * (outer_loop.py:181-250, 741-768, 847-917, 1031-1038) and its stopping
+ * rule (cond_fn, outer_loop.py:672-713; relative change of _f_dy,
+ * outer_loop.py:2163-2184).  Defaults: vulcan_cfg.py:125-152. */
+#define VK_MAX_ATOMS 8
+
+typedef struct {
+  double rtol, dt_var_min, dt_var_max, dt_min, dt_max, atol, mtol, pos_cut, nega_cut, loss_eps;
+  double trun_min, conv_tol;
+  int32_t batch_max_retries, count_min, count_max, na, zero_bot_row, photo_frq;
+} vk_ss_params;
+
+/* Device state of a batch of runs.  State-shaped arrays at off_s; compo is
+ * the (ni, na) atom composition of each column at off_s; atom arrays are
+ * (ncols, VK_MAX_ATOMS); the rest are (ncols).  photo_due may be NULL. */
+typedef struct {
+  double *y, *y_prev, *ymix, *sol, *delta_arr;
+  const double* compo;
+  double *atom_ini, *atom_loss, *atom_loss_prev;
+  double *dt, *t, *delta, *rel;
+  int32_t *accept, *retry, *nega, *loss, *dcount, *status, *attempts, *photo_due;
+} vk_ss_state;
+
+/* ymix, y_prev, atom_ini, zeroed counters and status; y and dt must be set. */
+int vk_ss_init_f64(vk_batch b, const double* atm, const vk_ss_params* p, const vk_ss_state* s,
+                   void* stream);
+/* One attempt for every column whose status has no DONE bit: a Ros2 step at
+ * (y, dt) into sol/delta_arr, then the device controller.  No host sync, no
+ * allocation: safe to capture in a hipGraph and replay. */
+int vk_ss_attempt_f64(vk_batch b, const double* k, const double* atm, const vk_ss_params* p,
+                      const vk_ss_state* s, void* ws, size_t ws_bytes, void* stream);
+
 /* Kernel timing with HIP events recorded on the launch stream around each
  * kernel of vk_ros2_step_f64 (no counterpart in the reference, which has no
  * device timing).  Kernel ids: 0 chemistry+LHS, 1 block factor, 2 stage-1

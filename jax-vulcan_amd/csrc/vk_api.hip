@@ -85,6 +85,11 @@ int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max
                          const double* k1, double* out0, double* out_blk, double* out_sup,
                          double* out_sub, const int* active, hipStream_t stream);
 
+int vk_launch_ss_control(const VkCol* cols, int ncols, const double* atm, const vk_ss_params& p,
+                         const vk_ss_state& s, hipStream_t st);
+int vk_launch_ss_init(const VkCol* cols, int ncols, const double* atm, const vk_ss_params& p,
+                      const vk_ss_state& s, hipStream_t st);
+
 struct vk_network_s {
   VkNetDev dev;      // device pointers into buf
   void* buf = nullptr;
@@ -500,20 +505,9 @@ size_t vk_ros2_workspace_bytes(vk_batch b) {
   return align_up<double>(b->block_doubles) + 8 * align_up<double>(b->state_doubles);
 }
 
-int vk_ros2_step_f64(vk_batch b, const double* y, const double* k, const double* atm,
-                     const double* dt, const uint8_t* fix_mask, double* sol, double* delta,
-                     void* ws, size_t ws_bytes, int32_t* status, void* stream) {
-  g_err[0] = 0;
-  if (int rc = check_batch(b)) return rc;
-  if (!y || !k || !atm || !dt || !sol || !delta || !ws) {
-    vk_set_error("vk_ros2_step_f64: null pointer");
-    return VK_ERR_ARG;
-  }
-  if (ws_bytes < vk_ros2_workspace_bytes(b)) {
-    vk_set_error("vk_ros2_step_f64: workspace %zu < %zu bytes", ws_bytes, vk_ros2_workspace_bytes(b));
-    return VK_ERR_ARG;
-  }
-  hipStream_t st = (hipStream_t)stream;
+static int ros2_step_impl(vk_batch b, const double* y, const double* k, const double* atm,
+                          const double* dt, const uint8_t* fix_mask, double* sol, double* delta,
+                          void* ws, int32_t* status, const int* active, hipStream_t st) {
   char* w = (char*)ws;
   double* blk = (double*)w; w += align_up<double>(b->block_doubles);
   const size_t sv = align_up<double>(b->state_doubles);
@@ -532,38 +526,95 @@ int vk_ros2_step_f64(vk_batch b, const double* y, const double* k, const double*
       KernelTimer t(0, st);
       if ((rc = vk_launch_chem_layer(sl, M_LHS, b->d_cols, b->ncols, b->max_nz, b->max_S,
                                      b->max_T, b->d_nets, y, nullptr, atm, k, dt, fix_mask,
-                                     nullptr, rhs_y, blk, sup, sub, nullptr, st)))
+                                     nullptr, rhs_y, blk, sup, sub, active, st)))
         return rc;
     }
     {
       KernelTimer t(1, st);
       if ((rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, blk, 1, 0, sup, sub, blk, rhs_y,
-                                    rmod, (int*)status, nullptr, st)))
+                                    rmod, (int*)status, active, st)))
         return rc;
     }
     SolveEpi e1{1, r, 0.0, y, nullptr, nullptr, yk2, nullptr};
     {
       KernelTimer t(2, st);
       if ((rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, rmod, rmod, k1, 1, e1,
-                                   nullptr, st)))
+                                   active, st)))
         return rc;
     }
     {
       KernelTimer t(3, st);
       if ((rc = vk_launch_chem_layer(sl, M_RHS2, b->d_cols, b->ncols, b->max_nz, b->max_S,
                                      b->max_T, b->d_nets, yk2, nullptr, atm, k, dt, fix_mask, k1,
-                                     rhs2, nullptr, nullptr, nullptr, nullptr, st)))
+                                     rhs2, nullptr, nullptr, nullptr, active, st)))
         return rc;
     }
     SolveEpi e2{2, 3.0 / (2.0 * r), 1.0 / (2.0 * r), y, k1, yk2, sol, delta};
     {
       KernelTimer t(4, st);
       if ((rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, rhs2, rmod, k2, 0, e2,
-                                   nullptr, st)))
+                                   active, st)))
         return rc;
     }
   }
   return VK_OK;
+}
+
+int vk_ros2_step_f64(vk_batch b, const double* y, const double* k, const double* atm,
+                     const double* dt, const uint8_t* fix_mask, double* sol, double* delta,
+                     void* ws, size_t ws_bytes, int32_t* status, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  if (!y || !k || !atm || !dt || !sol || !delta || !ws) {
+    vk_set_error("vk_ros2_step_f64: null pointer");
+    return VK_ERR_ARG;
+  }
+  if (ws_bytes < vk_ros2_workspace_bytes(b)) {
+    vk_set_error("vk_ros2_step_f64: workspace %zu < %zu bytes", ws_bytes, vk_ros2_workspace_bytes(b));
+    return VK_ERR_ARG;
+  }
+  return ros2_step_impl(b, y, k, atm, dt, fix_mask, sol, delta, ws, status, nullptr,
+                        (hipStream_t)stream);
+}
+
+static int check_ss(vk_batch b, const double* atm, const vk_ss_params* p, const vk_ss_state* s) {
+  if (int rc = check_batch(b)) return rc;
+  if (!atm || !p || !s || !s->y || !s->y_prev || !s->ymix || !s->sol || !s->delta_arr ||
+      !s->compo || !s->atom_ini || !s->atom_loss || !s->atom_loss_prev || !s->dt || !s->t ||
+      !s->delta || !s->rel || !s->accept || !s->retry || !s->nega || !s->loss || !s->dcount ||
+      !s->status || !s->attempts) {
+    vk_set_error("steady-state call: null pointer in arguments");
+    return VK_ERR_ARG;
+  }
+  if (p->na < 1 || p->na > VK_MAX_ATOMS) {
+    vk_set_error("steady-state call: na=%d outside 1..%d", p->na, VK_MAX_ATOMS);
+    return VK_ERR_ARG;
+  }
+  return VK_OK;
+}
+
+int vk_ss_init_f64(vk_batch b, const double* atm, const vk_ss_params* p, const vk_ss_state* s,
+                   void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_ss(b, atm, p, s)) return rc;
+  return vk_launch_ss_init(b->d_cols, b->ncols, atm, *p, *s, (hipStream_t)stream);
+}
+
+int vk_ss_attempt_f64(vk_batch b, const double* k, const double* atm, const vk_ss_params* p,
+                      const vk_ss_state* s, void* ws, size_t ws_bytes, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_ss(b, atm, p, s)) return rc;
+  if (!k || !ws || ws_bytes < vk_ros2_workspace_bytes(b)) {
+    vk_set_error("vk_ss_attempt_f64: missing rates or workspace (%zu < %zu bytes)", ws_bytes,
+                 vk_ros2_workspace_bytes(b));
+    return VK_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  if (int rc = ros2_step_impl(b, s->y, k, atm, s->dt, nullptr, s->sol, s->delta_arr, ws,
+                              s->status, s->status, st))
+    return rc;
+  KernelTimer t(5, st);
+  return vk_launch_ss_control(b->d_cols, b->ncols, atm, *p, *s, st);
 }
 
 }  // extern "C"

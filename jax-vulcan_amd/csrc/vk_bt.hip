@@ -221,7 +221,7 @@ __global__ __launch_bounds__(VK_NT) void k_bt_factor(
   __shared__ BtShared<SL> sh;
   const int c = blockIdx.x;
   if (c >= ncols) return;
-  if (active && !(active[c])) return;
+  if (active && (active[c] & VK_ST_DONE)) return;
   const VkCol col = cols[c];
   if (vk_slots(col.ni) != SL) return;
   const int nz = col.nz, ni = col.ni;
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(VK_NT) void k_bt_solve(
   __shared__ double vec[2][VK_G * SL];
   const int c = blockIdx.x;
   if (c >= ncols) return;
-  if (active && !(active[c])) return;
+  if (active && (active[c] & VK_ST_DONE)) return;
   const VkCol col = cols[c];
   if (vk_slots(col.ni) != SL) return;
   const int nz = col.nz, ni = col.ni;

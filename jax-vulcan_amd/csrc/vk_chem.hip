@@ -176,7 +176,7 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
   const int c = blockIdx.y;
   const int z = blockIdx.x;
   if (c >= L.ncols) return;
-  if (L.active && !L.active[c]) return;
+  if (L.active && (L.active[c] & VK_ST_DONE)) return;
   const VkCol col = L.cols[c];
   if (vk_slots(col.ni) != SL) return;  // columns of other buckets
   const int nz = col.nz, ni = col.ni, tid = threadIdx.x;
@@ -383,11 +383,18 @@ int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max
     return VK_ERR_UNSUPPORTED;
   }
   dim3 grid(max_nz, ncols);
+  // The LDS limit is raised once per instantiation, outside any graph capture
+  // that later launches may be part of.
+  static bool raised[8] = {false};
   switch (sl) {
 #define VK_CASE(n)                                                                        \
   case n:                                                                                 \
-    hipFuncSetAttribute((const void*)k_chem_layer<n>,                                     \
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);          \
+    if (!raised[n]) {                                                                     \
+      VK_CHECK_HIP(hipFuncSetAttribute((const void*)k_chem_layer<n>,                      \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,        \
+                                       160 * 1024));                                      \
+      raised[n] = true;                                                                   \
+    }                                                                                     \
     hipLaunchKernelGGL(k_chem_layer<n>, grid, dim3(VK_NT), shmem, stream, L);             \
     break;
     VK_CASE(1) VK_CASE(2) VK_CASE(3) VK_CASE(4) VK_CASE(5) VK_CASE(6) VK_CASE(7)

@@ -1,0 +1,284 @@
+// Per-column adaptive step control, kept on the device.
+//
+// Restates the accept/reject part of outer_loop.body_fn:
+//   clip (outer_loop.py:196-224), atom loss (:171-178), aggregated delta
+//   (:227-250), accept / force-accept / reject and the counters (:741-768),
+//   hydrostatic rebalance n0 * ymix (:847-861), state update (:900-917) and
+//   the dt update (:181-193, 1031-1038),
+// plus the stopping rule of the plain steady-state run: after an accepted
+// step, the largest masked relative change |y - y_prev| / y (the quantity of
+// _f_dy, outer_loop.py:2163-2184) below conv_tol once t > trun_min and
+// accept_count > count_min (the "ready" gate of cond_fn, :706-711), or
+// count_max accepted steps.
+//
+// One 256-thread workgroup per column; layer-parallel passes with fixed
+// reduction trees, so every column's result is independent of the batch.
+#include "vk_common.h"
+
+namespace {
+
+constexpr int NT = VK_NT;
+
+__device__ inline double clip_val(double v, double ymix_old, const vk_ss_params& p) {
+  double c = (v < p.pos_cut && v >= p.nega_cut) ? 0.0 : v;
+  if (ymix_old < p.mtol && c < 0.0) c = 0.0;
+  return c;
+}
+
+template <typename T, typename Op>
+__device__ inline T block_reduce(T v, T* red, Op op) {
+  for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  T r = red[0];
+  for (int w = 1; w < NT / 64; ++w) r = op(r, red[w]);
+  return r;
+}
+
+struct AddD { __device__ double operator()(double a, double b) const { return a + b; } };
+struct MaxD { __device__ double operator()(double a, double b) const { return a > b ? a : b; } };
+struct OrI { __device__ int operator()(int a, int b) const { return a | b; } };
+
+}  // namespace
+
+__global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ cols, int ncols,
+                                                    const double* __restrict__ atm,
+                                                    vk_ss_params p, vk_ss_state s) {
+  __shared__ double redd[NT / 64];
+  __shared__ int redi[NT / 64];
+  const int c = blockIdx.x;
+  if (c >= ncols) return;
+  if (s.status[c] & VK_ST_DONE) return;
+  const VkCol col = cols[c];
+  const int nz = col.nz, ni = col.ni, tid = threadIdx.x;
+  const int na = p.na;
+  const VkAtmOff ao = vk_atm_offsets(nz, ni);
+  const double* n0 = atm + col.off_a + ao.M;
+  const double* gas = atm + col.off_a + ao.gas;
+  double* y = s.y + col.off_s;
+  double* yp = s.y_prev + col.off_s;
+  double* ymix = s.ymix + col.off_s;
+  const double* sol = s.sol + col.off_s;
+  const double* dar = s.delta_arr + col.off_s;
+  const double* compo = s.compo + col.off_s;  // (ni, na)
+
+  // ---- pass 1: clip, ymix_new, delta, sign flags, atom sums ----------------
+  double dmax = 0.0;
+  int flags = 0;  // 1: a negative value after the clip, 2: a non-finite value
+  double at_clip[VK_MAX_ATOMS];
+  for (int a = 0; a < VK_MAX_ATOMS; ++a) at_clip[a] = 0.0;
+  for (int z = tid; z < nz; z += NT) {
+    double ys = 0.0;
+    for (int i = 0; i < ni; ++i) {
+      const int64_t e = (int64_t)z * ni + i;
+      const double sc = clip_val(sol[e], ymix[e], p);
+      if (gas[i] != 0.0) ys += sc;
+    }
+    for (int i = 0; i < ni; ++i) {
+      const int64_t e = (int64_t)z * ni + i;
+      const double sc = clip_val(sol[e], ymix[e], p);
+      if (sc < 0.0) flags |= 1;
+      if (!isfinite(sol[e])) flags |= 2;
+      for (int a = 0; a < na; ++a) at_clip[a] += compo[i * na + a] * sc;
+      // aggregated delta (outer_loop.py:238-248) with the pre-step ymix
+      double m = (ymix[e] < p.mtol) ? 0.0 : dar[e];
+      if (sc < p.atol) m = 0.0;
+      if (p.zero_bot_row && z == 0) m = 0.0;
+      if (sc > 0.0) {
+        const double r = m / fmax(fabs(sc), 1e-300);
+        dmax = r > dmax ? r : dmax;
+      }
+    }
+  }
+  dmax = block_reduce(dmax, redd, MaxD());
+  flags = block_reduce(flags, redi, OrI());
+  for (int a = 0; a < na; ++a) at_clip[a] = block_reduce(at_clip[a], redd, AddD());
+
+  // ---- accept / reject (outer_loop.py:741-768) -------------------------------
+  double* al = s.atom_loss + (int64_t)c * VK_MAX_ATOMS;
+  double* alp = s.atom_loss_prev + (int64_t)c * VK_MAX_ATOMS;
+  const double* ai = s.atom_ini + (int64_t)c * VK_MAX_ATOMS;
+  double loss_diff = 0.0;
+  for (int a = 0; a < na; ++a) {
+    const double aln = (at_clip[a] - ai[a]) / ai[a];
+    const double d = fabs(aln - alp[a]);
+    loss_diff = d > loss_diff ? d : loss_diff;
+  }
+  const double dt = s.dt[c];
+  const int retry = s.retry[c];
+  const bool all_nonneg = !(flags & 1);
+  const bool accept = all_nonneg && (loss_diff < p.loss_eps) && (dmax <= p.rtol);
+  const double next_dt_rej = dt * p.dt_var_min;
+  const bool force = ((next_dt_rej < p.dt_min) || (retry >= p.batch_max_retries)) && !accept;
+  const bool do_accept = accept || force;
+
+  // ---- pass 2: state update -----------------------------------------------
+  // accepted: y = n0 * ymix_new (rebalanced), ymix = y / ysum(y)
+  // forced:   y = max(y_prev, 0), ymix unchanged
+  // rejected: y = y_prev, ymix unchanged
+  double relmax = 0.0;
+  double at_b[VK_MAX_ATOMS];
+  for (int a = 0; a < VK_MAX_ATOMS; ++a) at_b[a] = 0.0;
+  for (int z = tid; z < nz; z += NT) {
+    double ys = 0.0;
+    for (int i = 0; i < ni; ++i) {
+      const int64_t e = (int64_t)z * ni + i;
+      const double sc = clip_val(sol[e], ymix[e], p);
+      if (gas[i] != 0.0) ys += sc;
+    }
+    double ysb = 0.0;
+    if (accept) {
+      for (int i = 0; i < ni; ++i) {
+        const int64_t e = (int64_t)z * ni + i;
+        const double sc = clip_val(sol[e], ymix[e], p);
+        const double sb = n0[z] * (sc / ys);
+        if (gas[i] != 0.0) ysb += sb;
+      }
+    }
+    for (int i = 0; i < ni; ++i) {
+      const int64_t e = (int64_t)z * ni + i;
+      const double yprev = yp[e];
+      double ynew, mixnew = ymix[e];
+      if (force) {
+        ynew = yprev < 0.0 ? 0.0 : yprev;
+      } else if (accept) {
+        const double sc = clip_val(sol[e], ymix[e], p);
+        ynew = n0[z] * (sc / ys);
+        mixnew = ynew / ysb;
+        for (int a = 0; a < na; ++a) at_b[a] += compo[i * na + a] * ynew;
+      } else {
+        ynew = yprev;
+      }
+      if (do_accept) {
+        // relative change vs the previous accepted state (outer_loop.py:2174-2183)
+        double dy = fabs(ynew - yprev);
+        if (mixnew < p.mtol) dy = 0.0;
+        if (ynew < p.atol) dy = 0.0;
+        if (ynew > 0.0) {
+          const double r = dy / ynew;
+          relmax = r > relmax ? r : relmax;
+        }
+        yp[e] = ynew;
+      }
+      y[e] = ynew;
+      ymix[e] = mixnew;
+    }
+  }
+  relmax = block_reduce(relmax, redd, MaxD());
+  if (accept)
+    for (int a = 0; a < na; ++a) at_b[a] = block_reduce(at_b[a], redd, AddD());
+
+  if (tid == 0) {
+    // atom loss bookkeeping (outer_loop.py:898, 914-917)
+    if (accept) {
+      for (int a = 0; a < na; ++a) {
+        al[a] = (at_b[a] - ai[a]) / ai[a];
+        alp[a] = al[a];
+      }
+    } else if (force) {
+      for (int a = 0; a < na; ++a) al[a] = alp[a];
+    }
+    const bool delta_too_big = dmax > p.rtol;
+    const bool reject = !do_accept;
+    if (reject && delta_too_big) s.dcount[c] += 1;
+    if (reject && !delta_too_big && (flags & 1)) s.nega[c] += 1;
+    if (reject && !delta_too_big && !(flags & 1)) s.loss[c] += 1;
+    const double t_old = s.t[c];
+    const double t_new = do_accept ? t_old + (force ? p.dt_min : dt) : t_old;
+    s.t[c] = t_new;
+    s.delta[c] = dmax;
+    const int acc = s.accept[c] + (do_accept ? 1 : 0);
+    s.accept[c] = acc;
+    s.retry[c] = do_accept ? 0 : retry + 1;
+    s.attempts[c] += 1;
+    // dt update (outer_loop.py:181-193, 1031-1038)
+    const double base = force ? p.dt_min : dt;
+    double dt_new;
+    if (do_accept) {
+      const double de = (dmax == 0.0) ? 0.01 * p.rtol : dmax;
+      double h = 0.9 * sqrt(p.rtol / de);
+      h = fmin(fmax(h, p.dt_var_min), p.dt_var_max);
+      dt_new = fmin(fmax(base * h, p.dt_min), p.dt_max);
+    } else {
+      dt_new = next_dt_rej;
+    }
+    s.dt[c] = dt_new;
+    int st = s.status[c];
+    if (do_accept) {
+      s.rel[c] = relmax;
+      const bool ready = (t_new > p.trun_min) && (acc > p.count_min);
+      if (ready && relmax < p.conv_tol) st |= VK_ST_CONVERGED;
+    }
+    if (acc >= p.count_max) st |= VK_ST_MAXSTEPS;
+    if (flags & 2) st |= VK_ST_NONFINITE;
+    s.status[c] = st;
+    // photolysis due before the next attempt (outer_loop.py:253-257)
+    if (s.photo_due) s.photo_due[c] = (s.retry[c] == 0 && p.photo_frq > 0 &&
+                                       (acc % p.photo_frq) == 0) ? 1 : 0;
+  }
+}
+
+// ymix, atom_ini / atom_loss, counters; y and dt are set by the caller.
+__global__ __launch_bounds__(NT) void k_ss_init(const VkCol* __restrict__ cols, int ncols,
+                                                 const double* __restrict__ atm, vk_ss_params p,
+                                                 vk_ss_state s) {
+  __shared__ double redd[NT / 64];
+  const int c = blockIdx.x;
+  if (c >= ncols) return;
+  const VkCol col = cols[c];
+  const int nz = col.nz, ni = col.ni, tid = threadIdx.x, na = p.na;
+  const VkAtmOff ao = vk_atm_offsets(nz, ni);
+  const double* gas = atm + col.off_a + ao.gas;
+  double* y = s.y + col.off_s;
+  double* yp = s.y_prev + col.off_s;
+  double* ymix = s.ymix + col.off_s;
+  const double* compo = s.compo + col.off_s;
+  double at[VK_MAX_ATOMS];
+  for (int a = 0; a < VK_MAX_ATOMS; ++a) at[a] = 0.0;
+  for (int z = tid; z < nz; z += NT) {
+    double ys = 0.0;
+    for (int i = 0; i < ni; ++i)
+      if (gas[i] != 0.0) ys += y[(int64_t)z * ni + i];
+    for (int i = 0; i < ni; ++i) {
+      const int64_t e = (int64_t)z * ni + i;
+      ymix[e] = y[e] / ys;
+      yp[e] = y[e];
+      for (int a = 0; a < na; ++a) at[a] += compo[i * na + a] * y[e];
+    }
+  }
+  for (int a = 0; a < na; ++a) at[a] = block_reduce(at[a], redd, AddD());
+  if (tid == 0) {
+    for (int a = 0; a < VK_MAX_ATOMS; ++a) {
+      const double v = a < na ? at[a] : 1.0;
+      s.atom_ini[(int64_t)c * VK_MAX_ATOMS + a] = v;
+      s.atom_loss[(int64_t)c * VK_MAX_ATOMS + a] = 0.0;
+      s.atom_loss_prev[(int64_t)c * VK_MAX_ATOMS + a] = 0.0;
+    }
+    s.t[c] = 0.0;
+    s.delta[c] = 0.0;
+    s.rel[c] = 0.0;
+    s.accept[c] = 0;
+    s.retry[c] = 0;
+    s.nega[c] = 0;
+    s.loss[c] = 0;
+    s.dcount[c] = 0;
+    s.attempts[c] = 0;
+    s.status[c] = 0;
+    if (s.photo_due) s.photo_due[c] = p.photo_frq > 0 ? 1 : 0;
+  }
+}
+
+int vk_launch_ss_control(const VkCol* cols, int ncols, const double* atm, const vk_ss_params& p,
+                         const vk_ss_state& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_ss_control, dim3(ncols), dim3(NT), 0, st, cols, ncols, atm, p, s);
+  VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}
+
+int vk_launch_ss_init(const VkCol* cols, int ncols, const double* atm, const vk_ss_params& p,
+                      const vk_ss_state& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_ss_init, dim3(ncols), dim3(NT), 0, st, cols, ncols, atm, p, s);
+  VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}

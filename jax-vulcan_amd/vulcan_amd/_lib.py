@@ -40,6 +40,8 @@ _SIGS = {
     "vk_ros2_step_f64": (i32, [p, p, p, p, p, p, p, p, p, sz, p, p]),
     "vk_residual_f64": (i32, [p, p, p, p, p, p]),
     "vk_jacobian_blocks_f64": (i32, [p, p, p, p, p, p, p, p]),
+    "vk_ss_init_f64": (i32, [p, p, p, p, p]),
+    "vk_ss_attempt_f64": (i32, [p, p, p, p, p, p, sz, p]),
     "vk_timing_enable": (i32, [i32]),
     "vk_timing_reset": (i32, []),
     "vk_timing_read": (i32, [p, p, i32]),
@@ -56,6 +58,23 @@ def exported_symbols():
     return list(_SIGS)
 
 
+class _Bound:
+    """Only the entry points with a declared signature are reachable, so a
+    64-bit device pointer can never be passed through ctypes' default
+    (32-bit int) conversion."""
+
+    def __init__(self, lib):
+        self._cdll = lib
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, name, fn)
+
+    def __getattr__(self, name):
+        raise AttributeError(f"{name} has no ctypes signature in vulcan_amd._lib._SIGS")
+
+
 def load():
     """Load the library once; raise if it was never built."""
     global _lib
@@ -65,13 +84,8 @@ def load():
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
             f"g.build()'` (hipcc, gfx950).  There is no CPU fallback.")
-    lib = C.CDLL(str(LIB_PATH))
-    for name, (res, args) in _SIGS.items():
-        fn = getattr(lib, name)
-        fn.restype = res
-        fn.argtypes = args
-    _lib = lib
-    return lib
+    _lib = _Bound(C.CDLL(str(LIB_PATH)))
+    return _lib
 
 
 def check(rc: int, what: str = ""):

@@ -1,0 +1,208 @@
+"""Adaptive steady-state runs with the step control on the device.
+
+``SteadyStateRunner`` is the GPU counterpart of the reference's
+``OuterLoop`` runner for the step-control part of ``body_fn`` /
+``cond_fn`` (outer_loop.py:492-1112): every column of a batch keeps its
+state, step size, counters and status on the device, one *attempt* is a
+Ros2 step plus the controller for every unfinished column, and attempts
+can be replayed from a captured HIP graph.  The host only polls the status
+words every ``check_every`` attempts.
+
+Photolysis, when the columns carry a ``PhotoSetup``, refreshes the
+photolysis rows of k before an attempt with retry_count == 0 and
+accept_count % photo_frq == 0 (outer_loop.py:716-724).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .core import F64, Batch, device, device_network, pack_atm_record
+
+VK_MAX_ATOMS = 8
+ST_SINGULAR, ST_NONFINITE, ST_CONVERGED, ST_MAXSTEPS = 1, 2, 4, 8
+ST_DONE = ST_CONVERGED | ST_MAXSTEPS
+
+
+class SsParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "rtol", "dt_var_min", "dt_var_max", "dt_min", "dt_max", "atol", "mtol", "pos_cut",
+        "nega_cut", "loss_eps", "trun_min", "conv_tol")] + [(n, C.c_int32) for n in (
+            "batch_max_retries", "count_min", "count_max", "na", "zero_bot_row", "photo_frq")]
+
+
+class SsState(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "y", "y_prev", "ymix", "sol", "delta_arr", "compo", "atom_ini", "atom_loss",
+        "atom_loss_prev", "dt", "t", "delta", "rel", "accept", "retry", "nega", "loss",
+        "dcount", "status", "attempts", "photo_due")]
+
+
+@dataclass
+class RunConfig:
+    """Solver tolerances (vulcan_cfg.py:125-152) and the stopping rule."""
+    rtol: float = 0.2
+    dt_var_min: float = 0.5
+    dt_var_max: float = 2.0
+    dt_min: float = 1e-14
+    dt_max: float = 1e22 * 1e-5
+    atol: float = 0.1
+    mtol: float = 1e-22
+    pos_cut: float = 0.0
+    nega_cut: float = -1.0
+    loss_eps: float = 0.1
+    batch_max_retries: int = 8
+    trun_min: float = 1e2
+    count_min: int = 120
+    count_max: int = 5000
+    conv_tol: float = 1e-4
+    photo_frq: int = 5
+
+    def c_params(self, na: int, zero_bot_row: bool = False) -> SsParams:
+        p = SsParams()
+        for f in ("rtol", "dt_var_min", "dt_var_max", "dt_min", "dt_max", "atol", "mtol",
+                  "pos_cut", "nega_cut", "loss_eps", "trun_min", "conv_tol"):
+            setattr(p, f, float(getattr(self, f)))
+        p.batch_max_retries = int(self.batch_max_retries)
+        p.count_min = int(self.count_min)
+        p.count_max = int(self.count_max)
+        p.na = int(na)
+        p.zero_bot_row = int(bool(zero_bot_row))
+        p.photo_frq = int(self.photo_frq)
+        return p
+
+
+class SteadyStateRunner:
+    """Batch of adaptive Ros2 runs, uneven nz / ni / networks allowed."""
+
+    def __init__(self, columns: Sequence, cfg: RunConfig = RunConfig(), compo=None,
+                 photo: Optional[object] = None):
+        self.dev = device()
+        self.cfg = cfg
+        cols = list(columns)
+        self.columns = cols
+        nets, net_ids, seen = [], [], {}
+        for c in cols:
+            key = id(c.net)
+            if key not in seen:
+                seen[key] = len(nets)
+                nets.append(device_network(c.net))
+            net_ids.append(seen[key])
+        self.batch = Batch([c.nz for c in cols], nets, net_ids)
+        b = self.batch
+        self.na = 1 if compo is None else int(np.asarray(compo[0]).shape[1])
+        if not 1 <= self.na <= VK_MAX_ATOMS:
+            raise ValueError(f"na must be in 1..{VK_MAX_ATOMS}")
+        dev, n = self.dev, b.ncols
+        z = lambda: torch.zeros(b.state_len, dtype=F64, device=dev)
+        self.y = b.pack_state([c.y for c in cols], dev)
+        self.y_prev, self.ymix, self.sol, self.delta_arr = z(), z(), z(), z()
+        comp = torch.zeros(b.state_len, dtype=F64)
+        for ci, c in enumerate(cols):
+            cm = np.ones((c.ni, 1)) if compo is None else np.asarray(compo[ci], dtype=np.float64)
+            o = int(b.off_s[ci])
+            comp[o:o + cm.size] = torch.as_tensor(cm.reshape(-1))
+        self.compo = comp.to(dev)
+        self.k = b.pack_k([c.k for c in cols], dev)
+        self.atm = b.pack_atm([c.atm for c in cols], dev)
+        za = lambda: torch.zeros(n * VK_MAX_ATOMS, dtype=F64, device=dev)
+        self.atom_ini, self.atom_loss, self.atom_loss_prev = za(), za(), za()
+        self.dt = torch.tensor([float(c.dt) for c in cols], dtype=F64, device=dev)
+        zc = lambda: torch.zeros(n, dtype=F64, device=dev)
+        self.t, self.delta, self.rel = zc(), zc(), zc()
+        zi = lambda: torch.zeros(n, dtype=torch.int32, device=dev)
+        (self.accept, self.retry, self.nega, self.loss, self.dcount, self.status,
+         self.attempts, self.photo_due) = (zi() for _ in range(8))
+        zero_bot = any(bool(c.atm.use_botflux) for c in cols)
+        self.params = cfg.c_params(self.na, zero_bot)
+        self.state = SsState(*[t.data_ptr() for t in (
+            self.y, self.y_prev, self.ymix, self.sol, self.delta_arr, self.compo, self.atom_ini,
+            self.atom_loss, self.atom_loss_prev, self.dt, self.t, self.delta, self.rel,
+            self.accept, self.retry, self.nega, self.loss, self.dcount, self.status,
+            self.attempts, self.photo_due)])
+        self.ws = b.workspace()
+        self.photo = None
+        if photo is not None or any(getattr(c, "photo", None) is not None for c in cols):
+            from .photo import PhotoBatch
+            self.photo = PhotoBatch(self, cols)
+        self.graph = None
+        self._init()
+
+    # ------------------------------------------------------------------
+    def _init(self):
+        lib = _lib.load()
+        _lib.check(lib.vk_ss_init_f64(self.batch.handle, self.atm.data_ptr(), C.byref(self.params),
+                                      C.byref(self.state), _lib.stream_ptr()), "vk_ss_init_f64")
+
+    def attempt(self):
+        """One attempt (photo refresh if due, Ros2 step, controller) on the current stream."""
+        lib = _lib.load()
+        if self.photo is not None:
+            self.photo.update()
+        _lib.check(lib.vk_ss_attempt_f64(self.batch.handle, self.k.data_ptr(), self.atm.data_ptr(),
+                                         C.byref(self.params), C.byref(self.state),
+                                         self.ws.data_ptr(), self.ws.numel(), _lib.stream_ptr()),
+                   "vk_ss_attempt_f64")
+
+    def capture(self, n_attempts: int):
+        """Capture n attempts in one HIP graph (replayed by run())."""
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.attempt()          # warm the code paths outside the capture
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(n_attempts):
+                self.attempt()
+        self.graph = (g, n_attempts)
+        self.reset_to_initial()
+        return g
+
+    def reset_to_initial(self):
+        b = self.batch
+        self.y.copy_(b.pack_state([c.y for c in self.columns], self.dev))
+        self.dt.copy_(torch.tensor([float(c.dt) for c in self.columns], dtype=F64))
+        self.k.copy_(b.pack_k([c.k for c in self.columns], self.dev))
+        if self.photo is not None:
+            self.photo.reset()
+        self._init()
+
+    def done(self) -> bool:
+        return bool(((self.status & ST_DONE) != 0).all().item())
+
+    def run(self, max_attempts: int = 100000, check_every: int = 50, use_graph: bool = False):
+        """Attempt until every column is done or max_attempts; returns attempts made."""
+        done_attempts = 0
+        if use_graph and self.graph is None:
+            self.capture(check_every)
+        while done_attempts < max_attempts:
+            if use_graph:
+                self.graph[0].replay()
+                done_attempts += self.graph[1]
+            else:
+                for _ in range(check_every):
+                    self.attempt()
+                done_attempts += check_every
+            if self.done():
+                break
+        torch.cuda.synchronize()
+        return done_attempts
+
+    # ------------------------------------------------------------------
+    def results(self):
+        b = self.batch
+        ys = [t.detach().cpu().numpy() for t in b.unpack_state(self.y)]
+        mixes = [t.detach().cpu().numpy() for t in b.unpack_state(self.ymix)]
+        return dict(
+            y=ys, ymix=mixes, status=self.status.cpu().numpy(), accept=self.accept.cpu().numpy(),
+            attempts=self.attempts.cpu().numpy(), t=self.t.cpu().numpy(), dt=self.dt.cpu().numpy(),
+            rel=self.rel.cpu().numpy(), delta=self.delta.cpu().numpy(),
+            counts=dict(nega=self.nega.cpu().numpy(), loss=self.loss.cpu().numpy(),
+                        delta=self.dcount.cpu().numpy()))

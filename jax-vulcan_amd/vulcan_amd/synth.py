@@ -201,12 +201,25 @@ def random_network(rng, ni: int, n_rev: int, n_photo: int = 0):
     return net, parents, rows
 
 
-def random_rate_params(rng, n_rev: int):
-    """(A, b, E, log10 K_eq) per reversible reaction."""
+def random_rate_params(rng, n_rev: int, net: NetworkArrays | None = None,
+                       consistent: bool = False):
+    """(A, b, E, log10 K_eq) per reversible reaction.
+
+    consistent=False draws log10 K_eq ~ U[-5, 5] per reaction.
+    consistent=True draws a potential g_i ~ U[-1.25, 1.25] per species and sets
+    log10 K_eq = sum_products g - sum_reactants g (detailed balance, so the
+    chemistry has one equilibrium; the span of log10 K stays [-5, 5])."""
     A = 10.0 ** rng.uniform(-14.0, -9.0, size=n_rev)
     b = rng.uniform(-2.0, 2.0, size=n_rev)
     E = rng.uniform(0.0, 2.0e4, size=n_rev)
     logK = rng.uniform(-5.0, 5.0, size=n_rev)
+    if consistent:
+        ni = net.ni
+        g = np.concatenate([rng.uniform(-1.25, 1.25, size=ni), [0.0]])
+        for q in range(n_rev):
+            i = 2 * q + 1
+            logK[q] = (np.sum(g[net.product_idx[i]] * net.product_stoich[i])
+                       - np.sum(g[net.reactant_idx[i]] * net.reactant_stoich[i]))
     return A, b, E, logK
 
 
@@ -223,9 +236,10 @@ def rates_from_params(params, Tco: np.ndarray, n_photo: int = 0):
     return k
 
 
-def random_rates(rng, Tco: np.ndarray, n_rev: int, n_photo: int = 0):
+def random_rates(rng, Tco: np.ndarray, n_rev: int, n_photo: int = 0, net=None,
+                 consistent: bool = False):
     """k (nr+1, nz) with freshly drawn parameters."""
-    return rates_from_params(random_rate_params(rng, n_rev), Tco, n_photo)
+    return rates_from_params(random_rate_params(rng, n_rev, net, consistent), Tco, n_photo)
 
 
 # --------------------------------------------------------------------------
@@ -300,7 +314,7 @@ def random_photo(rng, ni, parents, rows, nbin=2500):
 
 def make_column(seed: int, nz: int, ni: int, n_rev: int, n_photo: int = 0,
                 nbin: int = 2500, dt: float = 1e-6, name: str = "column",
-                p_bot: float = 1e9, p_top: float = 1e-2) -> Column:
+                p_bot: float = 1e9, p_top: float = 1e-2, consistent: bool = False) -> Column:
     """Draw one column with the workload-1 distributions."""
     rng = np.random.default_rng(seed)
     P = np.logspace(np.log10(p_bot), np.log10(p_top), nz)
@@ -311,7 +325,7 @@ def make_column(seed: int, nz: int, ni: int, n_rev: int, n_photo: int = 0,
     ymix /= ymix.sum(axis=1, keepdims=True)
     y = M[:, None] * ymix
     net, parents, rows = random_network(rng, ni, n_rev, n_photo)
-    k = random_rates(rng, Tco, n_rev, n_photo)
+    k = random_rates(rng, Tco, n_rev, n_photo, net, consistent)
     Kzz = 10.0 ** rng.uniform(5.0, 10.0, size=nz - 1)
     atm = AtmStatic(
         Kzz=Kzz, Dzz=np.zeros((nz - 1, ni)), dzi=geo["dzi"], vz=np.zeros(nz - 1),

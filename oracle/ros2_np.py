@@ -673,3 +673,73 @@ def newton_steady_state(y0, k, atm, net, tol=1e-9, max_iter=100):
         diag, sup_d, sub_d = jacobian_blocks(y, k, atm, net)
         y = y + block_thomas_diag_offdiag(diag, sup_d, sub_d, -res)
     raise RuntimeError(f"Newton did not converge: {nrm:.3e}")
+
+
+# ---------------------------------------------------------------------------
+# steady-state driver (outer_loop.py body_fn / cond_fn subset)
+# ---------------------------------------------------------------------------
+
+class RunParams(NamedTuple):
+    """Stopping rule of the adaptive run: accepted-step relative change below
+    conv_tol once ready (t > trun_min and count > count_min, outer_loop.py:
+    706-711), or count_max accepted steps (outer_loop.py:673-675)."""
+    conv_tol: float = 1e-4
+    count_min: int = 120
+    trun_min: float = 1e2
+    count_max: int = 5000
+    photo_frq: int = 5
+
+
+class PhotoState(NamedTuple):
+    ph: object          # synth.PhotoSetup
+    dz: np.ndarray
+    dflux_u: np.ndarray
+
+
+def photo_update(y, ymix, k, pst: PhotoState):
+    """tau -> flux -> J -> k rows (outer_loop.py:253-352)."""
+    ph = pst.ph
+    tau = compute_tau(y, pst.dz, ph.absp_idx, ph.absp_cross, ph.scat_idx, ph.scat_cross)
+    af, sf, dd, du = compute_flux(tau, ph.sflux_top, ymix, ph.absp_idx, ph.absp_cross,
+                                  ph.scat_idx, ph.scat_cross, ph.bins, ph.mu_zenith, ph.edd,
+                                  ph.ag0, ph.hc, pst.dflux_u)
+    J = compute_J(af, ph.cross_J, ph.din12_indx, ph.dbin1, ph.dbin2)
+    k = update_k_with_J(k, J, ph.branch_re_idx, ph.branch_active, ph.f_diurnal)
+    return k, pst._replace(dflux_u=du), dict(tau=tau, aflux=af, J=J)
+
+
+def steady_state_run(y0, k, atm, net, dt0, p: StepParams = StepParams(),
+                     rp: RunParams = RunParams(), compo=None, photo: PhotoState | None = None,
+                     max_attempts=None, trace=None):
+    """Adaptive Ros2 run of one column until the stopping rule; returns
+    (y, info).  Photolysis, when given, refreshes k before an attempt with
+    retry_count == 0 and accept_count % photo_frq == 0 (outer_loop.py:716-724)."""
+    nz, ni = y0.shape
+    if compo is None:
+        compo = np.ones((ni, 1))
+    s, ini = init_control(y0, dt0, compo)
+    n0 = atm.M
+    attempts = 0
+    rel = np.inf
+    status = "max_steps"
+    while True:
+        if photo is not None and s.retry_count == 0 and s.accept_count % rp.photo_frq == 0:
+            k, photo, _ = photo_update(s.y, s.ymix, k, photo)
+        sol, delta = ros2_step(s.y, k, s.dt, atm, net)
+        s_new, d = control_update(s, sol, delta, n0, compo, ini, p)
+        attempts += 1
+        if d["do_accept"]:
+            rel = rel_change(s_new.y, s.y_prev, s_new.ymix, p.mtol, p.atol)
+        if trace is not None:
+            trace.append(dict(dt=s.dt, accept=d["do_accept"], delta=s_new.delta, rel=rel))
+        s = s_new
+        ready = (s.t > rp.trun_min) and (s.accept_count > rp.count_min)
+        if d["do_accept"] and ready and rel < rp.conv_tol:
+            status = "converged"
+            break
+        if s.accept_count >= rp.count_max:
+            break
+        if max_attempts is not None and attempts >= max_attempts:
+            status = "max_attempts"
+            break
+    return s.y, dict(state=s, attempts=attempts, rel=rel, status=status, k=k)
