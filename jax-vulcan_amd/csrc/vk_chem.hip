@@ -366,6 +366,25 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
   }
 }
 
+// Largest dynamic LDS a launch of fn may use; raises the function's limit to it.
+static int raise_lds_limit(const void* fn) {
+  int dev = 0, per_block = 64 * 1024;
+  hipFuncAttributes fa{};
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+          hipSuccess ||
+      hipFuncGetAttributes(&fa, fn) != hipSuccess) {
+    (void)hipGetLastError();
+    return 64 * 1024 - 2048;
+  }
+  const int lim = per_block - (int)fa.sharedSizeBytes;
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lim) != hipSuccess) {
+    (void)hipGetLastError();
+    return 64 * 1024 - (int)fa.sharedSizeBytes;
+  }
+  return lim;
+}
+
 int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max_nz, int max_S,
                          int max_T, const VkNetDev* nets, const double* y, const double* M,
                          const double* atm, const double* k, const double* dt, const uint8_t* fix,
@@ -383,17 +402,18 @@ int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max
     return VK_ERR_UNSUPPORTED;
   }
   dim3 grid(max_nz, ncols);
-  // The LDS limit is raised once per instantiation, outside any graph capture
+  // The dynamic-LDS limit of each instantiation is raised once (to what the
+  // device allows beyond the kernel's static LDS), outside any graph capture
   // that later launches may be part of.
-  static bool raised[8] = {false};
+  static int limit[8] = {0};
   switch (sl) {
 #define VK_CASE(n)                                                                        \
   case n:                                                                                 \
-    if (!raised[n]) {                                                                     \
-      VK_CHECK_HIP(hipFuncSetAttribute((const void*)k_chem_layer<n>,                      \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,        \
-                                       160 * 1024));                                      \
-      raised[n] = true;                                                                   \
+    if (!limit[n]) limit[n] = raise_lds_limit((const void*)k_chem_layer<n>);             \
+    if ((int)shmem > limit[n]) {                                                          \
+      vk_set_error("chem layer needs %zu bytes of dynamic LDS, limit %d", shmem,         \
+                   limit[n]);                                                             \
+      return VK_ERR_UNSUPPORTED;                                                          \
     }                                                                                     \
     hipLaunchKernelGGL(k_chem_layer<n>, grid, dim3(VK_NT), shmem, stream, L);             \
     break;
