@@ -131,19 +131,22 @@ int vk_jacobian_blocks_f64(vk_batch b, const double* y, const double* k, const d
 
 typedef struct {
   double rtol, dt_var_min, dt_var_max, dt_min, dt_max, atol, mtol, pos_cut, nega_cut, loss_eps;
-  double trun_min, conv_tol;
+  double trun_min, conv_tol, flux_cri;
   int32_t batch_max_retries, count_min, count_max, na, zero_bot_row, photo_frq;
 } vk_ss_params;
 
 /* Device state of a batch of runs.  State-shaped arrays at off_s; compo is
  * the (ni, na) atom composition of each column at off_s; atom arrays are
- * (ncols, VK_MAX_ATOMS); the rest are (ncols).  photo_due may be NULL. */
+ * (ncols, VK_MAX_ATOMS); the rest are (ncols).  photo_due and aflux_change
+ * may be NULL; with aflux_change, convergence also needs aflux_change <
+ * flux_cri (cond_fn, outer_loop.py:688-689). */
 typedef struct {
   double *y, *y_prev, *ymix, *sol, *delta_arr;
   const double* compo;
   double *atom_ini, *atom_loss, *atom_loss_prev;
   double *dt, *t, *delta, *rel;
   int32_t *accept, *retry, *nega, *loss, *dcount, *status, *attempts, *photo_due;
+  double* aflux_change;
 } vk_ss_state;
 
 /* ymix, y_prev, atom_ini, zeroed counters and status; y and dt must be set. */
@@ -154,6 +157,35 @@ int vk_ss_init_f64(vk_batch b, const double* atm, const vk_ss_params* p, const v
  * allocation: safe to capture in a hipGraph and replay. */
 int vk_ss_attempt_f64(vk_batch b, const double* k, const double* atm, const vk_ss_params* p,
                       const vk_ss_state* s, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- photolysis (photo.py) -----------------------------------------------
+ * One cross-section / stellar-flux setup shared by every column of a call.
+ * Per-column arrays: dz (nz) at off_l[c]; tau, w0, sflux, dflux_d, dflux_u,
+ * aflux (nz+1, nbin) at off_p[c]; J (n_br, nz) at off_l[c]*n_br; k rows at
+ * off_k (S, nz).  Replaces compute_tau_jax (photo.py:84-123),
+ * compute_flux_jax (photo.py:126-245; ag0 = 0 and ag0 != 0 branches),
+ * compute_J_jax_flat (photo.py:309-351, 377-392) and update_k_with_J
+ * (photo.py:455-478); no temperature-dependent cross sections. */
+typedef struct {
+  int32_t nbin, n_abs, n_scat, n_br, din12, ag0_zero;
+  double dbin1, dbin2, mu_zenith, edd, ag0, hc, f_diurnal, flux_atol;
+  const double *bins, *sflux_top, *absp_cross, *scat_cross, *cross_J, *dz;
+  const int32_t *absp_idx, *scat_idx, *branch_re_idx, *branch_active;
+  const int64_t *off_p, *off_l;
+} vk_photo_setup;
+
+/* due (may be NULL): per-column flag, columns with 0 are skipped; status
+ * (may be NULL): columns with a DONE bit are skipped; aflux_change (may be
+ * NULL): per-column max relative aflux change (outer_loop.py:330-336). */
+typedef struct {
+  double *tau, *w0, *sflux, *dflux_d, *dflux_u, *aflux, *J, *aflux_change;
+  const int32_t *due, *status;
+} vk_photo_state;
+
+/* parts: 1 = optics (tau, w0 from y), 2 = flux sweeps, 4 = J (+ k rows when
+ * k != NULL); OR them for the in-loop update. */
+int vk_photo_f64(vk_batch b, int parts, const vk_photo_setup* setup, const vk_photo_state* state,
+                 const double* y, double* k, void* stream);
 
 /* Kernel timing with HIP events recorded on the launch stream around each
  * kernel of vk_ros2_step_f64 (no counterpart in the reference, which has no

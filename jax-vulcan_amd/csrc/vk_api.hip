@@ -90,6 +90,10 @@ int vk_launch_ss_control(const VkCol* cols, int ncols, const double* atm, const 
 int vk_launch_ss_init(const VkCol* cols, int ncols, const double* atm, const vk_ss_params& p,
                       const vk_ss_state& s, hipStream_t st);
 
+int vk_launch_photo(int parts, const VkCol* cols, int ncols, int max_nz,
+                    const vk_photo_setup& P, const vk_photo_state& S, const double* y, double* k,
+                    hipStream_t st);
+
 struct vk_network_s {
   VkNetDev dev;      // device pointers into buf
   void* buf = nullptr;
@@ -615,6 +619,25 @@ int vk_ss_attempt_f64(vk_batch b, const double* k, const double* atm, const vk_s
     return rc;
   KernelTimer t(5, st);
   return vk_launch_ss_control(b->d_cols, b->ncols, atm, *p, *s, st);
+}
+
+int vk_photo_f64(vk_batch b, int parts, const vk_photo_setup* setup, const vk_photo_state* state,
+                 const double* y, double* k, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  if (!setup || !state || !setup->off_p || !setup->off_l || !setup->bins || !setup->sflux_top ||
+      !state->tau || !state->w0 || !state->sflux || !state->dflux_d || !state->dflux_u ||
+      !state->aflux || !state->J || ((parts & 1) && (!y || !setup->dz))) {
+    vk_set_error("vk_photo_f64: null pointer in arguments");
+    return VK_ERR_ARG;
+  }
+  if (setup->nbin < 2 || setup->n_br < 0 || (setup->n_abs > 0 && !setup->absp_cross) ||
+      (setup->n_br > 0 && (!setup->cross_J || !setup->branch_re_idx || !setup->branch_active))) {
+    vk_set_error("vk_photo_f64: inconsistent photolysis setup");
+    return VK_ERR_ARG;
+  }
+  return vk_launch_photo(parts, b->d_cols, b->ncols, b->max_nz, *setup, *state, y, k,
+                         (hipStream_t)stream);
 }
 
 }  // extern "C"

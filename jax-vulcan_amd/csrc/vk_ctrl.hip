@@ -208,7 +208,8 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
     if (do_accept) {
       s.rel[c] = relmax;
       const bool ready = (t_new > p.trun_min) && (acc > p.count_min);
-      if (ready && relmax < p.conv_tol) st |= VK_ST_CONVERGED;
+      const bool flux_ok = !s.aflux_change || s.aflux_change[c] < p.flux_cri;
+      if (ready && relmax < p.conv_tol && flux_ok) st |= VK_ST_CONVERGED;
     }
     if (acc >= p.count_max) st |= VK_ST_MAXSTEPS;
     if (flags & 2) st |= VK_ST_NONFINITE;

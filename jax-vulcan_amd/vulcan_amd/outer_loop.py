@@ -32,7 +32,7 @@ ST_DONE = ST_CONVERGED | ST_MAXSTEPS
 class SsParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "rtol", "dt_var_min", "dt_var_max", "dt_min", "dt_max", "atol", "mtol", "pos_cut",
-        "nega_cut", "loss_eps", "trun_min", "conv_tol")] + [(n, C.c_int32) for n in (
+        "nega_cut", "loss_eps", "trun_min", "conv_tol", "flux_cri")] + [(n, C.c_int32) for n in (
             "batch_max_retries", "count_min", "count_max", "na", "zero_bot_row", "photo_frq")]
 
 
@@ -40,7 +40,7 @@ class SsState(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in (
         "y", "y_prev", "ymix", "sol", "delta_arr", "compo", "atom_ini", "atom_loss",
         "atom_loss_prev", "dt", "t", "delta", "rel", "accept", "retry", "nega", "loss",
-        "dcount", "status", "attempts", "photo_due")]
+        "dcount", "status", "attempts", "photo_due", "aflux_change")]
 
 
 @dataclass
@@ -61,12 +61,13 @@ class RunConfig:
     count_min: int = 120
     count_max: int = 5000
     conv_tol: float = 1e-4
+    flux_cri: float = 0.1
     photo_frq: int = 5
 
     def c_params(self, na: int, zero_bot_row: bool = False) -> SsParams:
         p = SsParams()
         for f in ("rtol", "dt_var_min", "dt_var_max", "dt_min", "dt_max", "atol", "mtol",
-                  "pos_cut", "nega_cut", "loss_eps", "trun_min", "conv_tol"):
+                  "pos_cut", "nega_cut", "loss_eps", "trun_min", "conv_tol", "flux_cri"):
             setattr(p, f, float(getattr(self, f)))
         p.batch_max_retries = int(self.batch_max_retries)
         p.count_min = int(self.count_min)
@@ -120,16 +121,17 @@ class SteadyStateRunner:
          self.attempts, self.photo_due) = (zi() for _ in range(8))
         zero_bot = any(bool(c.atm.use_botflux) for c in cols)
         self.params = cfg.c_params(self.na, zero_bot)
-        self.state = SsState(*[t.data_ptr() for t in (
-            self.y, self.y_prev, self.ymix, self.sol, self.delta_arr, self.compo, self.atom_ini,
-            self.atom_loss, self.atom_loss_prev, self.dt, self.t, self.delta, self.rel,
-            self.accept, self.retry, self.nega, self.loss, self.dcount, self.status,
-            self.attempts, self.photo_due)])
         self.ws = b.workspace()
         self.photo = None
         if photo is not None or any(getattr(c, "photo", None) is not None for c in cols):
             from .photo import PhotoBatch
             self.photo = PhotoBatch(self, cols)
+        self.state = SsState(*[t.data_ptr() for t in (
+            self.y, self.y_prev, self.ymix, self.sol, self.delta_arr, self.compo, self.atom_ini,
+            self.atom_loss, self.atom_loss_prev, self.dt, self.t, self.delta, self.rel,
+            self.accept, self.retry, self.nega, self.loss, self.dcount, self.status,
+            self.attempts, self.photo_due)], None if self.photo is None
+            else self.photo.state.aflux_change.data_ptr())
         self.graph = None
         self._init()
 

@@ -561,14 +561,21 @@ def compute_flux(tau, sflux_top, ymix, absp_idx, absp_cross, scat_idx, scat_cros
     w0 = np.minimum(w0, 1.0 - 1e-8)
     sflux = sflux_top[None, :] * np.exp(tau / mu_ang)
     dir_flux = sflux * (-mu_ang)
-    if ag0 != 0.0:
-        raise NotImplementedError("only the ag0 = 0 branch is restated")
-    tran = np.exp(-1.0 / edd * (1.0 - w0) ** 0.5 * dtau)
-    zeta_p = 0.5 * (1.0 + (1.0 - w0) ** 0.5)
-    zeta_m = 0.5 * (1.0 - (1.0 - w0) ** 0.5)
-    ll = -1.0 * w0 / (1.0 / mu_ang ** 2 - 1.0 / edd ** 2 * (1.0 - w0))
-    g_p = 0.5 * (ll * (1.0 / edd + 1.0 / mu_ang))
-    g_m = 0.5 * (ll * (1.0 / edd - 1.0 / mu_ang))
+    if ag0 == 0.0:
+        tran = np.exp(-1.0 / edd * (1.0 - w0) ** 0.5 * dtau)
+        zeta_p = 0.5 * (1.0 + (1.0 - w0) ** 0.5)
+        zeta_m = 0.5 * (1.0 - (1.0 - w0) ** 0.5)
+        ll = -1.0 * w0 / (1.0 / mu_ang ** 2 - 1.0 / edd ** 2 * (1.0 - w0))
+        g_p = 0.5 * (ll * (1.0 / edd + 1.0 / mu_ang))
+        g_m = 0.5 * (ll * (1.0 / edd - 1.0 / mu_ang))
+    else:  # photo.py:183-197
+        wg = 1.0 - w0 * ag0
+        tran = np.exp(-1.0 / edd * ((1.0 - w0 * ag0) * (1.0 - w0)) ** 0.5 * dtau)
+        zeta_p = 0.5 * (1.0 + ((1.0 - w0) / wg) ** 0.5)
+        zeta_m = 0.5 * (1.0 - ((1.0 - w0) / wg) ** 0.5)
+        ll = ((1.0 - w0) * wg - 1.0) / (1.0 / mu_ang ** 2 - 1.0 / edd ** 2 * (1.0 - w0) * wg)
+        g_p = 0.5 * (ll * (1.0 / edd + 1.0 / (mu_ang * wg)) + w0 * ag0 * mu_ang / wg)
+        g_m = 0.5 * (ll * (1.0 / edd - 1.0 / (mu_ang * wg)) - w0 * ag0 * mu_ang / wg)
     ll = np.clip(ll, -1e10, 1e10)  # clipped after g_p/g_m, as photo.py:199
     chi = zeta_m ** 2 * tran ** 2 - zeta_p ** 2
     xi = zeta_p * zeta_m * (1.0 - tran ** 2)
@@ -682,22 +689,32 @@ def newton_steady_state(y0, k, atm, net, tol=1e-9, max_iter=100):
 class RunParams(NamedTuple):
     """Stopping rule of the adaptive run: accepted-step relative change below
     conv_tol once ready (t > trun_min and count > count_min, outer_loop.py:
-    706-711), or count_max accepted steps (outer_loop.py:673-675)."""
+    706-711) and, with photolysis, aflux_change < flux_cri (:688-689); or
+    count_max accepted steps (outer_loop.py:673-675)."""
     conv_tol: float = 1e-4
     count_min: int = 120
     trun_min: float = 1e2
     count_max: int = 5000
     photo_frq: int = 5
+    flux_cri: float = 0.1
+    flux_atol: float = 1.0
 
 
 class PhotoState(NamedTuple):
     ph: object          # synth.PhotoSetup
     dz: np.ndarray
     dflux_u: np.ndarray
+    aflux: np.ndarray
+    aflux_change: float = 0.0
 
 
-def photo_update(y, ymix, k, pst: PhotoState):
-    """tau -> flux -> J -> k rows (outer_loop.py:253-352)."""
+def photo_state(ph, dz, nz):
+    nbin = ph.bins.shape[0]
+    return PhotoState(ph, dz, np.zeros((nz + 1, nbin)), np.zeros((nz, nbin)), 0.0)
+
+
+def photo_update(y, ymix, k, pst: PhotoState, flux_atol=1.0):
+    """tau -> flux -> J -> k rows, aflux_change (outer_loop.py:253-352)."""
     ph = pst.ph
     tau = compute_tau(y, pst.dz, ph.absp_idx, ph.absp_cross, ph.scat_idx, ph.scat_cross)
     af, sf, dd, du = compute_flux(tau, ph.sflux_top, ymix, ph.absp_idx, ph.absp_cross,
@@ -705,7 +722,10 @@ def photo_update(y, ymix, k, pst: PhotoState):
                                   ph.ag0, ph.hc, pst.dflux_u)
     J = compute_J(af, ph.cross_J, ph.din12_indx, ph.dbin1, ph.dbin2)
     k = update_k_with_J(k, J, ph.branch_re_idx, ph.branch_active, ph.f_diurnal)
-    return k, pst._replace(dflux_u=du), dict(tau=tau, aflux=af, J=J)
+    mask = af > flux_atol
+    ratio = np.where(mask, np.abs(af - pst.aflux) / np.maximum(np.abs(af), 1e-300), 0.0)
+    change = float(np.max(ratio)) if np.any(mask) else 0.0
+    return k, pst._replace(dflux_u=du, aflux=af, aflux_change=change), dict(tau=tau, aflux=af, J=J)
 
 
 def steady_state_run(y0, k, atm, net, dt0, p: StepParams = StepParams(),
@@ -724,7 +744,7 @@ def steady_state_run(y0, k, atm, net, dt0, p: StepParams = StepParams(),
     status = "max_steps"
     while True:
         if photo is not None and s.retry_count == 0 and s.accept_count % rp.photo_frq == 0:
-            k, photo, _ = photo_update(s.y, s.ymix, k, photo)
+            k, photo, _ = photo_update(s.y, s.ymix, k, photo, rp.flux_atol)
         sol, delta = ros2_step(s.y, k, s.dt, atm, net)
         s_new, d = control_update(s, sol, delta, n0, compo, ini, p)
         attempts += 1
@@ -734,7 +754,8 @@ def steady_state_run(y0, k, atm, net, dt0, p: StepParams = StepParams(),
             trace.append(dict(dt=s.dt, accept=d["do_accept"], delta=s_new.delta, rel=rel))
         s = s_new
         ready = (s.t > rp.trun_min) and (s.accept_count > rp.count_min)
-        if d["do_accept"] and ready and rel < rp.conv_tol:
+        flux_ok = photo is None or photo.aflux_change < rp.flux_cri
+        if d["do_accept"] and ready and rel < rp.conv_tol and flux_ok:
             status = "converged"
             break
         if s.accept_count >= rp.count_max:

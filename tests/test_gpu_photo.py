@@ -1,0 +1,130 @@
+"""GPU photolysis vs the CPU oracle: tau, two-stream flux, J, k rows, in-loop runs.
+
+Tolerances: tau 1e-13 (positive sums, order differs only); flux 1e-9
+(device exp/sqrt vs libm carried through the two sweeps; with a random
+prior dflux_u of ~1e10 the down-sweep terms cancel, measured 3-5e-10 on
+dflux_d and 3e-11 on aflux); J 1e-12; the J known-answer test of
+test_photo_ion.py:20-41 at 1e-14 as the reference asks.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _photo_col(seed=41, nz=30, ni=20, n_rev=50, n_photo=8, nbin=300):
+    from vulcan_amd import synth
+    return synth.make_column(seed, nz=nz, ni=ni, n_rev=n_rev, n_photo=n_photo, nbin=nbin,
+                             dt=1e-10, name="ph", p_bot=1e5, p_top=1e-2, consistent=True)
+
+
+def _pd(ph):
+    from vulcan_amd.photo import PhotoData
+    return PhotoData(ph.absp_idx, ph.absp_cross, np.zeros(0, np.int64), np.zeros((0, 1, 1)),
+                     ph.scat_idx, ph.scat_cross)
+
+
+def test_tau_matches_oracle():
+    from oracle import ros2_np as O
+    from vulcan_amd import photo
+    c = _photo_col()
+    ph = c.photo
+    tau = _np(photo.compute_tau_jax(c.y, c.dz, _pd(ph)))
+    ref = O.compute_tau(c.y, c.dz, ph.absp_idx, ph.absp_cross, ph.scat_idx, ph.scat_cross)
+    assert np.max(np.abs(tau - ref) / np.maximum(np.abs(ref), 1e-300)) < 1e-13
+
+
+@pytest.mark.parametrize("ag0", [0.0, 0.3])
+def test_flux_matches_oracle(ag0):
+    from oracle import ros2_np as O
+    from vulcan_amd import photo
+    c = _photo_col()
+    ph = c.photo
+    ymix = c.y / c.y.sum(1, keepdims=True)
+    tau = O.compute_tau(c.y, c.dz, ph.absp_idx, ph.absp_cross, ph.scat_idx, ph.scat_cross)
+    rng = np.random.default_rng(0)
+    prev = rng.uniform(0, 1e10, (c.nz + 1, ph.bins.shape[0]))
+    got = photo.compute_flux_jax(tau, ph.sflux_top, ymix, _pd(ph), ph.bins, ph.mu_zenith,
+                                 ph.edd, ag0, ph.hc, prev, ag0_is_zero=(ag0 == 0.0))
+    ref = O.compute_flux(tau, ph.sflux_top, ymix, ph.absp_idx, ph.absp_cross, ph.scat_idx,
+                         ph.scat_cross, ph.bins, ph.mu_zenith, ph.edd, ag0, ph.hc, prev)
+    for name, g, r in zip(("aflux", "sflux", "dflux_d", "dflux_u"), got, ref):
+        g = _np(g)
+        scale = np.maximum(np.abs(r), 1e-300 + 1e-12 * np.abs(r).max())
+        err = np.max(np.abs(g - r) / scale)
+        print(f"{name}: {err:.3e}")
+        assert err < 1e-9, name
+
+
+def test_J_known_answer_photo_ion():
+    """test_photo_ion.py:29-41 arrays against its _manual_jion formula."""
+    from vulcan_amd import photo
+    aflux = np.array([[1.0, 2.0, 4.0, 8.0, 16.0], [0.5, 1.0, 1.5, 2.0, 2.5]])
+    cross = np.array([[1.0, 0.8, 0.6, 0.4, 0.2], [0.3, 0.2, 0.1, 0.05, 0.02],
+                      [0.0, 0.1, 0.2, 0.3, 0.4]])
+    split, d1, d2 = 3, 0.1, 1.5
+
+    def manual(cr):
+        left = np.sum(aflux[:, :split] * cr[:split], axis=1) * d1
+        left -= 0.5 * (aflux[:, 0] * cr[0] + aflux[:, split - 1] * cr[split - 1]) * d1
+        right = np.sum(aflux[:, split:] * cr[split:], axis=1) * d2
+        right -= 0.5 * (aflux[:, split] * cr[split] + aflux[:, -1] * cr[-1]) * d2
+        return left + right
+
+    J, JT = photo.compute_J_jax_flat(aflux, cross, np.zeros((0, 2, 5)), split, d1, d2)
+    J = _np(J)
+    for b in range(3):
+        want = manual(cross[b])
+        assert np.max(np.abs(J[b] - want) / np.maximum(np.abs(want), 1e-300)) < 1e-14
+    k = _np(photo.update_k_with_J(np.zeros((10, 2)), J, np.zeros((0, 2)), [7, 8, 9],
+                                  [True, True, True], [], [], 1.0))
+    np.testing.assert_array_equal(k[7:10], J)
+
+
+def test_J_matches_oracle_on_column():
+    from oracle import ros2_np as O
+    from vulcan_amd import photo
+    c = _photo_col()
+    ph = c.photo
+    rng = np.random.default_rng(1)
+    af = rng.uniform(0, 1e14, (c.nz, ph.bins.shape[0]))
+    J = _np(photo.compute_J_jax_flat(af, ph.cross_J, np.zeros((0, 1, 1)), ph.din12_indx,
+                                     ph.dbin1, ph.dbin2)[0])
+    ref = O.compute_J(af, ph.cross_J, ph.din12_indx, ph.dbin1, ph.dbin2)
+    assert np.max(np.abs(J - ref) / np.abs(ref)) < 1e-12
+
+
+def test_in_loop_photolysis_steady_state_matches_oracle():
+    """Photolysis refreshed in the loop (tau -> flux -> J -> k, cadence
+    photo_frq, lagged dflux_u) to convergence: mixing ratios within 1e-6."""
+    from oracle import ros2_np as O
+    from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
+    c = _photo_col(seed=42, nz=20, ni=14, n_rev=40, n_photo=5, nbin=200)
+    pst = O.photo_state(c.photo, c.dz, c.nz)
+    yo, info = O.steady_state_run(c.y, c.k.copy(), c.atm, c.net, c.dt, photo=pst)
+    assert info["status"] == "converged"
+    r = SteadyStateRunner([c], RunConfig())
+    r.run(max_attempts=20000, check_every=100)
+    res = r.results()
+    print(f"oracle {info['attempts']} attempts / gpu {res['attempts'][0]}")
+    assert res["status"][0] & 4
+    mo = info["state"].ymix
+    m = mo > 1e-12
+    err = float(np.max(np.abs(res["ymix"][0][m] - mo[m]) / mo[m]))
+    print(f"photo steady-state mixing ratio error {err:.3e}")
+    assert err < 1e-6
+    # the photolysis rows of k on the device equal the oracle's final J * f_diurnal
+    kd = r.k.view(-1).cpu().numpy().reshape(c.k.shape)
+    rows = c.photo.branch_re_idx
+    ko = info["k"][rows]
+    scale = np.abs(ko).max(axis=1, keepdims=True)
+    assert np.max(np.abs(kd[rows] - ko) / scale) < 1e-6
