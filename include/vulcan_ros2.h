@@ -26,6 +26,8 @@
  *   vk_bt_solve_f64      solver.solve_block_thomas_diag_offdiag  solver.py:67-112
  *   vk_ros2_step_f64     jax_step.jax_ros2_step                jax_step.py:249-315
  *   vk_residual_f64      steady_state_grad.steady_state_residual steady_state_grad.py:116-150
+ *   vk_ss_vjp_f64        steady_state_grad._ssi_bwd / _ss_bwd parameter VJP
+ *                                                               steady_state_grad.py:214-233, 306-327
  *   vk_jacobian_blocks_f64 steady_state_grad._build_jacobian_blocks
  *                                                               steady_state_grad.py:153-175
  */
@@ -121,6 +123,18 @@ int vk_residual_f64(vk_batch b, const double* y, const double* k, const double* 
 /* J_y blocks without c0: diag (natural, off_s*ni), sup_d = B+Bi, sub_d = C+Ci. */
 int vk_jacobian_blocks_f64(vk_batch b, const double* y, const double* k, const double* atm,
                            double* diag, double* sup_d, double* sub_d, void* stream);
+
+/* Parameter cotangents of the implicit gradient, the VJP of _ssi_bwd and
+ * _ss_bwd through f = chem_rhs + diffusion (steady_state_grad.py:214-233,
+ * 306-327), negated as the implicit-function theorem needs:
+ *   cot_k   (S, nz) at off_k:     -(d f / d k)^T lam
+ *   cot_atm record layout at off_a: -(d f / d theta)^T lam for every
+ *           floating leaf of the atmosphere record (Kzz, dzi, vz, Hpi, Ti,
+ *           Tco, g, M, ms, alpha, top_flux, bot_flux, bot_vdep, Dzz, vm,
+ *           vs); the gas mask and the mode flags get 0.
+ * lam at off_s (the adjoint state J_y^{-T} v).  Either output may be NULL. */
+int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* atm,
+                  const double* lam, double* cot_k, double* cot_atm, void* stream);
 
 /* ---- adaptive steady-state runs with the step control on the device ----
  * Replaces the accept/reject/dt logic of outer_loop.body_fn

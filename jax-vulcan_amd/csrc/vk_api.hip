@@ -94,6 +94,10 @@ int vk_launch_photo(int parts, const VkCol* cols, int ncols, int max_nz,
                     const vk_photo_setup& P, const vk_photo_state& S, const double* y, double* k,
                     hipStream_t st);
 
+int vk_launch_ss_vjp(const VkCol* cols, int ncols, int max_nz, int max_ni, int64_t max_rec,
+                     const VkNetDev* nets, const double* y, const double* atm, const double* k,
+                     const double* lam, double* cot_k, double* cot_atm, hipStream_t st);
+
 struct vk_network_s {
   VkNetDev dev;      // device pointers into buf
   void* buf = nullptr;
@@ -106,7 +110,8 @@ struct vk_batch_s {
   VkNetDev* d_nets = nullptr;
   std::vector<VkCol> cols;
   std::vector<int> slots;   // distinct SL buckets
-  int max_nz = 0, max_S = 0, max_T = 0;
+  int max_nz = 0, max_S = 0, max_T = 0, max_ni = 0;
+  int64_t max_rec = 0;
   int64_t state_doubles = 0, block_doubles = 0;
   bool uniform = false;
 };
@@ -148,8 +153,8 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
     return VK_ERR_UNSUPPORTED;
   }
   const int S = nr + 1;
-  std::vector<int> r_idx(S * T);
-  std::vector<double> r_st(S * T);
+  std::vector<int> r_idx(S * T), p_idx(S * T);
+  std::vector<double> r_st(S * T), p_st(S * T);
   std::vector<int> is3(S, 0);
   for (int r = 0; r < S; ++r) {
     for (int s = 0; s < T; ++s) {
@@ -166,6 +171,8 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
       }
       r_idx[r * T + s] = (int)ri;
       r_st[r * T + s] = st;
+      p_idx[r * T + s] = (int)pi;
+      p_st[r * T + s] = product_stoich[r * T + s];
     }
     is3[r] = is_three_body ? (is_three_body[r] != 0) : 0;
   }
@@ -211,6 +218,7 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
   // one device allocation for every table
   size_t off = 0;
   auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+  const size_t o_pidx = take(p_idx.size() * 4), o_pst = take(p_st.size() * 8);
   const size_t o_ridx = take(r_idx.size() * 4), o_rst = take(r_st.size() * 8),
                o_is3 = take(is3.size() * 4), o_lp = take(lp.size() * 4),
                o_li = take(std::max<size_t>(li.size(), 1) * 4),
@@ -221,6 +229,8 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
                o_jc = take(std::max<size_t>(jc.size(), 1) * 8);
   std::vector<char> host(off, 0);
   auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) memcpy(&host[o], src, bytes); };
+  put(o_pidx, p_idx.data(), p_idx.size() * 4);
+  put(o_pst, p_st.data(), p_st.size() * 8);
   put(o_ridx, r_idx.data(), r_idx.size() * 4);
   put(o_rst, r_st.data(), r_st.size() * 8);
   put(o_is3, is3.data(), is3.size() * 4);
@@ -249,6 +259,8 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
   n->dev.ni = ni; n->dev.S = S; n->dev.T = T; n->dev.pad = 0;
   n->dev.r_idx = (const int*)(b + o_ridx);
   n->dev.r_st = (const double*)(b + o_rst);
+  n->dev.p_idx = (const int*)(b + o_pidx);
+  n->dev.p_st = (const double*)(b + o_pst);
   n->dev.is3 = (const int*)(b + o_is3);
   n->dev.loss_ptr = (const int*)(b + o_lp);
   n->dev.loss_item = (const int*)(b + o_li);
@@ -309,6 +321,8 @@ int vk_batch_create(int ncols, const int32_t* nz, const int32_t* ni, const int32
     cl.off_s = off_s[c]; cl.off_k = off_k[c]; cl.off_a = off_a[c]; cl.off_b = off_b[c];
     sls.insert(vk_slots(ni[c]));
     b->max_nz = std::max(b->max_nz, (int)nz[c]);
+    b->max_ni = std::max(b->max_ni, (int)ni[c]);
+    b->max_rec = std::max(b->max_rec, vk_atm_offsets((int)nz[c], (int)ni[c]).total);
     b->max_S = std::max(b->max_S, (int)S[c]);
     b->max_T = std::max(b->max_T, nw->T);
     b->state_doubles = std::max<int64_t>(b->state_doubles, off_s[c] + (int64_t)nz[c] * ni[c]);
@@ -638,6 +652,18 @@ int vk_photo_f64(vk_batch b, int parts, const vk_photo_setup* setup, const vk_ph
   }
   return vk_launch_photo(parts, b->d_cols, b->ncols, b->max_nz, *setup, *state, y, k,
                          (hipStream_t)stream);
+}
+
+int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* atm,
+                  const double* lam, double* cot_k, double* cot_atm, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  if (!y || !k || !atm || !lam || (!cot_k && !cot_atm)) {
+    vk_set_error("vk_ss_vjp_f64: null pointer in arguments");
+    return VK_ERR_ARG;
+  }
+  return vk_launch_ss_vjp(b->d_cols, b->ncols, b->max_nz, b->max_ni, b->max_rec, b->d_nets, y,
+                          atm, k, lam, cot_k, cot_atm, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -71,6 +71,8 @@ struct VkNetDev {
   int ni, S, T, pad;
   const int* r_idx;      // (S*T) reactant species, PAD = ni
   const double* r_st;    // (S*T)
+  const int* p_idx;      // (S*T) product species, PAD = ni
+  const double* p_st;    // (S*T)
   const int* is3;        // (S) three-body flag
   const int* loss_ptr;   // (ni+1)
   const int* loss_item;  // row*T + slot

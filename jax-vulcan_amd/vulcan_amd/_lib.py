@@ -43,6 +43,7 @@ _SIGS = {
     "vk_ss_init_f64": (i32, [p, p, p, p, p]),
     "vk_ss_attempt_f64": (i32, [p, p, p, p, p, p, sz, p]),
     "vk_photo_f64": (i32, [p, i32, p, p, p, p, p]),
+    "vk_ss_vjp_f64": (i32, [p, p, p, p, p, p, p, p]),
     "vk_timing_enable": (i32, [i32]),
     "vk_timing_reset": (i32, []),
     "vk_timing_read": (i32, [p, p, i32]),
