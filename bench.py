@@ -39,6 +39,19 @@ def step_flops(nz: int, n: int, S: int) -> float:
     return factor_flops(nz, n) + f_sol + 44.0 * nz * S
 
 
+def factor_traffic(B: int):
+    """HBM bytes per factor launch from the committed rocprofv3 counter runs
+    (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a 1 GiB copy) at
+    1024 columns, scaled to B columns; None when the summary is absent."""
+    f = ROOT / "profiles" / "ros2_step_nz150_ni93_b1024_hbm_counters.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("factor_traffic_bytes_per_launch:"):
+            return int(float(line.split(":")[1]) * B / 1024)
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -49,6 +62,9 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=64,
                     help="column-steps timed on the host for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
+                    help="only run a plain elementwise copy of GIB GiB (reads GIB, writes "
+                         "GIB): the reference kernel that scales the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -59,6 +75,15 @@ def main() -> int:
     import numpy as np
     import torch
     from vulcan_amd import _lib, core, synth
+
+    if args.copy_probe > 0.0:
+        n = int(args.copy_probe * (1 << 30)) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n}))
+        return 0
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
@@ -135,7 +160,7 @@ def main() -> int:
     achieved = fac_flops / (fac_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-                "traffic": None}
+                "traffic": factor_traffic(B)}
 
     # ---- CPU baseline: C restatement of the reference step -------------------
     cpu = None

@@ -25,6 +25,7 @@
  *   vk_bt_factor_f64     solver.factor_block_thomas_diag_offdiag solver.py:33-64
  *   vk_bt_solve_f64      solver.solve_block_thomas_diag_offdiag  solver.py:67-112
  *   vk_ros2_step_f64     jax_step.jax_ros2_step                jax_step.py:249-315
+ *   vk_step_error_f64    integrate.jax_integrate_fixed_dt's delta  integrate.py:30-33
  *   vk_residual_f64      steady_state_grad.steady_state_residual steady_state_grad.py:116-150
  *   vk_ss_vjp_f64        steady_state_grad._ssi_bwd / _ss_bwd parameter VJP
  *                                                               steady_state_grad.py:214-233, 306-327
@@ -116,6 +117,11 @@ size_t vk_ros2_workspace_bytes(vk_batch b);
 int vk_ros2_step_f64(vk_batch b, const double* y, const double* k, const double* atm,
                      const double* dt, const uint8_t* fix_mask, double* sol, double* delta,
                      void* ws, size_t ws_bytes, int32_t* status, void* stream);
+
+/* Step error of the fixed-dt loop (integrate.py:30-33): out[c * stride] =
+ * max over column c of delta / max(|sol|, 1e-30); NaN propagates. */
+int vk_step_error_f64(vk_batch b, const double* sol, const double* delta, double* out,
+                      int64_t stride, void* stream);
 
 /* f(y) = chemistry + diffusion at y (off_s). */
 int vk_residual_f64(vk_batch b, const double* y, const double* k, const double* atm,

@@ -94,6 +94,8 @@ int vk_launch_photo(int parts, const VkCol* cols, int ncols, int max_nz,
                     const vk_photo_setup& P, const vk_photo_state& S, const double* y, double* k,
                     hipStream_t st);
 
+int vk_launch_delta_max(const VkCol* cols, int ncols, const double* sol, const double* delta,
+                        double* out, int64_t stride, hipStream_t st);
 int vk_launch_ss_vjp(const VkCol* cols, int ncols, int max_nz, int max_ni, int64_t max_rec,
                      const VkNetDev* nets, const double* y, const double* atm, const double* k,
                      const double* lam, double* cot_k, double* cot_atm, hipStream_t st);
@@ -664,6 +666,17 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
   }
   return vk_launch_ss_vjp(b->d_cols, b->ncols, b->max_nz, b->max_ni, b->max_rec, b->d_nets, y,
                           atm, k, lam, cot_k, cot_atm, (hipStream_t)stream);
+}
+
+int vk_step_error_f64(vk_batch b, const double* sol, const double* delta, double* out,
+                      int64_t stride, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  if (!sol || !delta || !out || stride < 1) {
+    vk_set_error("vk_step_error_f64: null pointer or stride < 1");
+    return VK_ERR_ARG;
+  }
+  return vk_launch_delta_max(b->d_cols, b->ncols, sol, delta, out, stride, (hipStream_t)stream);
 }
 
 }  // extern "C"

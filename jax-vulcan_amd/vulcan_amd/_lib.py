@@ -44,6 +44,7 @@ _SIGS = {
     "vk_ss_attempt_f64": (i32, [p, p, p, p, p, p, sz, p]),
     "vk_photo_f64": (i32, [p, i32, p, p, p, p, p]),
     "vk_ss_vjp_f64": (i32, [p, p, p, p, p, p, p, p]),
+    "vk_step_error_f64": (i32, [p, p, p, p, i64, p]),
     "vk_timing_enable": (i32, [i32]),
     "vk_timing_reset": (i32, []),
     "vk_timing_read": (i32, [p, p, i32]),

@@ -222,3 +222,23 @@ def test_ros2_step_batch_equals_single(cfg1):
     sol_b, _ = jax_step.jax_ros2_step(y2, k2, c.dt, c.atm, c.net)
     sol_0, _ = jax_step.jax_ros2_step(c.y, c.k, c.dt, c.atm, c.net)
     assert torch.equal(sol_b[0], sol_0)
+
+
+def test_integrate_fixed_dt_matches_oracle():
+    """integrate.jax_integrate_fixed_dt (integrate.py:20-36): 6 steps of a
+    small well-conditioned column; batch of 2 equals the single call."""
+    import torch
+    from oracle import ros2_np as O
+    from vulcan_amd import integrate, synth
+    c = synth.make_column(77, nz=24, ni=21, n_rev=40, dt=1e-8, p_bot=1e6, p_top=1e2)
+    yf, d = integrate.jax_integrate_fixed_dt(c.y, c.k, c.dt, 6, c.atm, c.net)
+    yo, do = O.integrate_fixed_dt(c.y, c.k, c.dt, 6, c.atm, c.net)
+    e = _layer_err(_np(yf), yo).max()
+    ed = np.max(np.abs(_np(d) - do) / np.abs(do))
+    print(f"integrate: state {e:.2e}, per-step delta {ed:.2e}")
+    assert e < 1e-10
+    assert ed < 1e-6
+    yb, db = integrate.jax_integrate_fixed_dt(np.stack([c.y, c.y]), c.k, c.dt, 6, c.atm, c.net)
+    assert torch.equal(yb[1], yf) and torch.equal(db[1], d)
+    y0, d0 = integrate.jax_integrate_fixed_dt(c.y, c.k, c.dt, 0, c.atm, c.net)
+    assert d0.numel() == 0 and np.array_equal(_np(y0), c.y)
