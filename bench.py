@@ -62,6 +62,9 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=64,
                     help="column-steps timed on the host for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--factor", choices=("auto", "tile", "rows"), default="auto",
+                    help="block-factor kernel for the nz=150 / ni=93 bucket (auto: the "
+                         "library default)")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only run a plain elementwise copy of GIB GiB (reads GIB, writes "
                          "GIB): the reference kernel that scales the HBM byte counters")
@@ -88,6 +91,9 @@ def main() -> int:
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
     lib = _lib.load()
+    if args.factor != "auto":
+        _lib.check(lib.vk_tune_set(0, (NI + 15) // 16, 0 if args.factor == "tile" else 1),
+                   "vk_tune_set")
     B = args.batch
     t_gen = time.time()
     cols = synth.columns_same_network(1002, B, NZ, NI, NREV, NPHOTO, dt=1e-10, name="bench")

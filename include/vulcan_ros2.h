@@ -142,6 +142,18 @@ int vk_jacobian_blocks_f64(vk_batch b, const double* y, const double* k, const d
 int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* atm,
                   const double* lam, double* cot_k, double* cot_atm, void* stream);
 
+/* ---- launch-shape tuning (no counterpart in the reference) ----
+ * Knob VK_TUNE_FACTOR picks the block-factor kernel per SL bucket
+ * (SL = ceil(ni/16)): VK_FACTOR_TILE (one 256-thread workgroup per column,
+ * 16x16 register tiles, any SL <= 7) or VK_FACTOR_ROWS (one row per lane,
+ * 64 or 128 lanes per column, SL <= 6).  Both give the same pivots; sums
+ * differ only in order.  Process-wide; set it between launches. */
+#define VK_TUNE_FACTOR 0
+#define VK_FACTOR_TILE 0
+#define VK_FACTOR_ROWS 1
+int vk_tune_set(int knob, int sl, int value);
+int vk_tune_get(int knob, int sl); /* -1 for an unknown knob / bucket */
+
 /* ---- adaptive steady-state runs with the step control on the device ----
  * Replaces the accept/reject/dt logic of outer_loop.body_fn
  * (outer_loop.py:181-250, 741-768, 847-917, 1031-1038) and its stopping

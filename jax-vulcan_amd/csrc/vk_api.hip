@@ -679,4 +679,23 @@ int vk_step_error_f64(vk_batch b, const double* sol, const double* delta, double
   return vk_launch_delta_max(b->d_cols, b->ncols, sol, delta, out, stride, (hipStream_t)stream);
 }
 
+int vk_tune_set(int knob, int sl, int value) {
+  g_err[0] = 0;
+  if (knob != VK_TUNE_FACTOR || sl < 1 || sl > 7) {
+    vk_set_error("vk_tune_set: unknown knob %d or bucket %d", knob, sl);
+    return VK_ERR_ARG;
+  }
+  if (value != VK_FACTOR_TILE && !(value == VK_FACTOR_ROWS && sl <= 6)) {
+    vk_set_error("vk_tune_set: factor variant %d not available for bucket %d", value, sl);
+    return VK_ERR_UNSUPPORTED;
+  }
+  g_factor_variant[sl] = value;
+  return VK_OK;
+}
+
+int vk_tune_get(int knob, int sl) {
+  if (knob != VK_TUNE_FACTOR || sl < 1 || sl > 7) return -1;
+  return g_factor_variant[sl];
+}
+
 }  // extern "C"

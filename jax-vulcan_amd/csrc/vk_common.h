@@ -103,6 +103,9 @@ __host__ __device__ inline int vk_slots(int ni) { return (ni + VK_G - 1) / VK_G;
 // pointers at off_s):
 //   kind 1: out1 = y + x / e0                         (Ros2 stage-1 yk2)
 //   kind 2: out1 = y + e0*k1 + e1*x, out2 = |out1 - yk2|  (Ros2 sol, delta)
+// block-factor kernel variant per SL bucket (VK_FACTOR_* in vulcan_ros2.h)
+extern int g_factor_variant[8];
+
 struct SolveEpi {
   int kind;
   double e0, e1;

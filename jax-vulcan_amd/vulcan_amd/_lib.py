@@ -45,6 +45,8 @@ _SIGS = {
     "vk_photo_f64": (i32, [p, i32, p, p, p, p, p]),
     "vk_ss_vjp_f64": (i32, [p, p, p, p, p, p, p, p]),
     "vk_step_error_f64": (i32, [p, p, p, p, i64, p]),
+    "vk_tune_set": (i32, [i32, i32, i32]),
+    "vk_tune_get": (i32, [i32, i32]),
     "vk_timing_enable": (i32, [i32]),
     "vk_timing_reset": (i32, []),
     "vk_timing_read": (i32, [p, p, i32]),

@@ -242,3 +242,63 @@ def test_integrate_fixed_dt_matches_oracle():
     assert torch.equal(yb[1], yf) and torch.equal(db[1], d)
     y0, d0 = integrate.jax_integrate_fixed_dt(c.y, c.k, c.dt, 0, c.atm, c.net)
     assert d0.numel() == 0 and np.array_equal(_np(y0), c.y)
+
+
+@pytest.mark.parametrize("ni", [5, 21, 40, 50, 64, 69, 93])
+def test_factor_variants_agree(ni):
+    """The row-per-lane and tile block factors take the same pivots and do the
+    same per-element arithmetic: bitwise-equal inverse stacks (tile-layout and
+    natural inputs, plain and transposed); the fused sweep differs only in
+    summation order."""
+    import torch
+    from vulcan_amd import _lib, solver
+    lib = _lib.load()
+    rng = np.random.default_rng(ni)
+    nz = 12
+    diag = rng.standard_normal((2, nz, ni, ni)) + 4.0 * np.eye(ni)
+    diag[1, 3] = rng.standard_normal((ni, ni))  # a block that needs pivoting
+    sup = rng.standard_normal((2, nz - 1, ni))
+    sub = rng.standard_normal((2, nz - 1, ni))
+    rhs = rng.standard_normal((2, nz, ni))
+    sl = (ni + 15) // 16
+    out = {}
+    try:
+        for variant in (0, 1):
+            _lib.check(lib.vk_tune_set(0, sl, variant), "vk_tune_set")
+            for tr in (False, True):
+                f = solver.factor_block_thomas_diag_offdiag(diag, sup, sub, transpose=tr)
+                x = solver.solve_block_thomas_diag_offdiag(f, rhs)
+                out[(variant, tr)] = (f.diag_inv.clone(), x.clone(), f.status.clone())
+    finally:
+        lib.vk_tune_set(0, sl, 0)
+    for tr in (False, True):
+        inv0, x0, s0 = out[(0, tr)]
+        inv1, x1, s1 = out[(1, tr)]
+        print(f"ni={ni} transpose={tr}: max |inv diff| {float((inv0 - inv1).abs().max()):.3e}")
+        assert torch.equal(inv0, inv1)
+        assert torch.equal(s0, s1)
+        assert float((x0 - x1).abs().max() / x0.abs().max()) < 1e-12
+    dense = [_dense(diag[b], sup[b], sub[b]) for b in range(2)]
+    x1 = _np(out[(1, False)][1])
+    for b in range(2):
+        ref = np.linalg.solve(dense[b], rhs[b].reshape(-1)).reshape(nz, ni)
+        # block 3 of system 1 is a plain random matrix: 1e-8 as for the
+        # VULCAN-shaped systems (test_block_thomas_diag.py:101)
+        assert relerr(x1[b], ref, 1e-12) < 1e-8
+
+
+def test_ros2_step_factor_variants_agree(cfg1):
+    """Whole Ros2 step with either block factor, well-conditioned dt: 1e-12."""
+    from vulcan_amd import _lib, jax_step
+    lib = _lib.load()
+    c = cfg1
+    sl = (c.ni + 15) // 16
+    sols = []
+    try:
+        for variant in (0, 1):
+            _lib.check(lib.vk_tune_set(0, sl, variant), "vk_tune_set")
+            sol, _ = jax_step.jax_ros2_step(c.y, c.k, 1e-18, c.atm, c.net)
+            sols.append(_np(sol))
+    finally:
+        lib.vk_tune_set(0, sl, 0)
+    assert _layer_err(sols[1], sols[0]).max() < 1e-12
