@@ -137,16 +137,17 @@ def main() -> int:
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     lib.vk_timing_enable(0)
-    ms = (C.c_double * 6)()
-    cnt = (C.c_longlong * 6)()
-    lib.vk_timing_read(ms, cnt, 6)
-    kern_ms = [ms[i] / max(cnt[i], 1) for i in range(5)]
+    ms = (C.c_double * 7)()
+    cnt = (C.c_longlong * 7)()
+    lib.vk_timing_read(ms, cnt, 7)
+    kern_ms = [ms[i] / max(cnt[i], 1) for i in range(7)]
     elapsed = t1 - t0
     value = B * args.steps / elapsed
     finite = bool(torch.isfinite(sol).all().item())
     n_singular = int((status != 0).sum().item())
     print(f"[bench] batch {B}: {elapsed / args.steps * 1e3:.2f} ms/step, "
-          f"{value:.1f} column-steps/s; kernels ms {['%.3f' % v for v in kern_ms]}",
+          f"{value:.1f} column-steps/s; kernels ms {['%.3f' % v for v in kern_ms[:5]]} "
+          f"transpose {kern_ms[6]:.3f}",
           file=sys.stderr, flush=True)
 
     # ---- plain single-column kernel ------------------------------------------
@@ -215,7 +216,8 @@ def main() -> int:
         "batched_over_single": round(value / single, 2),
         "kernel_ms_per_launch": {"chem_lhs": round(kern_ms[0], 4), "bt_factor": round(kern_ms[1], 4),
                                  "bt_back1": round(kern_ms[2], 4), "rhs2": round(kern_ms[3], 4),
-                                 "bt_solve2": round(kern_ms[4], 4)},
+                                 "bt_solve2": round(kern_ms[4], 4),
+                                 "rates_transpose": round(kern_ms[6], 4)},
         "step_tflops": round(B * step_flops(NZ, NI, S) / (elapsed / args.steps) / 1e12, 3),
         "finite": finite,
         "singular_columns": n_singular,

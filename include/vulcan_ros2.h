@@ -78,6 +78,21 @@ int vk_batch_create(int ncols, const int32_t* nz, const int32_t* ni, const int32
                     int nnets, vk_batch* out);
 int vk_batch_uniform(int ncols, int nz, vk_network net, vk_batch* out);
 int vk_batch_destroy(vk_batch b);
+
+/* Slot batch for continuous batching (no counterpart in the reference,
+ * whose runs are one column or a vmapped uniform batch): nslots columns of
+ * fixed capacity -- up to cap_nz layers, cap_S rate rows, cap_T reactant
+ * slots, ni <= cap_ni in the same 16-species bucket as cap_ni.  Slot c's
+ * arrays sit at off_s = c*cap_nz*cap_ni, off_k = c*cap_S*cap_nz, off_a =
+ * c*(record length of (cap_nz, cap_ni)), off_b = c*cap_nz*vk_block_doubles.
+ * Slots start empty: the caller keeps their status words at a DONE value
+ * until vk_batch_set_column places a column, and only the steady-state
+ * entry points accept a slot batch. */
+int vk_batch_slots(int nslots, int cap_nz, int cap_ni, int cap_S, int cap_T, vk_batch* out);
+/* Place a column of nz layers with network net in a slot: the descriptor is
+ * rewritten by a one-thread kernel on the stream, so it is ordered with the
+ * launches around it and legal between graph replays. */
+int vk_batch_set_column(vk_batch b, int slot, int nz, vk_network net, void* stream);
 /* Doubles needed by a block stack of one layer with ni species. */
 int64_t vk_block_doubles(int ni);
 /* Total doubles of the packed block stacks of a batch. */
@@ -187,6 +202,9 @@ int vk_ss_init_f64(vk_batch b, const double* atm, const vk_ss_params* p, const v
 /* One attempt for every column whose status has no DONE bit: a Ros2 step at
  * (y, dt) into sol/delta_arr, then the device controller.  No host sync, no
  * allocation: safe to capture in a hipGraph and replay. */
+/* vk_ss_init_f64 for the columns [first, first + count) only (a refilled slot). */
+int vk_ss_init_range_f64(vk_batch b, int first, int count, const double* atm,
+                         const vk_ss_params* p, const vk_ss_state* s, void* stream);
 int vk_ss_attempt_f64(vk_batch b, const double* k, const double* atm, const vk_ss_params* p,
                       const vk_ss_state* s, void* ws, size_t ws_bytes, void* stream);
 
@@ -222,7 +240,8 @@ int vk_photo_f64(vk_batch b, int parts, const vk_photo_setup* setup, const vk_ph
 /* Kernel timing with HIP events recorded on the launch stream around each
  * kernel of vk_ros2_step_f64 (no counterpart in the reference, which has no
  * device timing).  Kernel ids: 0 chemistry+LHS, 1 block factor, 2 stage-1
- * back sweep, 3 stage-2 rhs, 4 stage-2 solve.  vk_timing_read synchronises
+ * back sweep, 3 stage-2 rhs, 4 stage-2 solve, 5 step control, 6 rate-table
+ * transpose.  vk_timing_read synchronises
  * on the recorded events and returns accumulated milliseconds and launch
  * counts per id.  Not thread-safe; for benchmarks. */
 int vk_timing_enable(int on);

@@ -26,7 +26,7 @@ void vk_set_error(const char* fmt, ...) {
 // optional per-kernel timing with HIP events recorded on the launch stream
 // ---------------------------------------------------------------------------
 namespace {
-enum { VK_NKERN = 6 };
+enum { VK_NKERN = 7 };
 struct TimedLaunch {
   int kind;
   hipEvent_t a, b;
@@ -83,10 +83,15 @@ int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max
                          int max_T, const VkNetDev* nets, const double* y, const double* M,
                          const double* atm, const double* k, const double* dt, const uint8_t* fix,
                          const double* k1, double* out0, double* out_blk, double* out_sup,
-                         double* out_sub, const int* active, hipStream_t stream);
+                         double* out_sub, const int* active, int k_layer_major, hipStream_t stream);
+int vk_launch_rates_layer_major(const VkCol* cols, int ncols, int max_S, int max_nz, const double* k,
+                                double* kT, const int* active, hipStream_t stream);
 
 int vk_launch_ss_control(const VkCol* cols, int ncols, const double* atm, const vk_ss_params& p,
                          const vk_ss_state& s, hipStream_t st);
+int vk_launch_ss_init_range(const VkCol* cols, int ncols, int first, int count,
+                            const double* atm, const vk_ss_params& p, const vk_ss_state& s,
+                            hipStream_t st);
 int vk_launch_ss_init(const VkCol* cols, int ncols, const double* atm, const vk_ss_params& p,
                       const vk_ss_state& s, hipStream_t st);
 
@@ -114,8 +119,11 @@ struct vk_batch_s {
   std::vector<int> slots;   // distinct SL buckets
   int max_nz = 0, max_S = 0, max_T = 0, max_ni = 0;
   int64_t max_rec = 0;
-  int64_t state_doubles = 0, block_doubles = 0;
+  int64_t state_doubles = 0, block_doubles = 0, k_doubles = 0;
   bool uniform = false;
+  // slot batch (continuous batching): per-slot capacity and network table
+  bool slots_mode = false;
+  int cap_nz = 0, cap_ni = 0, cap_S = 0, cap_T = 0;
 };
 
 namespace {
@@ -127,6 +135,18 @@ size_t align_up(size_t n) { return (n * sizeof(T) + 255) & ~size_t(255); }
 
 int check_batch(vk_batch b) {
   if (!b) { vk_set_error("null batch"); return VK_ERR_ARG; }
+  return VK_OK;
+}
+
+// Entry points that touch every column: a slot batch has idle slots
+// without a network, so only the steady-state calls (which skip finished
+// and idle columns) accept it.
+int check_plain(vk_batch b, const char* who) {
+  if (int rc = check_batch(b)) return rc;
+  if (b->slots_mode) {
+    vk_set_error("%s: slot batches are for the steady-state entry points only", who);
+    return VK_ERR_ARG;
+  }
   return VK_OK;
 }
 
@@ -328,6 +348,7 @@ int vk_batch_create(int ncols, const int32_t* nz, const int32_t* ni, const int32
     b->max_S = std::max(b->max_S, (int)S[c]);
     b->max_T = std::max(b->max_T, nw->T);
     b->state_doubles = std::max<int64_t>(b->state_doubles, off_s[c] + (int64_t)nz[c] * ni[c]);
+    b->k_doubles = std::max<int64_t>(b->k_doubles, off_k[c] + (int64_t)S[c] * nz[c]);
     b->block_doubles =
         std::max<int64_t>(b->block_doubles, off_b[c] + (int64_t)nz[c] * vk_block_doubles(ni[c]));
   }
@@ -371,6 +392,81 @@ int vk_batch_uniform(int ncols, int nz, vk_network net, vk_batch* out) {
   return rc;
 }
 
+__global__ void k_set_slot(VkCol* cols, VkNetDev* nets, int slot, VkCol col, VkNetDev net) {
+  cols[slot] = col;
+  nets[slot] = net;
+}
+
+int vk_batch_slots(int nslots, int cap_nz, int cap_ni, int cap_S, int cap_T, vk_batch* out) {
+  g_err[0] = 0;
+  if (!out || nslots < 1 || cap_nz < 2 || cap_ni < 1 || cap_ni > vk_max_species() || cap_S < 1 ||
+      cap_T < 1) {
+    vk_set_error("vk_batch_slots: bad arguments (nslots=%d cap_nz=%d cap_ni=%d cap_S=%d cap_T=%d)",
+                 nslots, cap_nz, cap_ni, cap_S, cap_T);
+    return VK_ERR_ARG;
+  }
+  vk_batch_s* b = new vk_batch_s();
+  b->ncols = nslots;
+  b->slots_mode = true;
+  b->cap_nz = cap_nz; b->cap_ni = cap_ni; b->cap_S = cap_S; b->cap_T = cap_T;
+  b->cols.resize(nslots);
+  const int64_t cs = (int64_t)cap_nz * cap_ni, ck = (int64_t)cap_S * cap_nz;
+  const int64_t ca = vk_atm_offsets(cap_nz, cap_ni).total;
+  const int64_t cb = (int64_t)cap_nz * vk_block_doubles(cap_ni);
+  for (int c = 0; c < nslots; ++c) {
+    VkCol& cl = b->cols[c];
+    cl.nz = cap_nz; cl.ni = cap_ni; cl.S = cap_S; cl.net = c;
+    cl.off_s = c * cs; cl.off_k = c * ck; cl.off_a = c * ca; cl.off_b = c * cb;
+  }
+  b->slots.assign(1, vk_slots(cap_ni));
+  b->max_nz = cap_nz; b->max_ni = cap_ni; b->max_S = cap_S; b->max_T = cap_T;
+  b->max_rec = ca;
+  b->state_doubles = nslots * cs;
+  b->k_doubles = nslots * ck;
+  b->block_doubles = nslots * cb;
+  std::vector<VkNetDev> hn(nslots, VkNetDev{});
+  if (hipMalloc(&b->d_cols, sizeof(VkCol) * nslots) != hipSuccess ||
+      hipMalloc(&b->d_nets, sizeof(VkNetDev) * nslots) != hipSuccess) {
+    vk_batch_destroy(b);
+    vk_set_error("vk_batch_slots: hipMalloc failed");
+    return VK_ERR_NOMEM;
+  }
+  if (hipMemcpy(b->d_cols, b->cols.data(), sizeof(VkCol) * nslots, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->d_nets, hn.data(), sizeof(VkNetDev) * nslots, hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    vk_batch_destroy(b);
+    vk_set_error("vk_batch_slots: upload failed");
+    return VK_ERR_HIP;
+  }
+  *out = b;
+  return VK_OK;
+}
+
+int vk_batch_set_column(vk_batch b, int slot, int nz, vk_network net, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  if (!b->slots_mode) { vk_set_error("vk_batch_set_column: not a slot batch"); return VK_ERR_ARG; }
+  if (slot < 0 || slot >= b->ncols || !net) {
+    vk_set_error("vk_batch_set_column: slot %d of %d or null network", slot, b->ncols);
+    return VK_ERR_ARG;
+  }
+  if (nz < 2 || nz > b->cap_nz || net->ni > b->cap_ni || vk_slots(net->ni) != vk_slots(b->cap_ni) ||
+      net->nr + 1 > b->cap_S || net->T > b->cap_T) {
+    vk_set_error("vk_batch_set_column: nz=%d ni=%d S=%d T=%d does not fit the slot (nz<=%d, "
+                 "ni<=%d in the same 16-species bucket, S<=%d, T<=%d)", nz, net->ni, net->nr + 1,
+                 net->T, b->cap_nz, b->cap_ni, b->cap_S, b->cap_T);
+    return VK_ERR_ARG;
+  }
+  VkCol cl = b->cols[slot];
+  cl.nz = nz; cl.ni = net->ni; cl.S = net->nr + 1; cl.net = slot;
+  b->cols[slot] = cl;
+  hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(1), 0, (hipStream_t)stream, b->d_cols, b->d_nets,
+                     slot, cl, net->dev);
+  VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}
+
 int vk_batch_destroy(vk_batch b) {
   if (!b) return VK_OK;
   if (b->d_cols) (void)hipFree(b->d_cols);
@@ -399,7 +495,7 @@ static int chem_call(vk_batch b, int mode, const double* y, const double* M, con
   for (int sl : b->slots) {
     int rc = vk_launch_chem_layer(sl, mode, b->d_cols, b->ncols, b->max_nz, b->max_S, b->max_T,
                                   b->d_nets, y, M, atm, k, dt, fix, k1, out0, out_blk, sup, sub,
-                                  active, st);
+                                  active, 0, st);
     if (rc) return rc;
   }
   return VK_OK;
@@ -408,7 +504,7 @@ static int chem_call(vk_batch b, int mode, const double* y, const double* M, con
 int vk_chem_rhs_f64(vk_batch b, const double* y, const double* M, const double* atm,
                     const double* k, double* out, void* stream) {
   g_err[0] = 0;
-  if (int rc = check_batch(b)) return rc;
+  if (int rc = check_plain(b, "vk_chem_rhs_f64")) return rc;
   if (!y || !k || !out || (!M && !atm)) { vk_set_error("vk_chem_rhs_f64: null pointer"); return VK_ERR_ARG; }
   if (M && !b->uniform) { vk_set_error("vk_chem_rhs_f64: explicit M needs a uniform batch"); return VK_ERR_ARG; }
   return chem_call(b, M_CHEM_RHS, y, M, atm, k, nullptr, nullptr, nullptr, out, nullptr, nullptr,
@@ -418,7 +514,7 @@ int vk_chem_rhs_f64(vk_batch b, const double* y, const double* M, const double* 
 int vk_chem_jac_f64(vk_batch b, const double* y, const double* M, const double* atm,
                     const double* k, double* jac, void* stream) {
   g_err[0] = 0;
-  if (int rc = check_batch(b)) return rc;
+  if (int rc = check_plain(b, "vk_chem_jac_f64")) return rc;
   if (!y || !k || !jac || (!M && !atm)) { vk_set_error("vk_chem_jac_f64: null pointer"); return VK_ERR_ARG; }
   if (M && !b->uniform) { vk_set_error("vk_chem_jac_f64: explicit M needs a uniform batch"); return VK_ERR_ARG; }
   return chem_call(b, M_CHEM_JAC, y, M, atm, k, nullptr, nullptr, nullptr, jac, nullptr, nullptr,
@@ -428,7 +524,7 @@ int vk_chem_jac_f64(vk_batch b, const double* y, const double* M, const double* 
 int vk_residual_f64(vk_batch b, const double* y, const double* k, const double* atm, double* out,
                     void* stream) {
   g_err[0] = 0;
-  if (int rc = check_batch(b)) return rc;
+  if (int rc = check_plain(b, "vk_residual_f64")) return rc;
   if (!y || !k || !atm || !out) { vk_set_error("vk_residual_f64: null pointer"); return VK_ERR_ARG; }
   return chem_call(b, M_FTOT, y, nullptr, atm, k, nullptr, nullptr, nullptr, out, nullptr, nullptr,
                    nullptr, nullptr, (hipStream_t)stream);
@@ -437,7 +533,7 @@ int vk_residual_f64(vk_batch b, const double* y, const double* k, const double* 
 int vk_jacobian_blocks_f64(vk_batch b, const double* y, const double* k, const double* atm,
                            double* diag, double* sup_d, double* sub_d, void* stream) {
   g_err[0] = 0;
-  if (int rc = check_batch(b)) return rc;
+  if (int rc = check_plain(b, "vk_jacobian_blocks_f64")) return rc;
   if (!y || !k || !atm || !diag || !sup_d || !sub_d) {
     vk_set_error("vk_jacobian_blocks_f64: null pointer");
     return VK_ERR_ARG;
@@ -449,7 +545,7 @@ int vk_jacobian_blocks_f64(vk_batch b, const double* y, const double* k, const d
 int vk_bt_factor_f64(vk_batch b, const double* diag, const double* sup_d, const double* sub_d,
                      int transpose, double* inv, int32_t* status, void* stream) {
   g_err[0] = 0;
-  if (int rc = check_batch(b)) return rc;
+  if (int rc = check_plain(b, "vk_bt_factor_f64")) return rc;
   if (!diag || !sup_d || !sub_d || !inv) { vk_set_error("vk_bt_factor_f64: null pointer"); return VK_ERR_ARG; }
   for (int sl : b->slots) {
     int rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, diag, 0, transpose, sup_d, sub_d, inv,
@@ -462,7 +558,7 @@ int vk_bt_factor_f64(vk_batch b, const double* diag, const double* sup_d, const 
 int vk_bt_solve_f64(vk_batch b, const double* inv, const double* sup_d, const double* sub_d,
                     const double* rhs, double* x, double* scratch, void* stream) {
   g_err[0] = 0;
-  if (int rc = check_batch(b)) return rc;
+  if (int rc = check_plain(b, "vk_bt_solve_f64")) return rc;
   if (!inv || !sup_d || !sub_d || !rhs || !x || !scratch) {
     vk_set_error("vk_bt_solve_f64: null pointer");
     return VK_ERR_ARG;
@@ -478,7 +574,7 @@ int vk_bt_solve_f64(vk_batch b, const double* inv, const double* sup_d, const do
 
 int vk_bt_inverse_natural_f64(vk_batch b, const double* inv, double* nat, void* stream) {
   g_err[0] = 0;
-  if (int rc = check_batch(b)) return rc;
+  if (int rc = check_plain(b, "vk_bt_inverse_natural_f64")) return rc;
   for (int sl : b->slots) {
     int rc = vk_launch_tile_to_nat(sl, b->d_cols, b->ncols, b->max_nz, inv, nat,
                                    (hipStream_t)stream);
@@ -522,7 +618,8 @@ int vk_timing_read(double* ms, long long* count, int n) {
 
 size_t vk_ros2_workspace_bytes(vk_batch b) {
   if (!b) return 0;
-  return align_up<double>(b->block_doubles) + 8 * align_up<double>(b->state_doubles);
+  return align_up<double>(b->block_doubles) + 8 * align_up<double>(b->state_doubles) +
+         align_up<double>(b->k_doubles);
 }
 
 static int ros2_step_impl(vk_batch b, const double* y, const double* k, const double* atm,
@@ -539,14 +636,21 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
   double* k2 = (double*)w; w += sv;
   double* sup = (double*)w; w += sv;
   double* sub = (double*)w; w += sv;
+  double* kT = (double*)w;  // the rate tables layer-major, read contiguously by the chemistry
   const double r = ros2_gamma();
+  {
+    KernelTimer t(6, st);
+    if (int rc = vk_launch_rates_layer_major(b->d_cols, b->ncols, b->max_S, b->max_nz, k, kT,
+                                             active, st))
+      return rc;
+  }
   for (int sl : b->slots) {
     int rc;
     {
       KernelTimer t(0, st);
       if ((rc = vk_launch_chem_layer(sl, M_LHS, b->d_cols, b->ncols, b->max_nz, b->max_S,
-                                     b->max_T, b->d_nets, y, nullptr, atm, k, dt, fix_mask,
-                                     nullptr, rhs_y, blk, sup, sub, active, st)))
+                                     b->max_T, b->d_nets, y, nullptr, atm, kT, dt, fix_mask,
+                                     nullptr, rhs_y, blk, sup, sub, active, 1, st)))
         return rc;
     }
     {
@@ -565,8 +669,8 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
     {
       KernelTimer t(3, st);
       if ((rc = vk_launch_chem_layer(sl, M_RHS2, b->d_cols, b->ncols, b->max_nz, b->max_S,
-                                     b->max_T, b->d_nets, yk2, nullptr, atm, k, dt, fix_mask, k1,
-                                     rhs2, nullptr, nullptr, nullptr, active, st)))
+                                     b->max_T, b->d_nets, yk2, nullptr, atm, kT, dt, fix_mask, k1,
+                                     rhs2, nullptr, nullptr, nullptr, active, 1, st)))
         return rc;
     }
     SolveEpi e2{2, 3.0 / (2.0 * r), 1.0 / (2.0 * r), y, k1, yk2, sol, delta};
@@ -584,7 +688,7 @@ int vk_ros2_step_f64(vk_batch b, const double* y, const double* k, const double*
                      const double* dt, const uint8_t* fix_mask, double* sol, double* delta,
                      void* ws, size_t ws_bytes, int32_t* status, void* stream) {
   g_err[0] = 0;
-  if (int rc = check_batch(b)) return rc;
+  if (int rc = check_plain(b, "vk_ros2_step_f64")) return rc;
   if (!y || !k || !atm || !dt || !sol || !delta || !ws) {
     vk_set_error("vk_ros2_step_f64: null pointer");
     return VK_ERR_ARG;
@@ -617,7 +721,24 @@ int vk_ss_init_f64(vk_batch b, const double* atm, const vk_ss_params* p, const v
                    void* stream) {
   g_err[0] = 0;
   if (int rc = check_ss(b, atm, p, s)) return rc;
+  if (b->slots_mode) {
+    vk_set_error("vk_ss_init_f64: a slot batch initialises placed slots with vk_ss_init_range_f64");
+    return VK_ERR_ARG;
+  }
   return vk_launch_ss_init(b->d_cols, b->ncols, atm, *p, *s, (hipStream_t)stream);
+}
+
+int vk_ss_init_range_f64(vk_batch b, int first, int count, const double* atm,
+                         const vk_ss_params* p, const vk_ss_state* s, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_ss(b, atm, p, s)) return rc;
+  if (first < 0 || count < 1 || first + count > b->ncols) {
+    vk_set_error("vk_ss_init_range_f64: columns [%d, %d) outside [0, %d)", first, first + count,
+                 b->ncols);
+    return VK_ERR_ARG;
+  }
+  return vk_launch_ss_init_range(b->d_cols, b->ncols, first, count, atm, *p, *s,
+                                 (hipStream_t)stream);
 }
 
 int vk_ss_attempt_f64(vk_batch b, const double* k, const double* atm, const vk_ss_params* p,
@@ -640,7 +761,7 @@ int vk_ss_attempt_f64(vk_batch b, const double* k, const double* atm, const vk_s
 int vk_photo_f64(vk_batch b, int parts, const vk_photo_setup* setup, const vk_photo_state* state,
                  const double* y, double* k, void* stream) {
   g_err[0] = 0;
-  if (int rc = check_batch(b)) return rc;
+  if (int rc = check_plain(b, "vk_photo_f64")) return rc;
   if (!setup || !state || !setup->off_p || !setup->off_l || !setup->bins || !setup->sflux_top ||
       !state->tau || !state->w0 || !state->sflux || !state->dflux_d || !state->dflux_u ||
       !state->aflux || !state->J || ((parts & 1) && (!y || !setup->dz))) {
@@ -659,7 +780,7 @@ int vk_photo_f64(vk_batch b, int parts, const vk_photo_setup* setup, const vk_ph
 int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* atm,
                   const double* lam, double* cot_k, double* cot_atm, void* stream) {
   g_err[0] = 0;
-  if (int rc = check_batch(b)) return rc;
+  if (int rc = check_plain(b, "vk_ss_vjp_f64")) return rc;
   if (!y || !k || !atm || !lam || (!cot_k && !cot_atm)) {
     vk_set_error("vk_ss_vjp_f64: null pointer in arguments");
     return VK_ERR_ARG;
@@ -671,7 +792,7 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
 int vk_step_error_f64(vk_batch b, const double* sol, const double* delta, double* out,
                       int64_t stride, void* stream) {
   g_err[0] = 0;
-  if (int rc = check_batch(b)) return rc;
+  if (int rc = check_plain(b, "vk_step_error_f64")) return rc;
   if (!sol || !delta || !out || stride < 1) {
     vk_set_error("vk_step_error_f64: null pointer or stride < 1");
     return VK_ERR_ARG;

@@ -37,6 +37,7 @@ struct LayerArgs {
   double* out_sub;
   const int* active;
   int mode;
+  int k_layer_major;     // k holds (nz, S) per column (a transposed copy)
 };
 
 constexpr double GAMMA = 1.7071067811865475;  // 1 + 1/sqrt(2)
@@ -194,7 +195,9 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
   AtmView av;
   if (atmb) av = atm_view(atmb, nz, ni);
   const double Mz = L.M ? L.M[col.off_s / ni + z] : av.M[z];
-  const double* kz = L.k + col.off_k;  // k[r * nz + z]
+  // rate of row r at this layer: k[r * nz + z], or kT[z * S + r] (contiguous)
+  const double* kz = L.k + col.off_k + (L.k_layer_major ? (int64_t)z * S : (int64_t)z);
+  const int64_t kstride = L.k_layer_major ? 1 : nz;
 
   // y rows z-1, z, z+1 with a trailing 1.0 pad (PAD index = ni)
   for (int e = tid; e < 3 * (ni + 1); e += VK_NT) {
@@ -210,7 +213,7 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
   // reaction rates and rate derivatives (chem.py:92-99, 162-183)
   for (int r = tid; r < S; r += VK_NT) {
 #pragma clang fp contract(off)
-    const double kr = kz[(int64_t)r * nz + z];
+    const double kr = kz[(int64_t)r * kstride];
     double fac[4];
     double prodr = 1.0;
     for (int s = 0; s < T; ++s) {
@@ -389,9 +392,10 @@ int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max
                          int max_T, const VkNetDev* nets, const double* y, const double* M,
                          const double* atm, const double* k, const double* dt, const uint8_t* fix,
                          const double* k1, double* out0, double* out_blk, double* out_sup,
-                         double* out_sub, const int* active, hipStream_t stream) {
+                         double* out_sub, const int* active, int k_layer_major,
+                         hipStream_t stream) {
   LayerArgs L{cols, ncols, nets, y, M, atm, k, dt, fix, k1, out0, out_blk, out_sup, out_sub,
-              active, mode};
+              active, mode, k_layer_major};
   const bool want_jac = (mode == M_CHEM_JAC || mode == M_LHS || mode == M_JY);
   // ni+1 <= 16*sl+1 for the y rows
   const size_t shmem =
@@ -423,5 +427,43 @@ int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max
       return VK_ERR_UNSUPPORTED;
   }
   VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}
+
+// kT[off_k + z*S + r] = k[off_k + r*nz + z]: the rate tables layer-major,
+// through a 32 x 32 LDS tile so both sides are coalesced.
+__global__ __launch_bounds__(256) void k_rates_layer_major(const VkCol* __restrict__ cols, int ncols,
+                                                          const double* __restrict__ k,
+                                                          double* __restrict__ kT,
+                                                          const int* __restrict__ active, int c0) {
+  __shared__ double tile[32][33];
+  const int c = c0 + (int)blockIdx.z;
+  if (c >= ncols) return;
+  if (active && (active[c] & VK_ST_DONE)) return;
+  const VkCol col = cols[c];
+  const int S = col.S, nz = col.nz;
+  const int r0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
+  if (r0 >= S || z0 >= nz) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int i = ty; i < 32; i += 8) {
+    const int r = r0 + i, z = z0 + tx;
+    if (r < S && z < nz) tile[i][tx] = k[col.off_k + (int64_t)r * nz + z];
+  }
+  __syncthreads();
+  for (int i = ty; i < 32; i += 8) {
+    const int z = z0 + i, r = r0 + tx;
+    if (r < S && z < nz) kT[col.off_k + (int64_t)z * S + r] = tile[tx][i];
+  }
+}
+
+int vk_launch_rates_layer_major(const VkCol* cols, int ncols, int max_S, int max_nz, const double* k,
+                                double* kT, const int* active, hipStream_t stream) {
+  for (int c0 = 0; c0 < ncols; c0 += 65535) {  // grid z is limited to 65535
+    const int nc = ncols - c0 < 65535 ? ncols - c0 : 65535;
+    dim3 grid((max_S + 31) / 32, (max_nz + 31) / 32, nc);
+    hipLaunchKernelGGL(k_rates_layer_major, grid, dim3(256), 0, stream, cols, ncols, k, kT, active,
+                       c0);
+    VK_CHECK_HIP(hipGetLastError());
+  }
   return VK_OK;
 }

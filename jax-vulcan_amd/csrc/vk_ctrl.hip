@@ -223,9 +223,9 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
 // ymix, atom_ini / atom_loss, counters; y and dt are set by the caller.
 __global__ __launch_bounds__(NT) void k_ss_init(const VkCol* __restrict__ cols, int ncols,
                                                  const double* __restrict__ atm, vk_ss_params p,
-                                                 vk_ss_state s) {
+                                                 vk_ss_state s, int first) {
   __shared__ double redd[NT / 64];
-  const int c = blockIdx.x;
+  const int c = first + (int)blockIdx.x;
   if (c >= ncols) return;
   const VkCol col = cols[c];
   const int nz = col.nz, ni = col.ni, tid = threadIdx.x, na = p.na;
@@ -279,7 +279,15 @@ int vk_launch_ss_control(const VkCol* cols, int ncols, const double* atm, const 
 
 int vk_launch_ss_init(const VkCol* cols, int ncols, const double* atm, const vk_ss_params& p,
                       const vk_ss_state& s, hipStream_t st) {
-  hipLaunchKernelGGL(k_ss_init, dim3(ncols), dim3(NT), 0, st, cols, ncols, atm, p, s);
+  hipLaunchKernelGGL(k_ss_init, dim3(ncols), dim3(NT), 0, st, cols, ncols, atm, p, s, 0);
+  VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}
+
+int vk_launch_ss_init_range(const VkCol* cols, int ncols, int first, int count,
+                            const double* atm, const vk_ss_params& p, const vk_ss_state& s,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_ss_init, dim3(count), dim3(NT), 0, st, cols, ncols, atm, p, s, first);
   VK_CHECK_HIP(hipGetLastError());
   return VK_OK;
 }

@@ -46,6 +46,9 @@ _SIGS = {
     "vk_ss_vjp_f64": (i32, [p, p, p, p, p, p, p, p]),
     "vk_step_error_f64": (i32, [p, p, p, p, i64, p]),
     "vk_tune_set": (i32, [i32, i32, i32]),
+    "vk_batch_slots": (i32, [i32, i32, i32, i32, i32, p]),
+    "vk_batch_set_column": (i32, [p, i32, i32, p, p]),
+    "vk_ss_init_range_f64": (i32, [p, i32, i32, p, p, p, p]),
     "vk_tune_get": (i32, [i32, i32]),
     "vk_timing_enable": (i32, [i32]),
     "vk_timing_reset": (i32, []),
