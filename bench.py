@@ -62,6 +62,11 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=64,
                     help="column-steps timed on the host for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--uneven", type=int, default=512,
+                    help="columns of the uneven-size batch leg (nz 60..200, ni 50/69/93; "
+                         "0 skips it)")
+    ap.add_argument("--uneven-attempts", type=int, default=10,
+                    help="steady-state attempts per column timed in the uneven leg")
     ap.add_argument("--factor", choices=("auto", "tile", "rows"), default="auto",
                     help="block-factor kernel for the nz=150 / ni=93 bucket (auto: the "
                          "library default)")
@@ -125,6 +130,10 @@ def main() -> int:
         return run, sol, status
 
     # ---- batched throughput ------------------------------------------------
+    main_factor = args.factor
+    if args.factor == "auto":
+        from vulcan_amd import autotune
+        main_factor = autotune.tune_columns(cols, verbose=True)[(NI + 15) // 16]
     run, sol, status = stepper(B)
     for _ in range(args.warmup):
         run()
@@ -160,6 +169,39 @@ def main() -> int:
     torch.cuda.synchronize()
     single = args.single_steps / (time.perf_counter() - t0)
     print(f"[bench] single column: {single:.1f} column-steps/s", file=sys.stderr, flush=True)
+
+    # ---- uneven batch: adaptive steady-state attempts, one graph replay --------
+    uneven = None
+    if args.uneven > 0:
+        from vulcan_amd import autotune
+        from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
+        t_gen = time.time()
+        cols3 = synth.config3(1003, args.uneven)
+        print(f"[bench] uneven batch: {len(cols3)} columns generated in {time.time() - t_gen:.1f}s",
+              file=sys.stderr, flush=True)
+        chosen = autotune.tune_columns(cols3, verbose=True) if args.factor == "auto" else {}
+        runner = SteadyStateRunner(cols3, RunConfig())
+        runner.capture(args.uneven_attempts)
+        runner.graph[0].replay()  # warm replay
+        torch.cuda.synchronize()
+        runner.reset_to_initial()
+        torch.cuda.synchronize()
+        a0 = runner.attempts.sum().item()
+        t0 = time.perf_counter()
+        runner.graph[0].replay()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        col_steps = runner.attempts.sum().item() - a0
+        uneven = {"columns": len(cols3), "attempts_per_column": args.uneven_attempts,
+                  "column_steps": int(col_steps), "seconds": round(el, 4),
+                  "column_steps_per_s": round(col_steps / el, 2),
+                  "ms_per_attempt": round(el / args.uneven_attempts * 1e3, 3),
+                  "sizes": "nz~U{60..200}, ni in {50,69,93}, nr=13*ni, independent columns",
+                  "launch": f"one HIP graph of {args.uneven_attempts} attempts",
+                  "factor_kernel_by_bucket": {str(k): v for k, v in chosen.items()}}
+        print(f"[bench] uneven batch: {col_steps / el:.1f} column-steps/s, "
+              f"{el / args.uneven_attempts * 1e3:.2f} ms per attempt", file=sys.stderr, flush=True)
+        del runner
 
     # ---- roofline of the dominant kernel (block factor) ---------------------
     fac_ms = kern_ms[1]
@@ -212,6 +254,7 @@ def main() -> int:
         "config": {"workload": "Ros2 step, nz=150 ni=93 S=2461 (1200 reversible + 60 photo rows), "
                                f"{B} independent columns per step, dt=1e-10",
                    "batch_columns": B, "nz": NZ, "ni": NI, "rate_rows": S},
+        "factor_kernel": main_factor,
         "single_column_steps_per_s": round(single, 3),
         "batched_over_single": round(value / single, 2),
         "kernel_ms_per_launch": {"chem_lhs": round(kern_ms[0], 4), "bt_factor": round(kern_ms[1], 4),
@@ -221,6 +264,7 @@ def main() -> int:
         "step_tflops": round(B * step_flops(NZ, NI, S) / (elapsed / args.steps) / 1e12, 3),
         "finite": finite,
         "singular_columns": n_singular,
+        "uneven_batch": uneven,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -1,0 +1,103 @@
+"""Launch-shape autotuner for batches of uneven columns.
+
+For every 16-species bucket (SL = ceil(ni/16)) present in a batch, time the
+block-factor kernels the library offers for that bucket (``VK_FACTOR_TILE``:
+one 256-thread workgroup per column with 16x16 register tiles;
+``VK_FACTOR_ROWS``: half a row per lane, 64-192 lanes per column) on the
+batch's own columns of that bucket, and keep the faster one
+(``vk_tune_set``).  Both kernels take the same pivots and give bitwise-equal
+inverses, so tuning never changes results.  Decisions are cached per
+(SL, nz rounded up to 32, column count rounded up to a power of two).
+
+The timed call is one plain Ros2 step of the bucket's columns (y -> sol), so
+a runner's state is not touched.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from typing import Dict, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .core import F64, Batch, device, device_network
+
+VK_TUNE_FACTOR = 0
+VARIANTS = {0: "tile", 1: "rows"}
+_CACHE: Dict[Tuple[int, int, int], int] = {}
+
+
+def _key(sl: int, nz_max: int, ncols: int) -> Tuple[int, int, int]:
+    return (sl, 32 * math.ceil(nz_max / 32), 1 << max(0, math.ceil(math.log2(max(ncols, 1)))))
+
+
+def _time_step(batch: Batch, y, k, atm, dt, reps: int) -> float:
+    lib = _lib.load()
+    ws = batch.workspace()
+    sol = torch.empty_like(y)
+    delta = torch.empty_like(y)
+    status = torch.zeros(batch.ncols, dtype=torch.int32, device=y.device)
+    st = torch.cuda.current_stream().cuda_stream
+
+    def run():
+        _lib.check(lib.vk_ros2_step_f64(batch.handle, y.data_ptr(), k.data_ptr(), atm.data_ptr(),
+                                        dt.data_ptr(), None, sol.data_ptr(), delta.data_ptr(),
+                                        ws.data_ptr(), ws.numel(), status.data_ptr(), st),
+                   "vk_ros2_step_f64")
+
+    run()  # first launch: module load, LDS attributes
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def tune_columns(columns: Sequence, reps: int = 2, verbose: bool = False) -> Dict[int, str]:
+    """Pick the block-factor kernel for each bucket of ``columns``; returns
+    {SL: variant name}.  Cached decisions are applied without timing."""
+    lib = _lib.load()
+    dev = device()
+    by_sl: Dict[int, list] = {}
+    for c in columns:
+        by_sl.setdefault((int(c.ni) + 15) // 16, []).append(c)
+    chosen = {}
+    for sl, cols in sorted(by_sl.items()):
+        key = _key(sl, max(int(c.nz) for c in cols), len(cols))
+        if key in _CACHE:
+            _lib.check(lib.vk_tune_set(VK_TUNE_FACTOR, sl, _CACHE[key]), "vk_tune_set")
+            chosen[sl] = VARIANTS[_CACHE[key]]
+            continue
+        nets, ids, seen = [], [], {}
+        for c in cols:
+            if id(c.net) not in seen:
+                seen[id(c.net)] = len(nets)
+                nets.append(device_network(c.net))
+            ids.append(seen[id(c.net)])
+        b = Batch([c.nz for c in cols], nets, ids)
+        y = b.pack_state([c.y for c in cols], dev)
+        k = b.pack_k([c.k for c in cols], dev)
+        atm = b.pack_atm([c.atm for c in cols], dev)
+        dt = torch.tensor([float(c.dt) for c in cols], dtype=F64, device=dev)
+        times = {}
+        for v in (0, 1):
+            if lib.vk_tune_set(VK_TUNE_FACTOR, sl, v) != 0:
+                continue  # variant not offered for this bucket
+            times[v] = _time_step(b, y, k, atm, dt, reps)
+        best = min(times, key=times.get)
+        _lib.check(lib.vk_tune_set(VK_TUNE_FACTOR, sl, best), "vk_tune_set")
+        _CACHE[key] = best
+        chosen[sl] = VARIANTS[best]
+        if verbose:
+            print(f"[autotune] SL={sl} ({len(cols)} columns, nz<={key[1]}): " +
+                  ", ".join(f"{VARIANTS[v]} {t * 1e3:.2f} ms" for v, t in times.items()) +
+                  f" -> {VARIANTS[best]}", flush=True)
+    return chosen
+
+
+def clear_cache():
+    _CACHE.clear()
