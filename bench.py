@@ -67,6 +67,12 @@ def main() -> int:
                          "0 skips it)")
     ap.add_argument("--uneven-attempts", type=int, default=10,
                     help="steady-state attempts per column timed in the uneven leg")
+    ap.add_argument("--serve", type=int, default=0,
+                    help="serving leg: this many steady-state solve requests (sizes of the "
+                         "uneven batch, Poisson arrivals at --serve-rate per s, 200 GB budget)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-alone", type=int, default=1,
+                    help="requests also solved one at a time, for the comparison")
     ap.add_argument("--factor", choices=("auto", "tile", "rows"), default="auto",
                     help="block-factor kernel for the nz=150 / ni=93 bucket (auto: the "
                          "library default)")
@@ -203,6 +209,42 @@ def main() -> int:
               f"{el / args.uneven_attempts * 1e3:.2f} ms per attempt", file=sys.stderr, flush=True)
         del runner
 
+    # ---- serving: continuous batching of steady-state solves -----------------
+    serving = None
+    if args.serve > 0:
+        from vulcan_amd import autotune
+        from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
+        from vulcan_amd.serving import ServingEngine, latency_summary
+        sizes = synth.config3_sizes(1005, args.serve)
+        reqs = [synth.make_column(1005 * 100003 + i, nz=a, ni=b_, n_rev=r, dt=1e-10,
+                                  name=f"req{i}", consistent=True)
+                for i, (a, b_, r) in enumerate(sizes)]
+        arrivals = synth.poisson_arrivals(1005, args.serve, args.serve_rate)
+        arrivals = arrivals - arrivals[0]
+        if args.factor == "auto":
+            autotune.tune_columns(reqs, verbose=True)
+        eng = ServingEngine(hbm_budget_bytes=200e9, cfg=RunConfig(), chunk=25)
+        t0 = time.perf_counter()
+        res = eng.run(reqs, arrivals, report_every=30.0)
+        wall = time.perf_counter() - t0
+        summ = latency_summary(res)
+        alone = []
+        for i in range(min(args.serve_alone, len(reqs))):
+            r1 = SteadyStateRunner([reqs[i]], RunConfig())
+            t1 = time.perf_counter()
+            r1.run(max_attempts=100000, check_every=25, use_graph=True, report_every=30.0)
+            alone.append(time.perf_counter() - t1)
+        serving = {"requests": len(reqs), "arrival_rate_per_s": args.serve_rate,
+                   "hbm_budget_gb": 200, "slots": {str(k): v for k, v in eng.slot_counts.items()},
+                   "wall_s": round(wall, 3),
+                   "p50_latency_s": round(summ["p50_s"], 3), "p99_latency_s": round(summ["p99_s"], 3),
+                   "throughput_solves_per_s": round(summ["throughput_solves_per_s"], 4),
+                   "converged": summ["converged"], "mean_attempts": round(summ["attempts_mean"], 1),
+                   "one_at_a_time_solve_s": [round(a, 3) for a in alone],
+                   "sizes": "nz~U{60..200}, ni in {50,69,93}, nr=13*ni; steady state: relative "
+                            "change < 1e-4 or 5000 accepted steps"}
+        print(f"[bench] serving: {serving}", file=sys.stderr, flush=True)
+
     # ---- roofline of the dominant kernel (block factor) ---------------------
     fac_ms = kern_ms[1]
     fac_flops = B * factor_flops(NZ, NI)
@@ -265,6 +307,7 @@ def main() -> int:
         "finite": finite,
         "singular_columns": n_singular,
         "uneven_batch": uneven,
+        "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

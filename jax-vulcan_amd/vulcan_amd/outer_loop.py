@@ -179,8 +179,13 @@ class SteadyStateRunner:
     def done(self) -> bool:
         return bool(((self.status & ST_DONE) != 0).all().item())
 
-    def run(self, max_attempts: int = 100000, check_every: int = 50, use_graph: bool = False):
-        """Attempt until every column is done or max_attempts; returns attempts made."""
+    def run(self, max_attempts: int = 100000, check_every: int = 50, use_graph: bool = False,
+            report_every: float = 0.0):
+        """Attempt until every column is done or max_attempts; returns attempts made.
+        report_every > 0 prints progress after the check that crosses each interval (s)."""
+        import time
+        t0 = time.perf_counter()
+        next_report = report_every
         done_attempts = 0
         if use_graph and self.graph is None:
             self.capture(check_every)
@@ -194,6 +199,11 @@ class SteadyStateRunner:
                 done_attempts += check_every
             if self.done():
                 break
+            if report_every > 0 and time.perf_counter() - t0 >= next_report:
+                next_report += report_every
+                n_done = int(((self.status & ST_DONE) != 0).sum().item())
+                print(f"[steady] t={time.perf_counter() - t0:.0f}s attempts={done_attempts} "
+                      f"done={n_done}/{self.batch.ncols}", flush=True)
         torch.cuda.synchronize()
         return done_attempts
 

@@ -235,11 +235,14 @@ class ServingEngine:
             self.slot_counts[sl] = n
 
     def run(self, requests: Sequence, arrivals: Optional[Sequence[float]] = None,
-            realtime: bool = True, max_rounds: int = 10 ** 9) -> List[Result]:
+            realtime: bool = True, max_rounds: int = 10 ** 9,
+            report_every: float = 0.0) -> List[Result]:
         """Serve the requests; arrival times in seconds from the start (all 0 if None).
 
         With realtime=True a request is admitted only once the wall clock has
-        reached its arrival time; latency = finish - arrival."""
+        reached its arrival time; latency = finish - arrival.  report_every > 0
+        prints a progress line (completed / in flight / queued) after the
+        chunk that crosses each such interval."""
         reqs = list(requests)
         arr = np.zeros(len(reqs)) if arrivals is None else np.asarray(arrivals, dtype=np.float64)
         for i, r in enumerate(reqs):
@@ -252,6 +255,7 @@ class ServingEngine:
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
         rounds = 0
+        next_report = report_every
         while (pending or any(p.busy or p.queue for p in self.pools.values())) and rounds < max_rounds:
             rounds += 1
             now = clock()
@@ -276,6 +280,13 @@ class ServingEngine:
                 for s in done:
                     res = p.harvest(s, t_done)
                     results[res.index] = res
+            if report_every > 0 and time.perf_counter() - t0 >= next_report:
+                next_report += report_every
+                n_done = sum(r is not None for r in results)
+                n_busy = sum(len(p.busy) for p in self.pools.values())
+                n_q = sum(len(p.queue) for p in self.pools.values()) + len(pending)
+                print(f"[serve] t={time.perf_counter() - t0:.0f}s done={n_done} in_flight={n_busy} "
+                      f"waiting={n_q} chunks={rounds}", flush=True)
         for p in self.pools.values():
             p.stream.synchronize()
         return [r for r in results if r is not None]
