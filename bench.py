@@ -67,11 +67,11 @@ def main() -> int:
                          "0 skips it)")
     ap.add_argument("--uneven-attempts", type=int, default=10,
                     help="steady-state attempts per column timed in the uneven leg")
-    ap.add_argument("--serve", type=int, default=0,
+    ap.add_argument("--serve", type=int, default=8,
                     help="serving leg: this many steady-state solve requests (sizes of the "
                          "uneven batch, Poisson arrivals at --serve-rate per s, 200 GB budget)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
-    ap.add_argument("--serve-alone", type=int, default=1,
+    ap.add_argument("--serve-alone", type=int, default=0,
                     help="requests also solved one at a time, for the comparison")
     ap.add_argument("--factor", choices=("auto", "tile", "rows"), default="auto",
                     help="block-factor kernel for the nz=150 / ni=93 bucket (auto: the "

@@ -124,6 +124,11 @@ struct vk_batch_s {
   // slot batch (continuous batching): per-slot capacity and network table
   bool slots_mode = false;
   int cap_nz = 0, cap_ni = 0, cap_S = 0, cap_T = 0;
+  // one helper stream per SL bucket (default priority) and fork/join events:
+  // the buckets of an uneven batch run concurrently
+  std::vector<hipStream_t> aux;
+  hipEvent_t ev_fork = nullptr;
+  std::vector<hipEvent_t> ev_join;
 };
 
 namespace {
@@ -469,6 +474,9 @@ int vk_batch_set_column(vk_batch b, int slot, int nz, vk_network net, void* stre
 
 int vk_batch_destroy(vk_batch b) {
   if (!b) return VK_OK;
+  for (hipStream_t a : b->aux) (void)hipStreamDestroy(a);
+  for (hipEvent_t e : b->ev_join) (void)hipEventDestroy(e);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   if (b->d_cols) (void)hipFree(b->d_cols);
   if (b->d_nets) (void)hipFree(b->d_nets);
   delete b;
@@ -622,6 +630,30 @@ size_t vk_ros2_workspace_bytes(vk_batch b) {
          align_up<double>(b->k_doubles);
 }
 
+// Helper streams and events for running the SL buckets of a batch
+// concurrently; created on first use.  false: run the buckets in order.
+static bool ensure_aux(vk_batch b) {
+  if (b->slots.size() < 2) return false;
+  if (b->aux.size() == b->slots.size()) return true;
+  for (size_t i = b->aux.size(); i < b->slots.size(); ++i) {
+    hipStream_t a = nullptr;
+    hipEvent_t e = nullptr;
+    if (hipStreamCreateWithFlags(&a, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      if (a) (void)hipStreamDestroy(a);
+      return false;
+    }
+    b->aux.push_back(a);
+    b->ev_join.push_back(e);
+  }
+  if (!b->ev_fork && hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return true;
+}
+
 static int ros2_step_impl(vk_batch b, const double* y, const double* k, const double* atm,
                           const double* dt, const uint8_t* fix_mask, double* sol, double* delta,
                           void* ws, int32_t* status, const int* active, hipStream_t st) {
@@ -644,43 +676,60 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
                                              active, st))
       return rc;
   }
-  for (int sl : b->slots) {
+  // The kernel sequence of one SL bucket on stream s.
+  auto run_bucket = [&](int sl, hipStream_t s) -> int {
     int rc;
     {
-      KernelTimer t(0, st);
+      KernelTimer t(0, s);
       if ((rc = vk_launch_chem_layer(sl, M_LHS, b->d_cols, b->ncols, b->max_nz, b->max_S,
                                      b->max_T, b->d_nets, y, nullptr, atm, kT, dt, fix_mask,
-                                     nullptr, rhs_y, blk, sup, sub, active, 1, st)))
+                                     nullptr, rhs_y, blk, sup, sub, active, 1, s)))
         return rc;
     }
     {
-      KernelTimer t(1, st);
+      KernelTimer t(1, s);
       if ((rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, blk, 1, 0, sup, sub, blk, rhs_y,
-                                    rmod, (int*)status, active, st)))
+                                    rmod, (int*)status, active, s)))
         return rc;
     }
     SolveEpi e1{1, r, 0.0, y, nullptr, nullptr, yk2, nullptr};
     {
-      KernelTimer t(2, st);
+      KernelTimer t(2, s);
       if ((rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, rmod, rmod, k1, 1, e1,
-                                   active, st)))
+                                   active, s)))
         return rc;
     }
     {
-      KernelTimer t(3, st);
+      KernelTimer t(3, s);
       if ((rc = vk_launch_chem_layer(sl, M_RHS2, b->d_cols, b->ncols, b->max_nz, b->max_S,
                                      b->max_T, b->d_nets, yk2, nullptr, atm, kT, dt, fix_mask, k1,
-                                     rhs2, nullptr, nullptr, nullptr, active, 1, st)))
+                                     rhs2, nullptr, nullptr, nullptr, active, 1, s)))
         return rc;
     }
     SolveEpi e2{2, 3.0 / (2.0 * r), 1.0 / (2.0 * r), y, k1, yk2, sol, delta};
     {
-      KernelTimer t(4, st);
+      KernelTimer t(4, s);
       if ((rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, rhs2, rmod, k2, 0, e2,
-                                   active, st)))
+                                   active, s)))
         return rc;
     }
+    return VK_OK;
+  };
+  if (ensure_aux(b)) {
+    // fork: every bucket's stream waits for the work before this point on st
+    VK_CHECK_HIP(hipEventRecord(b->ev_fork, st));
+    for (size_t i = 0; i < b->slots.size(); ++i) {
+      VK_CHECK_HIP(hipStreamWaitEvent(b->aux[i], b->ev_fork, 0));
+      if (int rc = run_bucket(b->slots[i], b->aux[i])) return rc;
+      VK_CHECK_HIP(hipEventRecord(b->ev_join[i], b->aux[i]));
+    }
+    // join: st waits for every bucket
+    for (size_t i = 0; i < b->slots.size(); ++i)
+      VK_CHECK_HIP(hipStreamWaitEvent(st, b->ev_join[i], 0));
+    return VK_OK;
   }
+  for (int sl : b->slots)
+    if (int rc = run_bucket(sl, st)) return rc;
   return VK_OK;
 }
 

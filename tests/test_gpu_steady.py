@@ -90,3 +90,21 @@ def test_uneven_batch_is_bitwise_alone():
         ra = alone.results()
         assert np.array_equal(ra["y"][0], rm["y"][i]), i
         assert ra["attempts"][0] == rm["attempts"][i]
+
+
+def test_uneven_batch_graph_replay_equals_eager():
+    """Three size buckets run on forked streams inside each attempt; captured
+    in one HIP graph (fork / join through events) and replayed, the batch
+    gives exactly the eager results."""
+    from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
+    cols = [_small(41, nz=18, ni=12, n_rev=40), _small(42, nz=25, ni=20, n_rev=60),
+            _small(43, nz=12, ni=37, n_rev=90), _small(44, nz=20, ni=14, n_rev=40)]
+    cfg = RunConfig(count_max=300)
+    eager = SteadyStateRunner(cols, cfg)
+    eager.run(max_attempts=600, check_every=50)
+    graph = SteadyStateRunner(cols, cfg)
+    graph.run(max_attempts=600, check_every=50, use_graph=True)
+    a, b = eager.results(), graph.results()
+    for i in range(len(cols)):
+        assert np.array_equal(a["y"][i], b["y"][i]), i
+    assert np.array_equal(a["attempts"], b["attempts"])
