@@ -160,12 +160,15 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
 /* ---- launch-shape tuning (no counterpart in the reference) ----
  * Knob VK_TUNE_FACTOR picks the block-factor kernel per SL bucket
  * (SL = ceil(ni/16)): VK_FACTOR_TILE (one 256-thread workgroup per column,
- * 16x16 register tiles, any SL <= 7) or VK_FACTOR_ROWS (one row per lane,
- * 64 or 128 lanes per column, SL <= 6).  Both give the same pivots; sums
- * differ only in order.  Process-wide; set it between launches. */
+ * 16x16 register tiles, any SL <= 7), VK_FACTOR_ROWS (half a row per lane,
+ * 64-192 lanes per column, SL <= 6; bitwise equal to TILE) or
+ * VK_FACTOR_BLOCKED (16-column panels eliminated in LDS, the rest updated
+ * by rank-16 FP64 MFMA products; same pivots, rounding differs in those
+ * products).  Process-wide; set it between launches. */
 #define VK_TUNE_FACTOR 0
 #define VK_FACTOR_TILE 0
 #define VK_FACTOR_ROWS 1
+#define VK_FACTOR_BLOCKED 2
 int vk_tune_set(int knob, int sl, int value);
 int vk_tune_get(int knob, int sl); /* -1 for an unknown knob / bucket */
 

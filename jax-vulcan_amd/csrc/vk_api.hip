@@ -73,6 +73,7 @@ int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag
                         int transpose, const double* sup_d, const double* sub_d, double* inv,
                         const double* rhs, double* rmod, int* status, const int* active,
                         hipStream_t stream);
+bool vk_factor_fuses_sweep(int sl);
 int vk_launch_bt_solve(int sl, const VkCol* cols, int ncols, const double* inv,
                        const double* sup_d, const double* sub_d, const double* rhs,
                        double* rscratch, double* x, int mode, SolveEpi epi, const int* active,
@@ -686,17 +687,21 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
                                      nullptr, rhs_y, blk, sup, sub, active, 1, s)))
         return rc;
     }
+    // The factor fuses the stage-1 forward sweep unless the blocked variant
+    // is selected; then the stage-1 solve runs both sweeps.
+    const bool fused = vk_factor_fuses_sweep(sl);
     {
       KernelTimer t(1, s);
-      if ((rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, blk, 1, 0, sup, sub, blk, rhs_y,
-                                    rmod, (int*)status, active, s)))
+      if ((rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, blk, 1, 0, sup, sub, blk,
+                                    fused ? rhs_y : nullptr, fused ? rmod : nullptr,
+                                    (int*)status, active, s)))
         return rc;
     }
     SolveEpi e1{1, r, 0.0, y, nullptr, nullptr, yk2, nullptr};
     {
       KernelTimer t(2, s);
-      if ((rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, rmod, rmod, k1, 1, e1,
-                                   active, s)))
+      if ((rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, fused ? rmod : rhs_y,
+                                   rmod, k1, fused ? 1 : 0, e1, active, s)))
         return rc;
     }
     {
@@ -855,7 +860,8 @@ int vk_tune_set(int knob, int sl, int value) {
     vk_set_error("vk_tune_set: unknown knob %d or bucket %d", knob, sl);
     return VK_ERR_ARG;
   }
-  if (value != VK_FACTOR_TILE && !(value == VK_FACTOR_ROWS && sl <= 6)) {
+  if (value != VK_FACTOR_TILE && !(value == VK_FACTOR_ROWS && sl <= 6) &&
+      value != VK_FACTOR_BLOCKED) {
     vk_set_error("vk_tune_set: factor variant %d not available for bucket %d", value, sl);
     return VK_ERR_UNSUPPORTED;
   }

@@ -428,10 +428,19 @@ int vk_launch_bt_factor_rows(int sl, const VkCol* cols, int ncols, const double*
                              const double* sub_d, double* inv, const double* rhs, double* rmod,
                              int* status, const int* active, hipStream_t stream);
 
+int vk_launch_bt_factor_blk(int sl, const VkCol* cols, int ncols, const double* diag,
+                            int diag_tile, int transpose, const double* sup_d, const double* sub_d,
+                            double* inv, int* status, const int* active, hipStream_t stream);
+
 // Factor variant per SL bucket: VK_FACTOR_TILE (256 threads, 16x16 register
-// tiles) or VK_FACTOR_ROWS (one row per lane).  Set by vk_tune_set.
+// tiles), VK_FACTOR_ROWS (half a row per lane) or VK_FACTOR_BLOCKED (16-column
+// panels, MFMA trailing updates; no fused forward sweep).  Set by vk_tune_set.
 int g_factor_variant[8] = {VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE,
                            VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE};
+
+bool vk_factor_fuses_sweep(int sl) {
+  return !(sl >= 1 && sl <= 7 && g_factor_variant[sl] == VK_FACTOR_BLOCKED);
+}
 
 int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag, int diag_tile,
                         int transpose, const double* sup_d, const double* sub_d, double* inv,
@@ -440,6 +449,14 @@ int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag
   if (sl >= 1 && sl <= 6 && g_factor_variant[sl] == VK_FACTOR_ROWS)
     return vk_launch_bt_factor_rows(sl, cols, ncols, diag, diag_tile, transpose, sup_d, sub_d,
                                     inv, rhs, rmod, status, active, stream);
+  if (sl >= 1 && sl <= 7 && g_factor_variant[sl] == VK_FACTOR_BLOCKED) {
+    if (rhs) {
+      vk_set_error("blocked block factor: the forward sweep is not fused");
+      return VK_ERR_UNSUPPORTED;
+    }
+    return vk_launch_bt_factor_blk(sl, cols, ncols, diag, diag_tile, transpose, sup_d, sub_d, inv,
+                                   status, active, stream);
+  }
   VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_bt_factor<SL_>, dim3(ncols), dim3(VK_NT), 0, stream,
                                         cols, ncols, diag, diag_tile, transpose, sup_d, sub_d,
                                         inv, rhs, rmod, status, active));

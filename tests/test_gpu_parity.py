@@ -302,3 +302,62 @@ def test_ros2_step_factor_variants_agree(cfg1):
     finally:
         lib.vk_tune_set(0, sl, 0)
     assert _layer_err(sols[1], sols[0]).max() < 1e-12
+
+
+@pytest.mark.parametrize("ni", [5, 21, 40, 50, 64, 69, 93, 100])
+def test_blocked_factor_matches_tile(ni):
+    """The blocked (panel + FP64 MFMA) factor takes the same pivots as the
+    tile factor; its inverses differ only by the rounding of the rank-16
+    panel products, amplified by the conditioning of the plain random block
+    (measured 2e-15 at ni=21 growing to 1.3e-12 at ni=50): 1e-10 of the
+    stack's scale, plain and transposed, and the solves match a dense
+    LAPACK solve."""
+    from vulcan_amd import _lib, solver
+    lib = _lib.load()
+    rng = np.random.default_rng(100 + ni)
+    nz = 10
+    diag = rng.standard_normal((2, nz, ni, ni)) + 4.0 * np.eye(ni)
+    diag[1, 2] = rng.standard_normal((ni, ni)) + 0.5 * np.eye(ni)  # needs pivoting
+    sup = rng.standard_normal((2, nz - 1, ni))
+    sub = rng.standard_normal((2, nz - 1, ni))
+    rhs = rng.standard_normal((2, nz, ni))
+    sl = (ni + 15) // 16
+    out = {}
+    try:
+        for variant in (0, 2):
+            _lib.check(lib.vk_tune_set(0, sl, variant), "vk_tune_set")
+            for tr in (False, True):
+                f = solver.factor_block_thomas_diag_offdiag(diag, sup, sub, transpose=tr)
+                x = solver.solve_block_thomas_diag_offdiag(f, rhs)
+                out[(variant, tr)] = (_np(solver.diag_inv_natural(f)), _np(x), _np(f.status))
+    finally:
+        lib.vk_tune_set(0, sl, 0)
+    for tr in (False, True):
+        i0, x0, s0 = out[(0, tr)]
+        i2, x2, s2 = out[(2, tr)]
+        d = np.abs(i2 - i0).max() / np.abs(i0).max()
+        print(f"ni={ni} transpose={tr}: inverse stack rel diff {d:.2e}")
+        assert d < 1e-10
+        assert np.array_equal(s0, s2)
+        assert np.abs(x2 - x0).max() / np.abs(x0).max() < 1e-9
+    for b in range(2):
+        ref = np.linalg.solve(_dense(diag[b], sup[b], sub[b]), rhs[b].reshape(-1)).reshape(nz, ni)
+        assert relerr(out[(2, False)][1][b], ref, 1e-12) < 1e-8
+
+
+def test_ros2_step_blocked_factor_agrees(cfg1):
+    """Whole Ros2 step with the blocked factor (unfused stage-1 solve) vs the
+    tile factor at a well-conditioned dt: 1e-12 per layer."""
+    from vulcan_amd import _lib, jax_step
+    lib = _lib.load()
+    c = cfg1
+    sl = (c.ni + 15) // 16
+    sols = []
+    try:
+        for variant in (0, 2):
+            _lib.check(lib.vk_tune_set(0, sl, variant), "vk_tune_set")
+            sol, _ = jax_step.jax_ros2_step(c.y, c.k, 1e-18, c.atm, c.net)
+            sols.append(_np(sol))
+    finally:
+        lib.vk_tune_set(0, sl, 0)
+    assert _layer_err(sols[1], sols[0]).max() < 1e-12
