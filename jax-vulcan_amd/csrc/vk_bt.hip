@@ -15,6 +15,8 @@
 // tid%16) holds the SL x SL register tile a[s][t] = A[16s+tr][16t+tc].
 #include "vk_common.h"
 
+#include <utility>
+
 namespace {
 
 constexpr int NW = VK_NT / 64;  // waves per workgroup
@@ -30,111 +32,184 @@ struct BtShared {
   int iperm[VK_G * SL];
 };
 
+// Lane KC of every 16-lane row, broadcast to the whole row (DPP row_newbcast).
+template <int KC>
+__device__ __forceinline__ double row_bcast(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, 0x150 + KC, 0xF, 0xF, false),
+                          __builtin_amdgcn_update_dpp(0, lo, 0x150 + KC, 0xF, 0xF, false));
+}
+
+// v = max(v, v moved by a DPP pattern); rows outside ROW_MASK keep v.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_fmax(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xF, false);
+  const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xF, false);
+  return fmax(v, __hiloint2double(hi2, lo2));
+}
+
+// Max over a wave whose 16-lane rows each hold one value in every lane (no
+// NaN): row_bcast 15 and 31 carry rows 0-2 into row 3, whose lane 63 is read
+// back as a scalar.
+__device__ __forceinline__ double rows_max(double v) {
+  v = dpp_fmax<0x142, 0xA>(v);
+  v = dpp_fmax<0x143, 0xC>(v);
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+
+template <int SL>
+struct GjCtx {
+  BtShared<SL>* sh;
+  int n, kb, bad;
+  unsigned live;  // bit s: row 16s+tr is a row of A not yet used as a pivot
+};
+
+// One Gauss-Jordan step with pivot column k = 16 kb + KC, which the column
+// rotation keeps in register column 0.  Returns false once k reaches n.
+//
+// The pivot is the first maximum of |A[i][k]| over the unused rows (the
+// getrf sequence).  NaN entries are never chosen; a step left without a
+// candidate flags the matrix.  Per-tile-row branches test bits of a mask
+// that passes through readfirstlane, so the compiler keeps them as separate
+// uniform branches instead of merging the register-tile stores into one
+// dynamically indexed store (which would move the tile to scratch memory).
+template <int SL, int KC>
+__device__ __forceinline__ bool gj_step(double (&a)[SL][SL], GjCtx<SL>& c) {
+  const int k = c.kb * VK_G + KC;
+  if (k >= c.n) return false;  // uniform: the last block may be partial
+  BtShared<SL>& sh = *c.sh;
+  const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int buf = k & 1;
+  // Column k of this thread's rows; pivot rank |A[i][k]|, -1 for rows that
+  // are not candidates.
+  double colv[SL];
+  double best = -1.0;
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    colv[s] = row_bcast<KC>(a[s][0]);
+    const double r = ((c.live >> s) & 1u) ? fabs(colv[s]) : -1.0;
+    best = r > best ? r : best;
+  }
+  const double m = rows_max(best);
+  // First row of the wave attaining m: smallest tile row s, then smallest tr.
+  int brow = 0x7fffffff;
+  if (m >= 0.0) {
+#pragma unroll
+    for (int s = SL - 1; s >= 0; --s) {
+      const unsigned long long bal = __ballot(((c.live >> s) & 1u) && fabs(colv[s]) == m);
+      if (bal) brow = s * VK_G + wave * 4 + (int)(__builtin_ctzll(bal) >> 4);
+    }
+  }
+  brow = __builtin_amdgcn_readfirstlane(brow);
+  // The owning row group publishes the whole candidate row (rotated order).
+  if (brow != 0x7fffffff) {
+    const unsigned wsm = __builtin_amdgcn_readfirstlane(1u << (brow >> 4));
+    if (tr == (brow & 15)) {
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        if (wsm & (1u << s)) {
+#pragma unroll
+          for (int t = 0; t < SL; ++t) sh.cand_row[buf][wave][t * VK_G + tc] = a[s][t];
+        }
+      }
+    }
+  }
+  if (lane == 0) { sh.cand_val[buf][wave] = m; sh.cand_idx[buf][wave] = brow; }
+  __syncthreads();
+  // Global pivot: best of the wave candidates.
+  double pval = sh.cand_val[buf][0];
+  int p = sh.cand_idx[buf][0];
+  int pw = 0;
+#pragma unroll
+  for (int w = 1; w < NW; ++w) {
+    const double v = sh.cand_val[buf][w];
+    const int r = sh.cand_idx[buf][w];
+    if (v > pval || (v == pval && r < p)) { pval = v; p = r; pw = w; }
+  }
+  p = __builtin_amdgcn_readfirstlane(p);
+  pw = __builtin_amdgcn_readfirstlane(pw);
+  if ((unsigned)p >= (unsigned)c.n) {  // no candidate left: keep indices in range
+    p = k;
+    c.bad = 1;
+  }
+  double u[SL];
+#pragma unroll
+  for (int t = 0; t < SL; ++t) u[t] = sh.cand_row[buf][pw][t * VK_G + tc];
+  const double piv = sh.cand_row[buf][pw][KC];
+  if (!(piv != 0.0) || !isfinite(piv)) c.bad = 1;
+  const double rpiv = 1.0 / piv;
+  if (tid == 0) sh.perm[k] = p;
+  // Rank-1 update of every row; column k and row p are rewritten below.
+  double l[SL];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) l[s] = colv[s] * rpiv;
+#pragma unroll
+  for (int s = 0; s < SL; ++s)
+#pragma unroll
+    for (int t = 0; t < SL; ++t) a[s][t] = fma(-l[s], u[t], a[s][t]);
+  if (tc == KC) {
+#pragma unroll
+    for (int s = 0; s < SL; ++s) a[s][0] = -l[s];
+  }
+  const unsigned psm = __builtin_amdgcn_readfirstlane(1u << (p >> 4));
+  if (tr == (p & 15)) {
+    double pr[SL];
+#pragma unroll
+    for (int t = 0; t < SL; ++t) pr[t] = u[t] * rpiv;
+    if (tc == KC) pr[0] = rpiv;
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      if (psm & (1u << s)) {
+#pragma unroll
+        for (int t = 0; t < SL; ++t) a[s][t] = pr[t];
+      }
+    }
+    c.live &= ~psm;
+  }
+  return true;
+}
+
+// The 16 steps of column block kb, in order, stopping at n.
+template <int SL, int... KC>
+__device__ __forceinline__ void gj_block(double (&a)[SL][SL], GjCtx<SL>& c,
+                                         std::integer_sequence<int, KC...>) {
+  (void)(gj_step<SL, KC>(a, c) && ...);
+}
+
 // Invert the register-resident matrix a (n x n inside the padded N x N tile)
 // in place.  On return a holds inv(A) in natural order.  Returns nonzero
 // in every thread when a zero or non-finite pivot was met.
+//
+// The tile columns rotate left by one after every 16-column block, so the
+// pivot column of step k = 16 kb + kc is register column 0 with kc static,
+// and the block loop stays rolled; after SL blocks the order is the
+// original one again.
 template <int SL>
 __device__ int gj_invert(double (&a)[SL][SL], int n, BtShared<SL>& sh) {
   const int tid = threadIdx.x;
   const int tr = tid >> 4, tc = tid & 15;
-  const int wave = tid >> 6;
-  const int lane = tid & 63;
-  unsigned pivmask = 0;  // bit s: row 16s+tr already used as a pivot
-  int bad = 0;
-
+  unsigned live = 0;
 #pragma unroll
+  for (int s = 0; s < SL; ++s)
+    if (s * VK_G + tr < n) live |= 1u << s;
+  GjCtx<SL> c{&sh, n, 0, 0, live};
+#pragma unroll 1
   for (int kb = 0; kb < SL; ++kb) {
-    for (int kc = 0; kc < VK_G; ++kc) {
-      const int k = kb * VK_G + kc;
-      if (k >= n) break;
-      const int buf = k & 1;
-      // Column k of this thread's rows, broadcast from lane (tr, kc).
-      double colv[SL];
+    c.kb = kb;
+    gj_block<SL>(a, c, std::make_integer_sequence<int, VK_G>{});
 #pragma unroll
-      for (int s = 0; s < SL; ++s) colv[s] = __shfl(a[s][kb], kc, 16);
-      // Best unpivoted row of this 16-lane group (first max: smallest row).
-      // A NaN entry ranks below every number but above "no candidate".
-      double bval = -1.0;
-      int brow = 0x7fffffff;
+    for (int s = 0; s < SL; ++s) {
+      const double first = a[s][0];
 #pragma unroll
-      for (int s = 0; s < SL; ++s) {
-        const int i = s * VK_G + tr;
-        const bool ok = (i < n) && !((pivmask >> s) & 1u);
-        double av = fabs(colv[s]);
-        if (!(av >= 0.0)) av = -0.5;
-        if (ok && av > bval) { bval = av; brow = i; }
-      }
-      // Reduce over the four groups of this wave.
-#pragma unroll
-      for (int off = 16; off < 64; off <<= 1) {
-        const double ov = __shfl_xor(bval, off, 64);
-        const int orow = __shfl_xor(brow, off, 64);
-        if (ov > bval || (ov == bval && orow < brow)) { bval = ov; brow = orow; }
-      }
-      brow = __builtin_amdgcn_readfirstlane(brow);
-      // The owning group publishes the whole candidate row.
-      if (brow != 0x7fffffff) {
-        const int ws = brow >> 4;
-        const bool own = (tr == (brow & 15));
-#pragma unroll
-        for (int s = 0; s < SL; ++s) {
-          if (s == ws && own) {
-#pragma unroll
-            for (int t = 0; t < SL; ++t) sh.cand_row[buf][wave][t * VK_G + tc] = a[s][t];
-          }
-        }
-      }
-      if (lane == 0) { sh.cand_val[buf][wave] = bval; sh.cand_idx[buf][wave] = brow; }
-      __syncthreads();
-      // Global pivot: best of the wave candidates.
-      double pval = sh.cand_val[buf][0];
-      int p = sh.cand_idx[buf][0];
-      int pw = 0;
-#pragma unroll
-      for (int w = 1; w < NW; ++w) {
-        const double v = sh.cand_val[buf][w];
-        const int r = sh.cand_idx[buf][w];
-        if (v > pval || (v == pval && r < p)) { pval = v; p = r; pw = w; }
-      }
-      p = __builtin_amdgcn_readfirstlane(p);
-      pw = __builtin_amdgcn_readfirstlane(pw);
-      if ((unsigned)p >= (unsigned)n) {  // cannot happen while k < n; keeps indices in range
-        p = k;
-        bad = 1;
-      }
-      double u[SL];
-#pragma unroll
-      for (int t = 0; t < SL; ++t) u[t] = sh.cand_row[buf][pw][t * VK_G + tc];
-      const double piv = sh.cand_row[buf][pw][k];
-      if (!(piv != 0.0) || !isfinite(piv)) bad = 1;
-      const double rpiv = 1.0 / piv;
-      if (tid == 0) sh.perm[k] = p;
-      // Rank-1 update of every row; column k and row p are rewritten below.
-      double l[SL];
-#pragma unroll
-      for (int s = 0; s < SL; ++s) l[s] = colv[s] * rpiv;
-#pragma unroll
-      for (int s = 0; s < SL; ++s)
-#pragma unroll
-        for (int t = 0; t < SL; ++t) a[s][t] = fma(-l[s], u[t], a[s][t]);
-      if (tc == kc) {
-#pragma unroll
-        for (int s = 0; s < SL; ++s) a[s][kb] = -l[s];
-      }
-      const int ps = p >> 4;
-      if (tr == (p & 15)) {
-#pragma unroll
-        for (int s = 0; s < SL; ++s) {
-          if (s == ps) {
-#pragma unroll
-            for (int t = 0; t < SL; ++t) a[s][t] = u[t] * rpiv;
-            if (tc == kc) a[s][kb] = rpiv;
-          }
-        }
-        pivmask |= 1u << ps;
-      }
+      for (int t = 0; t + 1 < SL; ++t) a[s][t] = a[s][t + 1];
+      a[s][SL - 1] = first;
     }
   }
+  const int bad = c.bad;
   // Padding indices map to themselves.
   for (int i = n + tid; i < VK_G * SL; i += VK_NT) sh.perm[i] = i;
   for (int i = tid; i < VK_G * SL; i += VK_NT) sh.iperm[i] = i;
@@ -209,12 +284,14 @@ __device__ inline double nat_elem(const double* blk, int ni, int i, int q, int t
 }  // namespace
 
 // Factor one column per workgroup.  diag is either natural (nz, ni, ni)
-// blocks at off_s*ni (diag_tile = 0) or tile-layout blocks at off_b
-// (diag_tile = 1, which may alias inv).  When rhs is non-null the forward
-// sweep r' is fused and written to rmod.
-template <int SL>
+// blocks at off_s*ni (TILE_IN false) or tile-layout blocks at off_b
+// (TILE_IN true, which may alias inv).  When rhs is non-null the forward
+// sweep r' is fused and written to rmod.  The two input layouts are separate
+// instantiations: the natural path's bounds and transposition arithmetic
+// would otherwise stay live beside the register tile.
+template <int SL, bool TILE_IN>
 __global__ __launch_bounds__(VK_NT) void k_bt_factor(
-    const VkCol* __restrict__ cols, int ncols, const double* diag, int diag_tile,
+    const VkCol* __restrict__ cols, int ncols, const double* diag,
     int transpose, const double* __restrict__ sup_d, const double* __restrict__ sub_d,
     double* inv, const double* __restrict__ rhs, double* __restrict__ rmod,
     int* __restrict__ status, const int* __restrict__ active) {
@@ -236,7 +313,7 @@ __global__ __launch_bounds__(VK_NT) void k_bt_factor(
     const double* Dj_nat = diag + col.off_s * ni + (int64_t)j * ni * ni;
     const double* Dj_tile = diag + col.off_b + (int64_t)j * BLK;
     if (j == 0) {
-      if (diag_tile) {
+      if constexpr (TILE_IN) {
         load_tile<SL>(a, Dj_tile);
       } else {
 #pragma unroll
@@ -281,15 +358,22 @@ __global__ __launch_bounds__(VK_NT) void k_bt_factor(
           }
         }
       }
+      // One tile row of D_j at a time, so the loads do not pile up beside
+      // the register tile.
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int s = 0; s < SL; ++s)
+      for (int s = 0; s < SL; ++s) {
 #pragma unroll
         for (int t = 0; t < SL; ++t) {
-          const double d = diag_tile
-              ? Dj_tile[(s * SL + t) * VK_NT + tid]
-              : nat_elem(Dj_nat, ni, s * VK_G + tr, t * VK_G + tc, transpose);
+          double d;
+          if constexpr (TILE_IN)
+            d = Dj_tile[(s * SL + t) * VK_NT + tid];
+          else
+            d = nat_elem(Dj_nat, ni, s * VK_G + tr, t * VK_G + tc, transpose);
           a[s][t] = d - (cv[s] * bv[t]) * a[s][t];
         }
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
     bad |= gj_invert<SL>(a, ni, sh);
     store_tile<SL>(a, inv + col.off_b + (int64_t)j * BLK);
@@ -457,9 +541,15 @@ int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag
     return vk_launch_bt_factor_blk(sl, cols, ncols, diag, diag_tile, transpose, sup_d, sub_d, inv,
                                    status, active, stream);
   }
-  VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_bt_factor<SL_>, dim3(ncols), dim3(VK_NT), 0, stream,
-                                        cols, ncols, diag, diag_tile, transpose, sup_d, sub_d,
-                                        inv, rhs, rmod, status, active));
+  if (diag_tile) {
+    VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_bt_factor<SL_, true>), dim3(ncols), dim3(VK_NT), 0,
+                                          stream, cols, ncols, diag, transpose, sup_d, sub_d,
+                                          inv, rhs, rmod, status, active));
+  } else {
+    VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_bt_factor<SL_, false>), dim3(ncols), dim3(VK_NT), 0,
+                                          stream, cols, ncols, diag, transpose, sup_d, sub_d,
+                                          inv, rhs, rmod, status, active));
+  }
   VK_CHECK_HIP(hipGetLastError());
   return VK_OK;
 }

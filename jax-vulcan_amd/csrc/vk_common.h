@@ -132,12 +132,23 @@ void vk_set_error(const char* fmt, ...);
 // device helpers
 // ---------------------------------------------------------------------------
 
-// Sum over the 16 lanes that share a grid row (lanes tr*16 .. tr*16+15).
+// v plus v rotated right by N lanes within each 16-lane row (DPP row_ror).
+template <int N>
+__device__ __forceinline__ double vk_row_ror_add(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x120 + N, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x120 + N, 0xF, 0xF, false);
+  return v + __hiloint2double(hi, lo);
+}
+
+// Sum over the 16 lanes that share a grid row (lanes tr*16 .. tr*16+15), in
+// every lane.  After the rotation by 8 a lane's value depends on its index
+// mod 8 only, so rotating by 4, 2, 1 next pairs lane i with lane i^4, i^2,
+// i^1: the butterfly sum ((v_i + v_i^8) + ...), the same in every lane.
 __device__ inline double vk_rowsum16(double v) {
-  v += __shfl_xor(v, 8, 16);
-  v += __shfl_xor(v, 4, 16);
-  v += __shfl_xor(v, 2, 16);
-  v += __shfl_xor(v, 1, 16);
+  v = vk_row_ror_add<8>(v);
+  v = vk_row_ror_add<4>(v);
+  v = vk_row_ror_add<2>(v);
+  v = vk_row_ror_add<1>(v);
   return v;
 }
 
