@@ -6,6 +6,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
+#include <numeric>
+#include <queue>
 #include <set>
 #include <vector>
 
@@ -166,6 +169,38 @@ const char* vk_last_error(void) { return g_err; }
 int vk_version(void) { return 100; }
 int vk_max_species(void) { return VK_G * 7; }
 
+// Longest-processing-time assignment of outputs to the VK_NT lanes of a
+// workgroup: costliest first, each to the least loaded lane (lowest lane on
+// ties).  Each lane's outputs come back in ascending id order.
+static std::vector<std::vector<int>> lpt_lanes(const std::vector<int>& cost) {
+  std::vector<int> order(cost.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  using Load = std::pair<long, int>;
+  std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+  for (int l = 0; l < VK_NT; ++l) heap.push({0, l});
+  std::vector<std::vector<int>> lanes(VK_NT);
+  for (int id : order) {
+    Load x = heap.top();
+    heap.pop();
+    lanes[x.second].push_back(id);
+    heap.push({x.first + cost[id], x.second});
+  }
+  for (auto& v : lanes) std::sort(v.begin(), v.end());
+  return lanes;
+}
+
+// Lane-interleaved record table from per-lane record lists, padded with
+// no-op records to the longest lane.  Returns the records per lane.
+static int interleave(const std::vector<std::vector<VkRec>>& per_lane, std::vector<VkRec>& out) {
+  size_t k = 1;
+  for (auto& v : per_lane) k = std::max(k, v.size());
+  out.assign(k * VK_NT, VkRec{0.0, -1, -1});
+  for (int l = 0; l < VK_NT; ++l)
+    for (size_t j = 0; j < per_lane[l].size(); ++j) out[j * VK_NT + l] = per_lane[l][j];
+  return (int)k;
+}
+
 int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
                       const int64_t* product_idx, const double* reactant_stoich,
                       const double* product_stoich, const uint8_t* is_three_body,
@@ -243,6 +278,68 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
     for (auto& x : JL[e]) { ji.push_back(x.first); jc.push_back(x.second); }
   }
   jp[JL.size()] = (int)ji.size();
+  // Jacobian tile records: one output per entry of the SL x SL tile layout
+  // (padding entries have no terms), each with its list in order.
+  const int SL = vk_slots(ni);
+  const int ntile = SL * SL * VK_NT;
+  auto tile_elem = [&](int idx, int& i, int& q) {
+    const int blk = idx / VK_NT, rc = idx % VK_NT;
+    i = (blk / SL) * VK_G + rc / VK_G;
+    q = (blk % SL) * VK_G + rc % VK_G;
+  };
+  std::vector<int> jcost(ntile);
+  for (int idx = 0; idx < ntile; ++idx) {
+    int i, q;
+    tile_elem(idx, i, q);
+    const size_t len = (i < ni && q < ni) ? JL[(size_t)i * ni + q].size() : 0;
+    jcost[idx] = (int)std::max<size_t>(len, 1);
+  }
+  std::vector<std::vector<VkRec>> jl(VK_NT);
+  {
+    const auto lanes = lpt_lanes(jcost);
+    for (int l = 0; l < VK_NT; ++l)
+      for (int idx : lanes[l]) {
+        int i, q;
+        tile_elem(idx, i, q);
+        if (i >= ni || q >= ni || JL[(size_t)i * ni + q].empty()) {
+          jl[l].push_back({0.0, -1, idx});
+          continue;
+        }
+        const auto& lst = JL[(size_t)i * ni + q];
+        for (size_t m = 0; m < lst.size(); ++m)
+          jl[l].push_back({lst[m].second, lst[m].first, m + 1 == lst.size() ? idx : -1});
+      }
+  }
+  std::vector<VkRec> jrec;
+  const int jrec_k = interleave(jl, jrec);
+  // loss / production chunk records: loss chunks of species 0..ni-1, then
+  // production chunks, VK_FCHUNK list entries each
+  std::vector<int> lcp(ni + 1), pcp(ni + 1);
+  std::vector<std::vector<VkRec>> chunks;
+  auto add_chunks = [&](const std::vector<std::pair<int, double>>& lst) {
+    for (size_t b = 0; b < lst.size(); b += VK_FCHUNK) {
+      std::vector<VkRec> ch;
+      const size_t e = std::min(lst.size(), b + VK_FCHUNK);
+      for (size_t m = b; m < e; ++m)
+        ch.push_back({lst[m].second, lst[m].first / T, m + 1 == e ? (int)chunks.size() : -1});
+      chunks.push_back(ch);
+    }
+  };
+  for (int i = 0; i < ni; ++i) { lcp[i] = (int)chunks.size(); add_chunks(L[i]); }
+  lcp[ni] = (int)chunks.size();
+  for (int i = 0; i < ni; ++i) { pcp[i] = (int)chunks.size(); add_chunks(P[i]); }
+  pcp[ni] = (int)chunks.size();
+  const int nchunk = (int)chunks.size();
+  std::vector<std::vector<VkRec>> fl(VK_NT);
+  {
+    std::vector<int> fcost(nchunk);
+    for (int ch = 0; ch < nchunk; ++ch) fcost[ch] = (int)chunks[ch].size();
+    const auto lanes = lpt_lanes(fcost);
+    for (int l = 0; l < VK_NT; ++l)
+      for (int ch : lanes[l]) fl[l].insert(fl[l].end(), chunks[ch].begin(), chunks[ch].end());
+  }
+  std::vector<VkRec> frec;
+  const int frec_k = interleave(fl, frec);
   // one device allocation for every table
   size_t off = 0;
   auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
@@ -254,7 +351,9 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
                o_pi = take(std::max<size_t>(pi_.size(), 1) * 4),
                o_pc = take(std::max<size_t>(pc.size(), 1) * 8), o_jp = take(jp.size() * 4),
                o_ji = take(std::max<size_t>(ji.size(), 1) * 4),
-               o_jc = take(std::max<size_t>(jc.size(), 1) * 8);
+               o_jc = take(std::max<size_t>(jc.size(), 1) * 8),
+               o_jr = take(jrec.size() * sizeof(VkRec)), o_fr = take(frec.size() * sizeof(VkRec)),
+               o_lcp = take(lcp.size() * 4), o_pcp = take(pcp.size() * 4);
   std::vector<char> host(off, 0);
   auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) memcpy(&host[o], src, bytes); };
   put(o_pidx, p_idx.data(), p_idx.size() * 4);
@@ -271,6 +370,10 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
   put(o_jp, jp.data(), jp.size() * 4);
   put(o_ji, ji.data(), ji.size() * 4);
   put(o_jc, jc.data(), jc.size() * 8);
+  put(o_jr, jrec.data(), jrec.size() * sizeof(VkRec));
+  put(o_fr, frec.data(), frec.size() * sizeof(VkRec));
+  put(o_lcp, lcp.data(), lcp.size() * 4);
+  put(o_pcp, pcp.data(), pcp.size() * 4);
   vk_network_s* n = new vk_network_s();
   if (hipMalloc(&n->buf, off) != hipSuccess) {
     delete n;
@@ -299,6 +402,14 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
   n->dev.jac_ptr = (const int*)(b + o_jp);
   n->dev.jac_item = (const int*)(b + o_ji);
   n->dev.jac_c = (const double*)(b + o_jc);
+  n->dev.jrec = (const VkRec*)(b + o_jr);
+  n->dev.frec = (const VkRec*)(b + o_fr);
+  n->dev.lc_ptr = (const int*)(b + o_lcp);
+  n->dev.pc_ptr = (const int*)(b + o_pcp);
+  n->dev.jrec_k = jrec_k;
+  n->dev.frec_k = frec_k;
+  n->dev.nchunk = nchunk;
+  n->dev.pad2 = 0;
   n->ni = ni; n->nr = nr; n->T = T;
   *out = n;
   return VK_OK;

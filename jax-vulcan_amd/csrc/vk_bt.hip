@@ -288,9 +288,11 @@ __device__ inline double nat_elem(const double* blk, int ni, int i, int q, int t
 // (TILE_IN true, which may alias inv).  When rhs is non-null the forward
 // sweep r' is fused and written to rmod.  The two input layouts are separate
 // instantiations: the natural path's bounds and transposition arithmetic
-// would otherwise stay live beside the register tile.
+// would otherwise stay live beside the register tile.  The tile-input kernel
+// is held to 128 VGPRs for SL <= 6 (four columns per CU); what it spills is
+// per-layer state, outside the pivot steps.
 template <int SL, bool TILE_IN>
-__global__ __launch_bounds__(VK_NT) void k_bt_factor(
+__global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(TILE_IN && SL <= 6 ? 4 : 1))) void k_bt_factor(
     const VkCol* __restrict__ cols, int ncols, const double* diag,
     int transpose, const double* __restrict__ sup_d, const double* __restrict__ sub_d,
     double* inv, const double* __restrict__ rhs, double* __restrict__ rmod,

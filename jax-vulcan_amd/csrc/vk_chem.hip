@@ -166,7 +166,8 @@ __device__ inline double block_sum(double v, double* red) {
 
 }  // namespace
 
-// Dynamic shared memory: rate[S] + drate[S*T] (Jacobian modes) + 3 y rows.
+// Dynamic shared memory: rate[S] + drate[S*T] (Jacobian modes) + 3 y rows
+// + the loss / production chunk sums.
 template <int SL>
 __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
   extern __shared__ double smem[];
@@ -174,6 +175,7 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
   __shared__ double red[4];
   __shared__ double dd_s[VK_G * SL];  // A+Ai (+ bottom vdep for M_LHS) per species
   __shared__ double bt_s[VK_G * SL];  // bottom vdep term, kept apart for M_JY
+  __shared__ int fix_s[VK_G * SL];    // M_LHS: row i of layer z is fixed
   const int c = blockIdx.y;
   const int z = blockIdx.x;
   if (c >= L.ncols) return;
@@ -259,6 +261,23 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
     }
   }
   __syncthreads();
+  // loss / production chunk sums from the lane-balanced records
+  double* part = yrow + 3 * (ni + 1);
+  if (want_rhs) {
+    const VkRec* fr = net.frec + tid;
+    double acc = 0.0;
+#pragma unroll 4
+    for (int kk = 0; kk < net.frec_k; ++kk) {
+#pragma clang fp contract(off)
+      const VkRec rc = fr[(int64_t)kk * VK_NT];
+      if (rc.item >= 0) acc = acc + rc.c * rate[rc.item];
+      if (rc.dest >= 0) {
+        part[rc.dest] = acc;
+        acc = 0.0;
+      }
+    }
+    __syncthreads();
+  }
 
   const double ysm = want_diff ? ysum3[0] : 0.0;
   const double ys0 = want_diff ? ysum3[1] : 0.0;
@@ -276,10 +295,8 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
     double f = 0.0;
     if (want_rhs) {
       double loss = 0.0, prod = 0.0;
-      for (int e = net.loss_ptr[i]; e < net.loss_ptr[i + 1]; ++e)
-        loss = loss + net.loss_c[e] * rate[net.loss_item[e] / T];
-      for (int e = net.prod_ptr[i]; e < net.prod_ptr[i + 1]; ++e)
-        prod = prod + net.prod_c[e] * rate[net.prod_item[e] / T];
+      for (int ch = net.lc_ptr[i]; ch < net.lc_ptr[i + 1]; ++ch) loss = loss + part[ch];
+      for (int ch = net.pc_ptr[i]; ch < net.pc_ptr[i + 1]; ++ch) prod = prod + part[ch];
       f = prod - loss;
     }
     const int64_t e_out = col.off_s + (int64_t)z * ni + i;
@@ -303,6 +320,7 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
       }
       const bool fixed = (L.fix != nullptr) && L.fix[e_out];
       if (L.mode == M_LHS) {
+        fix_s[i] = fixed ? 1 : 0;
         const double tot = f + diff;
         L.out0[e_out] = fixed ? 0.0 : tot;
         if (z < nz - 1) {
@@ -327,27 +345,29 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
   __syncthreads();
   // Jacobian entries from the entry-major CSR (chem.py:187-208)
   if (L.mode == M_LHS) {
-    // tile-layout D = c0 I - J - diag(dd), fix rows -> c0 e_i
+    // tile-layout D = c0 I - J - diag(dd), fix rows -> c0 e_i; each tile
+    // entry is closed by one record of one lane (the jac list of (i, q) in
+    // order), which writes it to its tile slot
     constexpr int BLK = vk_blk<SL>();
     double* dst = L.out_blk + col.off_b + (int64_t)z * BLK;
-    const int tr = tid >> 4, tc = tid & 15;
-#pragma unroll
-    for (int s = 0; s < SL; ++s) {
-#pragma unroll
-      for (int t = 0; t < SL; ++t) {
+    const VkRec* jr = net.jrec + tid;
+    double J = 0.0;
+#pragma unroll 4
+    for (int kk = 0; kk < net.jrec_k; ++kk) {
 #pragma clang fp contract(off)
-        const int i = s * VK_G + tr, q = t * VK_G + tc;
+      const VkRec rc = jr[(int64_t)kk * VK_NT];
+      if (rc.item >= 0) J = J + rc.c * drate[rc.item];
+      if (rc.dest >= 0) {
+        const int blk = rc.dest / VK_NT, rr = rc.dest % VK_NT;
+        const int i = (blk / SL) * VK_G + rr / VK_G, q = (blk % SL) * VK_G + rr % VK_G;
         double v = 0.0;
         if (i < ni && q < ni) {
-          const int e = i * ni + q;
-          double J = 0.0;
-          for (int p = net.jac_ptr[e]; p < net.jac_ptr[e + 1]; ++p)
-            J = J + net.jac_c[p] * drate[net.jac_item[p]];
           v = (i == q ? c0 : 0.0) - J;
           if (i == q) v = v + (-dd_s[i]);
-          if (L.fix && L.fix[col.off_s + (int64_t)z * ni + i]) v = (i == q) ? c0 : 0.0;
+          if (fix_s[i]) v = (i == q) ? c0 : 0.0;
         }
-        dst[(s * SL + t) * VK_NT + tid] = v;
+        dst[rc.dest] = v;
+        J = 0.0;
       }
     }
   } else {
@@ -397,10 +417,13 @@ int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max
   LayerArgs L{cols, ncols, nets, y, M, atm, k, dt, fix, k1, out0, out_blk, out_sup, out_sub,
               active, mode, k_layer_major};
   const bool want_jac = (mode == M_CHEM_JAC || mode == M_LHS || mode == M_JY);
-  // ni+1 <= 16*sl+1 for the y rows
+  // ni+1 <= 16*sl+1 for the y rows; loss / production chunks: at most
+  // 2*S*T list entries in at most 2*ni lists
+  const size_t n16 = (size_t)VK_G * (sl > 0 ? sl : 1);
+  const size_t nchunk_max = (2 * (size_t)max_S * max_T + VK_FCHUNK - 1) / VK_FCHUNK + 2 * n16;
   const size_t shmem =
       sizeof(double) * ((size_t)max_S + (want_jac ? (size_t)max_S * max_T : 0) +
-                        3 * (size_t)(VK_G * (sl > 0 ? sl : 1) + 1));
+                        3 * (n16 + 1) + nchunk_max);
   if (shmem > 160 * 1024) {
     vk_set_error("chem layer needs %zu bytes of LDS (S=%d, T=%d)", shmem, max_S, max_T);
     return VK_ERR_UNSUPPORTED;

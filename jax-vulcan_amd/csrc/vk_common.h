@@ -67,8 +67,32 @@ __host__ __device__ inline VkAtmOff vk_atm_offsets(int nz, int ni) {
 //   jac        : entry-major lists over (out, in) pairs of
 //                (row*T + in_slot, signed out stoich) in ascending
 //                (row, out slot, in slot) order.
+// Lane-balanced record streams, built once per network from the lists
+// above.  Record k of lane l sits at [k * VK_NT + l], so a wave's loads are
+// contiguous, and lanes carry near-equal record counts however uneven the
+// lists are.  A record adds c * src[item] to the lane's running sum (item
+// -1: no term); a record with dest >= 0 closes the sum into output dest and
+// restarts it at zero.
+//   jrec: one output per tile entry of D (dest = tile index), whose terms
+//         are its jac list in order, so each entry is the same sequential
+//         sum as a walk of that list (src = drate);
+//   frec: one output per chunk of at most VK_FCHUNK consecutive entries of a
+//         loss or production list (dest = chunk, src = rate, item = row);
+//         lc_ptr / pc_ptr give species i's loss / production chunks, summed
+//         in list order.
+struct VkRec {
+  double c;
+  int item, dest;
+};
+constexpr int VK_FCHUNK = 16;
+
 struct VkNetDev {
   int ni, S, T, pad;
+  const VkRec* jrec;     // (jrec_k * VK_NT)
+  const VkRec* frec;     // (frec_k * VK_NT)
+  const int* lc_ptr;     // (ni+1) loss chunks of species i
+  const int* pc_ptr;     // (ni+1) production chunks of species i
+  int jrec_k, frec_k, nchunk, pad2;
   const int* r_idx;      // (S*T) reactant species, PAD = ni
   const double* r_st;    // (S*T)
   const int* p_idx;      // (S*T) product species, PAD = ni
