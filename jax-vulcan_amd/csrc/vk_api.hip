@@ -195,7 +195,7 @@ static std::vector<std::vector<int>> lpt_lanes(const std::vector<int>& cost) {
 static int interleave(const std::vector<std::vector<VkRec>>& per_lane, std::vector<VkRec>& out) {
   size_t k = 1;
   for (auto& v : per_lane) k = std::max(k, v.size());
-  out.assign(k * VK_NT, VkRec{0.0, -1, -1});
+  out.assign(k * VK_NT, VkRec{0.0f, vk_rec_word(VK_REC_NONE_ITEM, VK_REC_NONE_DEST)});
   for (int l = 0; l < VK_NT; ++l)
     for (size_t j = 0; j < per_lane[l].size(); ++j) out[j * VK_NT + l] = per_lane[l][j];
   return (int)k;
@@ -238,6 +238,21 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
       p_st[r * T + s] = product_stoich[r * T + s];
     }
     is3[r] = is_three_body ? (is_three_body[r] != 0) : 0;
+  }
+  // The chemistry record streams hold coefficients as float and pack row
+  // indices in 18 bits, tile / chunk indices in 14.
+  for (int e = 0; e < S * T; ++e) {
+    if ((double)(float)r_st[e] != r_st[e] || (double)(float)p_st[e] != p_st[e]) {
+      vk_set_error("vk_network_create: stoichiometric coefficient %g is not exact in float",
+                   (double)(float)r_st[e] != r_st[e] ? r_st[e] : p_st[e]);
+      return VK_ERR_UNSUPPORTED;
+    }
+  }
+  if ((int64_t)S * T >= (int64_t)VK_REC_NONE_ITEM ||
+      (int64_t)vk_slots(ni) * vk_slots(ni) * VK_NT >= (int64_t)VK_REC_NONE_DEST ||
+      2 * (int64_t)S * T / VK_FCHUNK + 2 * ni >= (int64_t)VK_REC_NONE_DEST) {
+    vk_set_error("vk_network_create: network too large for the record streams (S*T=%d)", S * T);
+    return VK_ERR_UNSUPPORTED;
   }
   // species-major loss / production lists in ascending (row, slot) order
   std::vector<int> lp(ni + 1, 0), pp(ni + 1, 0);
@@ -302,12 +317,14 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
         int i, q;
         tile_elem(idx, i, q);
         if (i >= ni || q >= ni || JL[(size_t)i * ni + q].empty()) {
-          jl[l].push_back({0.0, -1, idx});
+          jl[l].push_back({0.0f, vk_rec_word(VK_REC_NONE_ITEM, (uint32_t)idx)});
           continue;
         }
         const auto& lst = JL[(size_t)i * ni + q];
         for (size_t m = 0; m < lst.size(); ++m)
-          jl[l].push_back({lst[m].second, lst[m].first, m + 1 == lst.size() ? idx : -1});
+          jl[l].push_back({(float)lst[m].second,
+                           vk_rec_word((uint32_t)lst[m].first,
+                                       m + 1 == lst.size() ? (uint32_t)idx : VK_REC_NONE_DEST)});
       }
   }
   std::vector<VkRec> jrec;
@@ -321,7 +338,9 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
       std::vector<VkRec> ch;
       const size_t e = std::min(lst.size(), b + VK_FCHUNK);
       for (size_t m = b; m < e; ++m)
-        ch.push_back({lst[m].second, lst[m].first / T, m + 1 == e ? (int)chunks.size() : -1});
+        ch.push_back({(float)lst[m].second,
+                      vk_rec_word((uint32_t)(lst[m].first / T),
+                                  m + 1 == e ? (uint32_t)chunks.size() : VK_REC_NONE_DEST)});
       chunks.push_back(ch);
     }
   };

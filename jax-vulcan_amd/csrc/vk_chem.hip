@@ -244,21 +244,16 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
     }
   }
   // column sums of y for the eddy terms (gas species only)
-  if (want_diff) {
-    for (int rr = 0; rr < 3; ++rr) {
-      const int zz = z - 1 + rr;
-      double v = 0.0;
-      if (zz >= 0 && zz < nz) {
-        // sequential order within a thread keeps the sum reproducible
-        if (tid == 0) {
-          for (int i = 0; i < ni; ++i)
-            if (av.gas[i] != 0.0) v += yrow[rr * (ni + 1) + i];
-          ysum3[rr] = v;
-        }
-      } else if (tid == 0) {
-        ysum3[rr] = 1.0;
-      }
+  if (want_diff && (tid & 63) == 0 && (tid >> 6) < 3) {
+    // row rr = wave: a sequential sum in one lane keeps it reproducible;
+    // non-gas species add +0.0, which leaves the sum unchanged
+    const int rr = tid >> 6, zz = z - 1 + rr;
+    double v = 1.0;
+    if (zz >= 0 && zz < nz) {
+      v = 0.0;
+      for (int i = 0; i < ni; ++i) v += (av.gas[i] != 0.0) ? yrow[rr * (ni + 1) + i] : 0.0;
     }
+    ysum3[rr] = v;
   }
   __syncthreads();
   // loss / production chunk sums from the lane-balanced records
@@ -270,9 +265,10 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
     for (int kk = 0; kk < net.frec_k; ++kk) {
 #pragma clang fp contract(off)
       const VkRec rc = fr[(int64_t)kk * VK_NT];
-      if (rc.item >= 0) acc = acc + rc.c * rate[rc.item];
-      if (rc.dest >= 0) {
-        part[rc.dest] = acc;
+      const uint32_t item = vk_rec_item(rc.w), dest = vk_rec_dest(rc.w);
+      if (item != VK_REC_NONE_ITEM) acc = acc + (double)rc.c * rate[item];
+      if (dest != VK_REC_NONE_DEST) {
+        part[dest] = acc;
         acc = 0.0;
       }
     }
@@ -356,9 +352,10 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
     for (int kk = 0; kk < net.jrec_k; ++kk) {
 #pragma clang fp contract(off)
       const VkRec rc = jr[(int64_t)kk * VK_NT];
-      if (rc.item >= 0) J = J + rc.c * drate[rc.item];
-      if (rc.dest >= 0) {
-        const int blk = rc.dest / VK_NT, rr = rc.dest % VK_NT;
+      const uint32_t item = vk_rec_item(rc.w), dest = vk_rec_dest(rc.w);
+      if (item != VK_REC_NONE_ITEM) J = J + (double)rc.c * drate[item];
+      if (dest != VK_REC_NONE_DEST) {
+        const int blk = (int)dest / VK_NT, rr = (int)dest % VK_NT;
         const int i = (blk / SL) * VK_G + rr / VK_G, q = (blk % SL) * VK_G + rr % VK_G;
         double v = 0.0;
         if (i < ni && q < ni) {
@@ -366,7 +363,7 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
           if (i == q) v = v + (-dd_s[i]);
           if (fix_s[i]) v = (i == q) ? c0 : 0.0;
         }
-        dst[rc.dest] = v;
+        dst[dest] = v;
         J = 0.0;
       }
     }

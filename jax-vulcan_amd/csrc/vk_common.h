@@ -70,9 +70,9 @@ __host__ __device__ inline VkAtmOff vk_atm_offsets(int nz, int ni) {
 // Lane-balanced record streams, built once per network from the lists
 // above.  Record k of lane l sits at [k * VK_NT + l], so a wave's loads are
 // contiguous, and lanes carry near-equal record counts however uneven the
-// lists are.  A record adds c * src[item] to the lane's running sum (item
-// -1: no term); a record with dest >= 0 closes the sum into output dest and
-// restarts it at zero.
+// lists are.  A record adds c * src[item] to the lane's running sum (no term
+// when item is VK_REC_NONE_ITEM); a record whose dest is not
+// VK_REC_NONE_DEST closes the sum into output dest and restarts it at zero.
 //   jrec: one output per tile entry of D (dest = tile index), whose terms
 //         are its jac list in order, so each entry is the same sequential
 //         sum as a walk of that list (src = drate);
@@ -80,10 +80,20 @@ __host__ __device__ inline VkAtmOff vk_atm_offsets(int nz, int ni) {
 //         loss or production list (dest = chunk, src = rate, item = row);
 //         lc_ptr / pc_ptr give species i's loss / production chunks, summed
 //         in list order.
+// Records are 8 bytes: the coefficient as a float (network creation checks
+// that every stoichiometric coefficient is exact in float, so widening it
+// back gives the float64 value) and item | dest << 18 in one word.
 struct VkRec {
-  double c;
-  int item, dest;
+  float c;
+  uint32_t w;
 };
+constexpr uint32_t VK_REC_NONE_ITEM = 0x3FFFFu;  // 18 bits
+constexpr uint32_t VK_REC_NONE_DEST = 0x3FFFu;   // 14 bits
+__host__ __device__ constexpr uint32_t vk_rec_item(uint32_t w) { return w & 0x3FFFFu; }
+__host__ __device__ constexpr uint32_t vk_rec_dest(uint32_t w) { return w >> 18; }
+__host__ __device__ constexpr uint32_t vk_rec_word(uint32_t item, uint32_t dest) {
+  return item | (dest << 18);
+}
 constexpr int VK_FCHUNK = 16;
 
 struct VkNetDev {
