@@ -169,7 +169,7 @@ const char* vk_last_error(void) { return g_err; }
 int vk_version(void) { return 100; }
 int vk_max_species(void) { return VK_G * 7; }
 
-// Longest-processing-time assignment of outputs to the VK_NT lanes of a
+// Longest-processing-time assignment of outputs to the VK_CHEM_NT lanes of a
 // workgroup: costliest first, each to the least loaded lane (lowest lane on
 // ties).  Each lane's outputs come back in ascending id order.
 static std::vector<std::vector<int>> lpt_lanes(const std::vector<int>& cost) {
@@ -178,8 +178,8 @@ static std::vector<std::vector<int>> lpt_lanes(const std::vector<int>& cost) {
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
   using Load = std::pair<long, int>;
   std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
-  for (int l = 0; l < VK_NT; ++l) heap.push({0, l});
-  std::vector<std::vector<int>> lanes(VK_NT);
+  for (int l = 0; l < VK_CHEM_NT; ++l) heap.push({0, l});
+  std::vector<std::vector<int>> lanes(VK_CHEM_NT);
   for (int id : order) {
     Load x = heap.top();
     heap.pop();
@@ -195,9 +195,9 @@ static std::vector<std::vector<int>> lpt_lanes(const std::vector<int>& cost) {
 static int interleave(const std::vector<std::vector<VkRec>>& per_lane, std::vector<VkRec>& out) {
   size_t k = 1;
   for (auto& v : per_lane) k = std::max(k, v.size());
-  out.assign(k * VK_NT, VkRec{0.0f, vk_rec_word(VK_REC_NONE_ITEM, VK_REC_NONE_DEST)});
-  for (int l = 0; l < VK_NT; ++l)
-    for (size_t j = 0; j < per_lane[l].size(); ++j) out[j * VK_NT + l] = per_lane[l][j];
+  out.assign(k * VK_CHEM_NT, VkRec{0.0f, vk_rec_word(VK_REC_NONE_ITEM, VK_REC_NONE_DEST)});
+  for (int l = 0; l < VK_CHEM_NT; ++l)
+    for (size_t j = 0; j < per_lane[l].size(); ++j) out[j * VK_CHEM_NT + l] = per_lane[l][j];
   return (int)k;
 }
 
@@ -309,10 +309,10 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
     const size_t len = (i < ni && q < ni) ? JL[(size_t)i * ni + q].size() : 0;
     jcost[idx] = (int)std::max<size_t>(len, 1);
   }
-  std::vector<std::vector<VkRec>> jl(VK_NT);
+  std::vector<std::vector<VkRec>> jl(VK_CHEM_NT);
   {
     const auto lanes = lpt_lanes(jcost);
-    for (int l = 0; l < VK_NT; ++l)
+    for (int l = 0; l < VK_CHEM_NT; ++l)
       for (int idx : lanes[l]) {
         int i, q;
         tile_elem(idx, i, q);
@@ -349,12 +349,12 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
   for (int i = 0; i < ni; ++i) { pcp[i] = (int)chunks.size(); add_chunks(P[i]); }
   pcp[ni] = (int)chunks.size();
   const int nchunk = (int)chunks.size();
-  std::vector<std::vector<VkRec>> fl(VK_NT);
+  std::vector<std::vector<VkRec>> fl(VK_CHEM_NT);
   {
     std::vector<int> fcost(nchunk);
     for (int ch = 0; ch < nchunk; ++ch) fcost[ch] = (int)chunks[ch].size();
     const auto lanes = lpt_lanes(fcost);
-    for (int l = 0; l < VK_NT; ++l)
+    for (int l = 0; l < VK_CHEM_NT; ++l)
       for (int ch : lanes[l]) fl[l].insert(fl[l].end(), chunks[ch].begin(), chunks[ch].end());
   }
   std::vector<VkRec> frec;

@@ -154,25 +154,19 @@ __device__ inline DiffCoef mol_coef(int z, int nz, int ni, int i, const AtmView&
   return d;
 }
 
-__device__ inline double block_sum(double v, double* red) {
-  // sum over the 256 threads; result valid in every thread
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  const int w = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[w] = v;
-  __syncthreads();
-  return (red[0] + red[1]) + (red[2] + red[3]);
-}
 
 }  // namespace
 
 // Dynamic shared memory: rate[S] + drate[S*T] (Jacobian modes) + 3 y rows
 // + the loss / production chunk sums.
-template <int SL>
-__global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
+// NT threads per workgroup: VK_CHEM_NT for the Jacobian modes, whose LDS
+// (rates and their derivatives) allows two workgroups per CU, and half that
+// for the others; each thread walks the record streams of lanes tid,
+// tid + NT, ... of the VK_CHEM_NT-lane tables.
+template <int SL, int NT>
+__global__ __launch_bounds__(NT) void k_chem_layer(LayerArgs L) {
   extern __shared__ double smem[];
   __shared__ double ysum3[3];
-  __shared__ double red[4];
   __shared__ double dd_s[VK_G * SL];  // A+Ai (+ bottom vdep for M_LHS) per species
   __shared__ double bt_s[VK_G * SL];  // bottom vdep term, kept apart for M_JY
   __shared__ int fix_s[VK_G * SL];    // M_LHS: row i of layer z is fixed
@@ -202,7 +196,7 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
   const int64_t kstride = L.k_layer_major ? 1 : nz;
 
   // y rows z-1, z, z+1 with a trailing 1.0 pad (PAD index = ni)
-  for (int e = tid; e < 3 * (ni + 1); e += VK_NT) {
+  for (int e = tid; e < 3 * (ni + 1); e += NT) {
     const int rr = e / (ni + 1), i = e % (ni + 1);
     const int zz = z - 1 + rr;
     double v = 1.0;
@@ -213,7 +207,7 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
   const double* y0 = yrow + (ni + 1);
 
   // reaction rates and rate derivatives (chem.py:92-99, 162-183)
-  for (int r = tid; r < S; r += VK_NT) {
+  for (int r = tid; r < S; r += NT) {
 #pragma clang fp contract(off)
     const double kr = kz[(int64_t)r * kstride];
     double fac[4];
@@ -259,17 +253,19 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
   // loss / production chunk sums from the lane-balanced records
   double* part = yrow + 3 * (ni + 1);
   if (want_rhs) {
-    const VkRec* fr = net.frec + tid;
-    double acc = 0.0;
+    for (int vl = tid; vl < VK_CHEM_NT; vl += NT) {
+      const VkRec* fr = net.frec + vl;
+      double acc = 0.0;
 #pragma unroll 4
-    for (int kk = 0; kk < net.frec_k; ++kk) {
+      for (int kk = 0; kk < net.frec_k; ++kk) {
 #pragma clang fp contract(off)
-      const VkRec rc = fr[(int64_t)kk * VK_NT];
-      const uint32_t item = vk_rec_item(rc.w), dest = vk_rec_dest(rc.w);
-      if (item != VK_REC_NONE_ITEM) acc = acc + (double)rc.c * rate[item];
-      if (dest != VK_REC_NONE_DEST) {
-        part[dest] = acc;
-        acc = 0.0;
+        const VkRec rc = fr[(int64_t)kk * VK_CHEM_NT];
+        const uint32_t item = vk_rec_item(rc.w), dest = vk_rec_dest(rc.w);
+        if (item != VK_REC_NONE_ITEM) acc = acc + (double)rc.c * rate[item];
+        if (dest != VK_REC_NONE_DEST) {
+          part[dest] = acc;
+          acc = 0.0;
+        }
       }
     }
     __syncthreads();
@@ -286,7 +282,7 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
   const double c0 = 1.0 / (GAMMA * dtc);
 
   // per-species outputs: chem rhs (+ diffusion), diagonal/off-diagonal terms
-  for (int i = tid; i < ni; i += VK_NT) {
+  for (int i = tid; i < ni; i += NT) {
 #pragma clang fp contract(off)
     double f = 0.0;
     if (want_rhs) {
@@ -346,30 +342,32 @@ __global__ __launch_bounds__(VK_NT) void k_chem_layer(LayerArgs L) {
     // order), which writes it to its tile slot
     constexpr int BLK = vk_blk<SL>();
     double* dst = L.out_blk + col.off_b + (int64_t)z * BLK;
-    const VkRec* jr = net.jrec + tid;
-    double J = 0.0;
+    for (int vl = tid; vl < VK_CHEM_NT; vl += NT) {
+      const VkRec* jr = net.jrec + vl;
+      double J = 0.0;
 #pragma unroll 4
-    for (int kk = 0; kk < net.jrec_k; ++kk) {
+      for (int kk = 0; kk < net.jrec_k; ++kk) {
 #pragma clang fp contract(off)
-      const VkRec rc = jr[(int64_t)kk * VK_NT];
-      const uint32_t item = vk_rec_item(rc.w), dest = vk_rec_dest(rc.w);
-      if (item != VK_REC_NONE_ITEM) J = J + (double)rc.c * drate[item];
-      if (dest != VK_REC_NONE_DEST) {
-        const int blk = (int)dest / VK_NT, rr = (int)dest % VK_NT;
-        const int i = (blk / SL) * VK_G + rr / VK_G, q = (blk % SL) * VK_G + rr % VK_G;
-        double v = 0.0;
-        if (i < ni && q < ni) {
-          v = (i == q ? c0 : 0.0) - J;
-          if (i == q) v = v + (-dd_s[i]);
-          if (fix_s[i]) v = (i == q) ? c0 : 0.0;
+        const VkRec rc = jr[(int64_t)kk * VK_CHEM_NT];
+        const uint32_t item = vk_rec_item(rc.w), dest = vk_rec_dest(rc.w);
+        if (item != VK_REC_NONE_ITEM) J = J + (double)rc.c * drate[item];
+        if (dest != VK_REC_NONE_DEST) {
+          const int blk = (int)dest / (VK_G * VK_G), rr = (int)dest % (VK_G * VK_G);
+          const int i = (blk / SL) * VK_G + rr / VK_G, q = (blk % SL) * VK_G + rr % VK_G;
+          double v = 0.0;
+          if (i < ni && q < ni) {
+            v = (i == q ? c0 : 0.0) - J;
+            if (i == q) v = v + (-dd_s[i]);
+            if (fix_s[i]) v = (i == q) ? c0 : 0.0;
+          }
+          dst[dest] = v;
+          J = 0.0;
         }
-        dst[dest] = v;
-        J = 0.0;
       }
     }
   } else {
     double* dst = L.out0 + col.off_s * ni + (int64_t)z * ni * ni;
-    for (int e = tid; e < ni * ni; e += VK_NT) {
+    for (int e = tid; e < ni * ni; e += NT) {
 #pragma clang fp contract(off)
       double J = 0.0;
       for (int p = net.jac_ptr[e]; p < net.jac_ptr[e + 1]; ++p)
@@ -429,18 +427,26 @@ int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max
   // The dynamic-LDS limit of each instantiation is raised once (to what the
   // device allows beyond the kernel's static LDS), outside any graph capture
   // that later launches may be part of.
-  static int limit[8] = {0};
+  static int limit[2][8] = {{0}};
+  const int big = want_jac ? 1 : 0;
   switch (sl) {
 #define VK_CASE(n)                                                                        \
-  case n:                                                                                 \
-    if (!limit[n]) limit[n] = raise_lds_limit((const void*)k_chem_layer<n>);             \
-    if ((int)shmem > limit[n]) {                                                          \
+  case n: {                                                                               \
+    const void* fn = big ? (const void*)k_chem_layer<n, VK_CHEM_NT>                       \
+                         : (const void*)k_chem_layer<n, VK_CHEM_NT / 2>;                  \
+    if (!limit[big][n]) limit[big][n] = raise_lds_limit(fn);                              \
+    if ((int)shmem > limit[big][n]) {                                                     \
       vk_set_error("chem layer needs %zu bytes of dynamic LDS, limit %d", shmem,         \
-                   limit[n]);                                                             \
+                   limit[big][n]);                                                        \
       return VK_ERR_UNSUPPORTED;                                                          \
     }                                                                                     \
-    hipLaunchKernelGGL(k_chem_layer<n>, grid, dim3(VK_NT), shmem, stream, L);             \
-    break;
+    if (big)                                                                              \
+      hipLaunchKernelGGL((k_chem_layer<n, VK_CHEM_NT>), grid, dim3(VK_CHEM_NT), shmem,    \
+                         stream, L);                                                      \
+    else                                                                                  \
+      hipLaunchKernelGGL((k_chem_layer<n, VK_CHEM_NT / 2>), grid, dim3(VK_CHEM_NT / 2),  \
+                         shmem, stream, L);                                               \
+  } break;
     VK_CASE(1) VK_CASE(2) VK_CASE(3) VK_CASE(4) VK_CASE(5) VK_CASE(6) VK_CASE(7)
 #undef VK_CASE
     default:

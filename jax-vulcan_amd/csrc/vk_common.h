@@ -67,8 +67,12 @@ __host__ __device__ inline VkAtmOff vk_atm_offsets(int nz, int ni) {
 //   jac        : entry-major lists over (out, in) pairs of
 //                (row*T + in_slot, signed out stoich) in ascending
 //                (row, out slot, in slot) order.
+// Threads per workgroup of the chemistry kernels (one (layer, column) each).
+constexpr int VK_CHEM_NT = 512;
+
 // Lane-balanced record streams, built once per network from the lists
-// above.  Record k of lane l sits at [k * VK_NT + l], so a wave's loads are
+// above, for the VK_CHEM_NT lanes of a chemistry workgroup.  Record k of
+// lane l sits at [k * VK_CHEM_NT + l], so a wave's loads are
 // contiguous, and lanes carry near-equal record counts however uneven the
 // lists are.  A record adds c * src[item] to the lane's running sum (no term
 // when item is VK_REC_NONE_ITEM); a record whose dest is not
