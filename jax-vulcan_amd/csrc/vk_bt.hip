@@ -33,11 +33,12 @@ struct BtShared {
 };
 
 // Lane KC of every 16-lane row, broadcast to the whole row (DPP row_newbcast).
+// Every lane is written, so the DPP move needs no old value.
 template <int KC>
 __device__ __forceinline__ double row_bcast(double v) {
   const int lo = __double2loint(v), hi = __double2hiint(v);
-  return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, 0x150 + KC, 0xF, 0xF, false),
-                          __builtin_amdgcn_update_dpp(0, lo, 0x150 + KC, 0xF, 0xF, false));
+  return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, 0x150 + KC, 0xF, 0xF, false),
+                          __builtin_amdgcn_mov_dpp(lo, 0x150 + KC, 0xF, 0xF, false));
 }
 
 // v = max(v, v moved by a DPP pattern); rows outside ROW_MASK keep v.
