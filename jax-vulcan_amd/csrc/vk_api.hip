@@ -191,10 +191,12 @@ static std::vector<std::vector<int>> lpt_lanes(const std::vector<int>& cost) {
 }
 
 // Lane-interleaved record table from per-lane record lists, padded with
-// no-op records to the longest lane.  Returns the records per lane.
+// no-op records to the longest lane rounded up to whole load groups.
+// Returns the records per lane.
 static int interleave(const std::vector<std::vector<VkRec>>& per_lane, std::vector<VkRec>& out) {
   size_t k = 1;
   for (auto& v : per_lane) k = std::max(k, v.size());
+  k = (k + VK_REC_GROUP - 1) / VK_REC_GROUP * VK_REC_GROUP;  // whole load groups
   out.assign(k * VK_CHEM_NT, VkRec{0.0f, vk_rec_word(VK_REC_NONE_ITEM, VK_REC_NONE_DEST)});
   for (int l = 0; l < VK_CHEM_NT; ++l)
     for (size_t j = 0; j < per_lane[l].size(); ++j) out[j * VK_CHEM_NT + l] = per_lane[l][j];

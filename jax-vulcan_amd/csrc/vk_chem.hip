@@ -154,6 +154,28 @@ __device__ inline DiffCoef mol_coef(int z, int nz, int ni, int i, const AtmView&
   return d;
 }
 
+// Apply f to the records of one lane stream (records rec[k * VK_CHEM_NT],
+// k < count, count a multiple of VK_REC_GROUP), in order.  Records are
+// loaded a group at a time, and the next group is requested before the
+// current one is worked through, so the loads overlap the work instead of
+// waiting behind its branches.
+template <class F>
+__device__ __forceinline__ void walk_records(const VkRec* rec, int count, F&& f) {
+  VkRec cur[VK_REC_GROUP], nxt[VK_REC_GROUP];
+#pragma unroll
+  for (int g = 0; g < VK_REC_GROUP; ++g) cur[g] = rec[(int64_t)g * VK_CHEM_NT];
+  for (int k0 = 0; k0 < count; k0 += VK_REC_GROUP) {
+    if (k0 + VK_REC_GROUP < count) {
+#pragma unroll
+      for (int g = 0; g < VK_REC_GROUP; ++g)
+        nxt[g] = rec[(int64_t)(k0 + VK_REC_GROUP + g) * VK_CHEM_NT];
+    }
+#pragma unroll
+    for (int g = 0; g < VK_REC_GROUP; ++g) f(cur[g]);
+#pragma unroll
+    for (int g = 0; g < VK_REC_GROUP; ++g) cur[g] = nxt[g];
+  }
+}
 
 }  // namespace
 
@@ -254,19 +276,16 @@ __global__ __launch_bounds__(NT) void k_chem_layer(LayerArgs L) {
   double* part = yrow + 3 * (ni + 1);
   if (want_rhs) {
     for (int vl = tid; vl < VK_CHEM_NT; vl += NT) {
-      const VkRec* fr = net.frec + vl;
       double acc = 0.0;
-#pragma unroll 4
-      for (int kk = 0; kk < net.frec_k; ++kk) {
+      walk_records(net.frec + vl, net.frec_k, [&](const VkRec& rc) {
 #pragma clang fp contract(off)
-        const VkRec rc = fr[(int64_t)kk * VK_CHEM_NT];
         const uint32_t item = vk_rec_item(rc.w), dest = vk_rec_dest(rc.w);
         if (item != VK_REC_NONE_ITEM) acc = acc + (double)rc.c * rate[item];
         if (dest != VK_REC_NONE_DEST) {
           part[dest] = acc;
           acc = 0.0;
         }
-      }
+      });
     }
     __syncthreads();
   }
@@ -343,12 +362,9 @@ __global__ __launch_bounds__(NT) void k_chem_layer(LayerArgs L) {
     constexpr int BLK = vk_blk<SL>();
     double* dst = L.out_blk + col.off_b + (int64_t)z * BLK;
     for (int vl = tid; vl < VK_CHEM_NT; vl += NT) {
-      const VkRec* jr = net.jrec + vl;
       double J = 0.0;
-#pragma unroll 4
-      for (int kk = 0; kk < net.jrec_k; ++kk) {
+      walk_records(net.jrec + vl, net.jrec_k, [&](const VkRec& rc) {
 #pragma clang fp contract(off)
-        const VkRec rc = jr[(int64_t)kk * VK_CHEM_NT];
         const uint32_t item = vk_rec_item(rc.w), dest = vk_rec_dest(rc.w);
         if (item != VK_REC_NONE_ITEM) J = J + (double)rc.c * drate[item];
         if (dest != VK_REC_NONE_DEST) {
@@ -363,7 +379,7 @@ __global__ __launch_bounds__(NT) void k_chem_layer(LayerArgs L) {
           dst[dest] = v;
           J = 0.0;
         }
-      }
+      });
     }
   } else {
     double* dst = L.out0 + col.off_s * ni + (int64_t)z * ni * ni;

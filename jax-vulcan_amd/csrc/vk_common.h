@@ -99,6 +99,9 @@ __host__ __device__ constexpr uint32_t vk_rec_word(uint32_t item, uint32_t dest)
   return item | (dest << 18);
 }
 constexpr int VK_FCHUNK = 16;
+// Records per lane are a multiple of VK_REC_GROUP: the kernels load a group
+// of records at once and the next group before working through the current.
+constexpr int VK_REC_GROUP = 8;
 
 struct VkNetDev {
   int ni, S, T, pad;
