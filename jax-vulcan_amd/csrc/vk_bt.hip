@@ -65,15 +65,18 @@ template <int SL>
 struct GjCtx {
   BtShared<SL>* sh;
   int n, kb, bad;
-  unsigned live;  // bit s: row 16s+tr is a row of A not yet used as a pivot
+  // per tile row s: 0 while row 16s+tr is a row of A not yet used as a
+  // pivot, -inf after (and for padding rows), added to |A[i][k]| to rank it
+  double bias[SL];
 };
 
 // One Gauss-Jordan step with pivot column k = 16 kb + KC, which the column
 // rotation keeps in register column 0.  Returns false once k reaches n.
 //
 // The pivot is the first maximum of |A[i][k]| over the unused rows (the
-// getrf sequence).  NaN entries are never chosen; a step left without a
-// candidate flags the matrix.  Per-tile-row branches test bits of a mask
+// getrf sequence).  A row's rank is |A[i][k]| + bias: used rows rank -inf
+// (or NaN, for an infinite entry), and fmax passes over NaN, so NaN entries
+// are never chosen; a step left without a candidate flags the matrix.  Per-tile-row branches test bits of a mask
 // that passes through readfirstlane, so the compiler keeps them as separate
 // uniform branches instead of merging the register-tile stores into one
 // dynamically indexed store (which would move the tile to scratch memory).
@@ -85,15 +88,14 @@ __device__ __forceinline__ bool gj_step(double (&a)[SL][SL], GjCtx<SL>& c) {
   const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int buf = k & 1;
-  // Column k of this thread's rows; pivot rank |A[i][k]|, -1 for rows that
-  // are not candidates.
-  double colv[SL];
-  double best = -1.0;
+  // Column k of this thread's rows and their pivot ranks.
+  double colv[SL], rank[SL];
+  double best = -INFINITY;
 #pragma unroll
   for (int s = 0; s < SL; ++s) {
     colv[s] = row_bcast<KC>(a[s][0]);
-    const double r = ((c.live >> s) & 1u) ? fabs(colv[s]) : -1.0;
-    best = r > best ? r : best;
+    rank[s] = fabs(colv[s]) + c.bias[s];
+    best = fmax(best, rank[s]);
   }
   const double m = rows_max(best);
   // First row of the wave attaining m: smallest tile row s, then smallest tr.
@@ -101,7 +103,7 @@ __device__ __forceinline__ bool gj_step(double (&a)[SL][SL], GjCtx<SL>& c) {
   if (m >= 0.0) {
 #pragma unroll
     for (int s = SL - 1; s >= 0; --s) {
-      const unsigned long long bal = __ballot(((c.live >> s) & 1u) && fabs(colv[s]) == m);
+      const unsigned long long bal = __ballot(rank[s] == m);
       if (bal) brow = s * VK_G + wave * 4 + (int)(__builtin_ctzll(bal) >> 4);
     }
   }
@@ -167,9 +169,9 @@ __device__ __forceinline__ bool gj_step(double (&a)[SL][SL], GjCtx<SL>& c) {
       if (psm & (1u << s)) {
 #pragma unroll
         for (int t = 0; t < SL; ++t) a[s][t] = pr[t];
+        c.bias[s] = -INFINITY;
       }
     }
-    c.live &= ~psm;
   }
   return true;
 }
@@ -193,11 +195,9 @@ template <int SL>
 __device__ int gj_invert(double (&a)[SL][SL], int n, BtShared<SL>& sh) {
   const int tid = threadIdx.x;
   const int tr = tid >> 4, tc = tid & 15;
-  unsigned live = 0;
+  GjCtx<SL> c{&sh, n, 0, 0, {}};
 #pragma unroll
-  for (int s = 0; s < SL; ++s)
-    if (s * VK_G + tr < n) live |= 1u << s;
-  GjCtx<SL> c{&sh, n, 0, 0, live};
+  for (int s = 0; s < SL; ++s) c.bias[s] = (s * VK_G + tr < n) ? 0.0 : -INFINITY;
 #pragma unroll 1
   for (int kb = 0; kb < SL; ++kb) {
     c.kb = kb;
