@@ -229,6 +229,43 @@ __global__ __launch_bounds__(NT) void k_chem_layer(LayerArgs L) {
   const double* y0 = yrow + (ni + 1);
 
   // reaction rates and rate derivatives (chem.py:92-99, 162-183)
+  if (T == 2) {
+    // Two rows per iteration with every table load issued first; the
+    // arithmetic is the general loop's below, term for term.
+    auto row2 = [&](int r, double kr, int sp0, int sp1, double st0, double st1, bool tb) {
+#pragma clang fp contract(off)
+      const double f0 = (st0 > 0.0) ? vk_ipow(y0[sp0], st0) : 1.0;
+      const double f1 = (st1 > 0.0) ? vk_ipow(y0[sp1], st1) : 1.0;
+      double rt = kr * ((1.0 * f0) * f1);
+      if (tb) rt = rt * Mz;
+      rate[r] = rt;
+      if (want_jac) {
+        const double pm0 = (st0 > 0.0) ? vk_ipow(y0[sp0], st0 - 1.0) : 0.0;
+        const double pm1 = (st1 > 0.0) ? vk_ipow(y0[sp1], st1 - 1.0) : 0.0;
+        double d0 = st0 * pm0 * (1.0 * f1) * kr;
+        double d1 = st1 * pm1 * (1.0 * f0) * kr;
+        if (tb) {
+          d0 = d0 * Mz;
+          d1 = d1 * Mz;
+        }
+        drate[2 * r] = d0;
+        drate[2 * r + 1] = d1;
+      }
+    };
+    for (int r0 = tid; r0 < S; r0 += 2 * NT) {
+      const int r1 = r0 + NT;
+      const bool has1 = r1 < S;
+      const int rb = has1 ? r1 : r0;
+      const double ka = kz[(int64_t)r0 * kstride], kb = kz[(int64_t)rb * kstride];
+      const int2 ia = *reinterpret_cast<const int2*>(net.r_idx + 2 * r0);
+      const int2 ib = *reinterpret_cast<const int2*>(net.r_idx + 2 * rb);
+      const double2 sa = *reinterpret_cast<const double2*>(net.r_st + 2 * r0);
+      const double2 sb = *reinterpret_cast<const double2*>(net.r_st + 2 * rb);
+      const int ta = net.is3[r0], tbb = net.is3[rb];
+      row2(r0, ka, ia.x, ia.y, sa.x, sa.y, ta != 0);
+      if (has1) row2(r1, kb, ib.x, ib.y, sb.x, sb.y, tbb != 0);
+    }
+  } else
   for (int r = tid; r < S; r += NT) {
 #pragma clang fp contract(off)
     const double kr = kz[(int64_t)r * kstride];
