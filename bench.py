@@ -163,7 +163,7 @@ def main() -> int:
     n_singular = int((status != 0).sum().item())
     print(f"[bench] batch {B}: {elapsed / args.steps * 1e3:.2f} ms/step, "
           f"{value:.1f} column-steps/s; kernels ms {['%.3f' % v for v in kern_ms[:5]]} "
-          f"transpose {kern_ms[6]:.3f}",
+          f"prep {kern_ms[6]:.3f}",
           file=sys.stderr, flush=True)
 
     # ---- plain single-column kernel ------------------------------------------
@@ -303,7 +303,7 @@ def main() -> int:
         "kernel_ms_per_launch": {"chem_lhs": round(kern_ms[0], 4), "bt_factor": round(kern_ms[1], 4),
                                  "bt_back1": round(kern_ms[2], 4), "rhs2": round(kern_ms[3], 4),
                                  "bt_solve2": round(kern_ms[4], 4),
-                                 "rates_transpose": round(kern_ms[6], 4)},
+                                 "step_prep": round(kern_ms[6], 4)},
         "step_tflops": round(B * step_flops(NZ, NI, S) / (elapsed / args.steps) / 1e12, 3),
         "finite": finite,
         "singular_columns": n_singular,

@@ -243,8 +243,9 @@ int vk_photo_f64(vk_batch b, int parts, const vk_photo_setup* setup, const vk_ph
 /* Kernel timing with HIP events recorded on the launch stream around each
  * kernel of vk_ros2_step_f64 (no counterpart in the reference, which has no
  * device timing).  Kernel ids: 0 chemistry+LHS, 1 block factor, 2 stage-1
- * back sweep, 3 stage-2 rhs, 4 stage-2 solve, 5 step control, 6 rate-table
- * transpose.  vk_timing_read synchronises
+ * back sweep, 3 stage-2 rhs, 4 stage-2 solve, 5 step control, 6 per-step
+ * preparation (rate-table transpose and the molecular-diffusion factors that
+ * do not depend on y).  vk_timing_read synchronises
  * on the recorded events and returns accumulated milliseconds and launch
  * counts per id.  Not thread-safe; for benchmarks. */
 int vk_timing_enable(int on);

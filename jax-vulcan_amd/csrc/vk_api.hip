@@ -87,7 +87,10 @@ int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max
                          int max_T, const VkNetDev* nets, const double* y, const double* M,
                          const double* atm, const double* k, const double* dt, const uint8_t* fix,
                          const double* k1, double* out0, double* out_blk, double* out_sup,
-                         double* out_sub, const int* active, int k_layer_major, hipStream_t stream);
+                         double* out_sub, const int* active, int k_layer_major,
+                         const double* mstat, int64_t mstride, hipStream_t stream);
+int vk_launch_mol_static(const VkCol* cols, int ncols, int max_nz, const double* atm, double* ms,
+                         int64_t stride, const int* active, hipStream_t stream);
 int vk_launch_rates_layer_major(const VkCol* cols, int ncols, int max_S, int max_nz, const double* k,
                                 double* kT, const int* active, hipStream_t stream);
 
@@ -636,7 +639,7 @@ static int chem_call(vk_batch b, int mode, const double* y, const double* M, con
   for (int sl : b->slots) {
     int rc = vk_launch_chem_layer(sl, mode, b->d_cols, b->ncols, b->max_nz, b->max_S, b->max_T,
                                   b->d_nets, y, M, atm, k, dt, fix, k1, out0, out_blk, sup, sub,
-                                  active, 0, st);
+                                  active, 0, nullptr, 0, st);
     if (rc) return rc;
   }
   return VK_OK;
@@ -759,7 +762,7 @@ int vk_timing_read(double* ms, long long* count, int n) {
 
 size_t vk_ros2_workspace_bytes(vk_batch b) {
   if (!b) return 0;
-  return align_up<double>(b->block_doubles) + 8 * align_up<double>(b->state_doubles) +
+  return align_up<double>(b->block_doubles) + 16 * align_up<double>(b->state_doubles) +
          align_up<double>(b->k_doubles);
 }
 
@@ -802,11 +805,17 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
   double* sup = (double*)w; w += sv;
   double* sub = (double*)w; w += sv;
   double* kT = (double*)w;  // the rate tables layer-major, read contiguously by the chemistry
+  w += align_up<double>(b->k_doubles);
+  double* mst = (double*)w;  // 8 components of the molecular-diffusion static factors
+  const int64_t mstride = (int64_t)(sv / sizeof(double));
   const double r = ros2_gamma();
   {
     KernelTimer t(6, st);
     if (int rc = vk_launch_rates_layer_major(b->d_cols, b->ncols, b->max_S, b->max_nz, k, kT,
                                              active, st))
+      return rc;
+    if (int rc = vk_launch_mol_static(b->d_cols, b->ncols, b->max_nz, atm, mst, mstride, active,
+                                      st))
       return rc;
   }
   // The kernel sequence of one SL bucket on stream s.
@@ -816,7 +825,7 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
       KernelTimer t(0, s);
       if ((rc = vk_launch_chem_layer(sl, M_LHS, b->d_cols, b->ncols, b->max_nz, b->max_S,
                                      b->max_T, b->d_nets, y, nullptr, atm, kT, dt, fix_mask,
-                                     nullptr, rhs_y, blk, sup, sub, active, 1, s)))
+                                     nullptr, rhs_y, blk, sup, sub, active, 1, mst, mstride, s)))
         return rc;
     }
     // The factor fuses the stage-1 forward sweep unless the blocked variant
@@ -840,7 +849,7 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
       KernelTimer t(3, s);
       if ((rc = vk_launch_chem_layer(sl, M_RHS2, b->d_cols, b->ncols, b->max_nz, b->max_S,
                                      b->max_T, b->d_nets, yk2, nullptr, atm, kT, dt, fix_mask, k1,
-                                     rhs2, nullptr, nullptr, nullptr, active, 1, s)))
+                                     rhs2, nullptr, nullptr, nullptr, active, 1, mst, mstride, s)))
         return rc;
     }
     SolveEpi e2{2, 3.0 / (2.0 * r), 1.0 / (2.0 * r), y, k1, yk2, sol, delta};

@@ -38,6 +38,8 @@ struct LayerArgs {
   const int* active;
   int mode;
   int k_layer_major;     // k holds (nz, S) per column (a transposed copy)
+  const double* mstat;   // optional molecular-diffusion static factors (k_mol_static)
+  int64_t mstride;       // component stride of mstat
 };
 
 constexpr double GAMMA = 1.7071067811865475;  // 1 + 1/sqrt(2)
@@ -150,6 +152,84 @@ __device__ inline DiffCoef mol_coef(int z, int nz, int ni, int i, const AtmView&
           Ax;
     d.B = 1.0 / dza * Dj / dzi[z] * (ysp + ys0) / 2.0 / ysp + Bx;
     d.C = 1.0 / dza * Djm / dzi[z - 1] * (ys0 + ysm) / 2.0 / ysm + Cx;
+  }
+  return d;
+}
+
+// The y-independent factors of mol_coef for species i at layer z, each the
+// value mol_coef forms on its way (same operations, same order), so that
+// mol_coef_pre below reproduces mol_coef bit for bit:
+//   m[0]  A prefix: -1/dzi0 * (D0/dzi0) (z = 0), -1/dzi_t * (Dt/dzi_t)
+//         (z = nz-1), -1/dza (inside)
+//   m[1], m[2]  inside: Dj/dzi[z], Djm/dzi[z-1]
+//   m[3]  B prefix: 1/dzi0 * (D0/dzi0) (z = 0), (1/dza * Dj)/dzi[z] (inside)
+//   m[4]  C prefix: 1/dzi_t * (Dt/dzi_t) (z = nz-1), (1/dza * Djm)/dzi[z-1]
+//   m[5], m[6], m[7]  the additive A, B, C terms (gravity, vm, settling)
+__device__ inline void mol_static(int z, int nz, int ni, int i, const AtmView& v, double (&m)[8]) {
+#pragma clang fp contract(off)
+  const double use_vm = v.flags[0] != 0.0 ? 1.0 : 0.0;
+  const double use_set = v.flags[1] != 0.0 ? 1.0 : 0.0;
+  const double* dzi = v.dzi;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) m[c] = 0.0;
+  if (z == 0) {
+    const double D0 = v.Dzz[i];
+    const double g0 = 1.0 / dzi[0] * D0 / 2.0 * grav_fac(v, i, 0, 0);
+    const double vm0 = v.vm[i], vs0 = v.vs[i];
+    m[5] = (1.0 - use_vm) * g0 + use_vm * (-posp(vm0) / dzi[0]) + use_set * (-posp(vs0) / dzi[0]);
+    m[6] = (1.0 - use_vm) * g0 + use_vm * (-negp(vm0) / dzi[0]) + use_set * (-negp(vs0) / dzi[0]);
+    m[0] = -1.0 / dzi[0] * (D0 / dzi[0]);
+    m[3] = 1.0 / dzi[0] * (D0 / dzi[0]);
+  } else if (z == nz - 1) {
+    const int t = nz - 2;
+    const double Dt = v.Dzz[(int64_t)t * ni + i];
+    const double gt = -1.0 / dzi[t] * Dt / 2.0 *
+                      (-1.0 / v.Hpi[t] + v.ms[i] * v.g[nz - 1] / (6.02214086e23 * 1.38064852e-16 * v.Ti[t]) +
+                       v.alpha[i] / v.Ti[t] * (v.Tco[nz - 1] - v.Tco[nz - 2]) / dzi[t]);
+    const double vmt = v.vm[(int64_t)(nz - 1) * ni + i], vst = v.vs[(int64_t)t * ni + i];
+    m[5] = (1.0 - use_vm) * gt + use_vm * (negp(vmt) / dzi[t]) + use_set * (negp(vst) / dzi[t]);
+    m[7] = (1.0 - use_vm) * gt + use_vm * (posp(vmt) / dzi[t]) + use_set * (posp(vst) / dzi[t]);
+    m[0] = -1.0 / dzi[t] * (Dt / dzi[t]);
+    m[4] = 1.0 / dzi[t] * (Dt / dzi[t]);
+  } else {
+    const double dza = 0.5 * (dzi[z - 1] + dzi[z]);
+    const double Dj = v.Dzz[(int64_t)z * ni + i], Djm = v.Dzz[(int64_t)(z - 1) * ni + i];
+    const double inv2 = 1.0 / (2.0 * dza);
+    const double gA = inv2 * (Dj * grav_fac(v, i, z, z) - Djm * grav_fac(v, i, z - 1, z));
+    const double gB = inv2 * Dj * grav_fac(v, i, z, z + 1);
+    const double gC = -inv2 * Djm * grav_fac(v, i, z - 1, z - 1);
+    const double vmj = v.vm[(int64_t)z * ni + i], vmm = v.vm[(int64_t)(z - 1) * ni + i];
+    const double vsj = v.vs[(int64_t)z * ni + i], vsm = v.vs[(int64_t)(z - 1) * ni + i];
+    const double Avm = -(posp(vmj) - negp(vmm)) / dza, Bvm = -negp(vmj) / dza, Cvm = posp(vmm) / dza;
+    const double Avs = -(posp(vsj) - negp(vsm)) / dza, Bvs = -negp(vsj) / dza, Cvs = posp(vsm) / dza;
+    m[5] = ((1.0 - use_vm) * gA + use_vm * Avm) + use_set * Avs;
+    m[6] = ((1.0 - use_vm) * gB + use_vm * Bvm) + use_set * Bvs;
+    m[7] = ((1.0 - use_vm) * gC + use_vm * Cvm) + use_set * Cvs;
+    m[0] = -1.0 / dza;
+    m[1] = Dj / dzi[z];
+    m[2] = Djm / dzi[z - 1];
+    m[3] = 1.0 / dza * Dj / dzi[z];
+    m[4] = 1.0 / dza * Djm / dzi[z - 1];
+  }
+}
+
+// mol_coef from the factors of mol_static (component c at m[c * st]).
+__device__ inline DiffCoef mol_coef_pre(int z, int nz, const double* m, int64_t st, double ysm,
+                                        double ys0, double ysp) {
+#pragma clang fp contract(off)
+  DiffCoef d;
+  if (z == 0) {
+    d.A = m[0] * (ysp + ys0) / 2.0 / ys0 + m[5 * st];
+    d.B = m[3 * st] * (ysp + ys0) / 2.0 / ysp + m[6 * st];
+    d.C = 0.0;
+  } else if (z == nz - 1) {
+    d.A = m[0] * (ys0 + ysm) / 2.0 / ys0 + m[5 * st];
+    d.C = m[4 * st] * (ys0 + ysm) / 2.0 / ysm + m[7 * st];
+    d.B = 0.0;
+  } else {
+    d.A = m[0] * (m[st] * (ysp + ys0) / 2.0 + m[2 * st] * (ys0 + ysm) / 2.0) / ys0 + m[5 * st];
+    d.B = m[3 * st] * (ysp + ys0) / 2.0 / ysp + m[6 * st];
+    d.C = m[4 * st] * (ys0 + ysm) / 2.0 / ysm + m[7 * st];
   }
   return d;
 }
@@ -349,7 +429,10 @@ __global__ __launch_bounds__(NT) void k_chem_layer(LayerArgs L) {
     }
     const int64_t e_out = col.off_s + (int64_t)z * ni + i;
     if (want_diff) {
-      const DiffCoef md = mol_coef(z, nz, ni, i, av, ysm, ys0, ysp);
+      const DiffCoef md =
+          L.mstat ? mol_coef_pre(z, nz, L.mstat + col.off_s + (int64_t)z * ni + i, L.mstride, ysm,
+                                 ys0, ysp)
+                  : mol_coef(z, nz, ni, i, av, ysm, ys0, ysp);
       const double At = ed.A + md.A, Bt = ed.B + md.B, Ct = ed.C + md.C;
       const double yi = y0[i];
       double diff;
@@ -461,9 +544,9 @@ int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max
                          const double* atm, const double* k, const double* dt, const uint8_t* fix,
                          const double* k1, double* out0, double* out_blk, double* out_sup,
                          double* out_sub, const int* active, int k_layer_major,
-                         hipStream_t stream) {
+                         const double* mstat, int64_t mstride, hipStream_t stream) {
   LayerArgs L{cols, ncols, nets, y, M, atm, k, dt, fix, k1, out0, out_blk, out_sup, out_sub,
-              active, mode, k_layer_major};
+              active, mode, k_layer_major, mstat, mstride};
   const bool want_jac = (mode == M_CHEM_JAC || mode == M_LHS || mode == M_JY);
   // ni+1 <= 16*sl+1 for the y rows; loss / production chunks: at most
   // 2*S*T list entries in at most 2*ni lists
@@ -544,5 +627,35 @@ int vk_launch_rates_layer_major(const VkCol* cols, int ncols, int max_S, int max
                        c0);
     VK_CHECK_HIP(hipGetLastError());
   }
+  return VK_OK;
+}
+
+// ms[c * stride + off_s + z*ni + i] = component c of mol_static for species
+// i at layer z of column c (grid: layers x columns, one thread per species).
+__global__ __launch_bounds__(128) void k_mol_static(const VkCol* __restrict__ cols, int ncols,
+                                                    const double* __restrict__ atm,
+                                                    double* __restrict__ ms, int64_t stride,
+                                                    const int* __restrict__ active) {
+  const int c = blockIdx.y, z = blockIdx.x;
+  if (c >= ncols) return;
+  if (active && (active[c] & VK_ST_DONE)) return;
+  const VkCol col = cols[c];
+  if (z >= col.nz) return;
+  const AtmView av = atm_view(atm + col.off_a, col.nz, col.ni);
+  for (int i = threadIdx.x; i < col.ni; i += blockDim.x) {
+    double m[8];
+    mol_static(z, col.nz, col.ni, i, av, m);
+    double* o = ms + col.off_s + (int64_t)z * col.ni + i;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) o[q * stride] = m[q];
+  }
+}
+
+int vk_launch_mol_static(const VkCol* cols, int ncols, int max_nz, const double* atm, double* ms,
+                         int64_t stride, const int* active, hipStream_t stream) {
+  if (ncols <= 0 || max_nz <= 0) return VK_OK;
+  hipLaunchKernelGGL(k_mol_static, dim3(max_nz, ncols), dim3(128), 0, stream, cols, ncols, atm,
+                     ms, stride, active);
+  VK_CHECK_HIP(hipGetLastError());
   return VK_OK;
 }

@@ -120,7 +120,7 @@ class SlotPool:
     def bytes_per_slot(cap_nz: int, cap_ni: int, cap_S: int) -> int:
         cs, ck = cap_nz * cap_ni, cap_S * cap_nz
         ca = atm_record_layout(cap_nz, cap_ni)[0]
-        ws = cap_nz * block_doubles(cap_ni) + 8 * cs + ck  # Ros2 workspace per slot
+        ws = cap_nz * block_doubles(cap_ni) + 16 * cs + ck  # Ros2 workspace per slot
         return 8 * (6 * cs + ck + ca + ws) + 8 * (3 * VK_MAX_ATOMS + 4) + 4 * 8
 
     # ------------------------------------------------------------------
