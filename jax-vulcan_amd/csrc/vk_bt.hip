@@ -88,6 +88,10 @@ __device__ __forceinline__ bool gj_step(double (&a)[SL][SL], GjCtx<SL>& c) {
   const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int buf = k & 1;
+  // The search and publication run at raised wave priority: the SIMDs are
+  // shared with other columns' waves, and a column's barrier waits for its
+  // slowest wave.
+  __builtin_amdgcn_s_setprio(2);
   // Column k of this thread's rows and their pivot ranks.
   double colv[SL], rank[SL];
   double best = -INFINITY;
@@ -122,6 +126,7 @@ __device__ __forceinline__ bool gj_step(double (&a)[SL][SL], GjCtx<SL>& c) {
     }
   }
   if (lane == 0) { sh.cand_val[buf][wave] = m; sh.cand_idx[buf][wave] = brow; }
+  __builtin_amdgcn_s_setprio(0);
   __syncthreads();
   // Global pivot: best of the wave candidates.
   double pval = sh.cand_val[buf][0];
