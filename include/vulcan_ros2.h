@@ -164,11 +164,14 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
  * 64-192 lanes per column, SL <= 6; bitwise equal to TILE) or
  * VK_FACTOR_BLOCKED (16-column panels eliminated in LDS, the rest updated
  * by rank-16 FP64 MFMA products; same pivots, rounding differs in those
- * products).  Process-wide; set it between launches. */
+ * products) or VK_FACTOR_TILE2 (TILE's algorithm with two waves per column,
+ * two rows per lane per tile row; bitwise equal to TILE).  Process-wide;
+ * set it between launches. */
 #define VK_TUNE_FACTOR 0
 #define VK_FACTOR_TILE 0
 #define VK_FACTOR_ROWS 1
 #define VK_FACTOR_BLOCKED 2
+#define VK_FACTOR_TILE2 3
 int vk_tune_set(int knob, int sl, int value);
 int vk_tune_get(int knob, int sl); /* -1 for an unknown knob / bucket */
 

@@ -32,35 +32,6 @@ struct BtShared {
   int iperm[VK_G * SL];
 };
 
-// Lane KC of every 16-lane row, broadcast to the whole row (DPP row_newbcast).
-// Every lane is written, so the DPP move needs no old value.
-template <int KC>
-__device__ __forceinline__ double row_bcast(double v) {
-  const int lo = __double2loint(v), hi = __double2hiint(v);
-  return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, 0x150 + KC, 0xF, 0xF, false),
-                          __builtin_amdgcn_mov_dpp(lo, 0x150 + KC, 0xF, 0xF, false));
-}
-
-// v = max(v, v moved by a DPP pattern); rows outside ROW_MASK keep v.
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ double dpp_fmax(double v) {
-  const int lo = __double2loint(v), hi = __double2hiint(v);
-  const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xF, false);
-  const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xF, false);
-  return fmax(v, __hiloint2double(hi2, lo2));
-}
-
-// Max over a wave whose 16-lane rows each hold one value in every lane (no
-// NaN): row_bcast 15 and 31 carry rows 0-2 into row 3, whose lane 63 is read
-// back as a scalar.
-__device__ __forceinline__ double rows_max(double v) {
-  v = dpp_fmax<0x142, 0xA>(v);
-  v = dpp_fmax<0x143, 0xC>(v);
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
-  return __hiloint2double(hi, lo);
-}
-
 template <int SL>
 struct GjCtx {
   BtShared<SL>* sh;
@@ -520,13 +491,18 @@ int vk_launch_bt_factor_rows(int sl, const VkCol* cols, int ncols, const double*
                              const double* sub_d, double* inv, const double* rhs, double* rmod,
                              int* status, const int* active, hipStream_t stream);
 
+int vk_launch_bt_factor2(int sl, const VkCol* cols, int ncols, const double* diag, int diag_tile,
+                         int transpose, const double* sup_d, const double* sub_d, double* inv,
+                         const double* rhs, double* rmod, int* status, const int* active,
+                         hipStream_t stream);
 int vk_launch_bt_factor_blk(int sl, const VkCol* cols, int ncols, const double* diag,
                             int diag_tile, int transpose, const double* sup_d, const double* sub_d,
                             double* inv, int* status, const int* active, hipStream_t stream);
 
 // Factor variant per SL bucket: VK_FACTOR_TILE (256 threads, 16x16 register
-// tiles), VK_FACTOR_ROWS (half a row per lane) or VK_FACTOR_BLOCKED (16-column
-// panels, MFMA trailing updates; no fused forward sweep).  Set by vk_tune_set.
+// tiles), VK_FACTOR_ROWS (half a row per lane), VK_FACTOR_BLOCKED (16-column
+// panels, MFMA trailing updates; no fused forward sweep) or VK_FACTOR_TILE2
+// (the tile kernel on 128 threads, vk_bt_tile2.hip).  Set by vk_tune_set.
 int g_factor_variant[8] = {VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE,
                            VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE};
 
@@ -538,6 +514,9 @@ int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag
                         int transpose, const double* sup_d, const double* sub_d, double* inv,
                         const double* rhs, double* rmod, int* status, const int* active,
                         hipStream_t stream) {
+  if (sl >= 1 && sl <= 7 && g_factor_variant[sl] == VK_FACTOR_TILE2)
+    return vk_launch_bt_factor2(sl, cols, ncols, diag, diag_tile, transpose, sup_d, sub_d, inv,
+                                rhs, rmod, status, active, stream);
   if (sl >= 1 && sl <= 6 && g_factor_variant[sl] == VK_FACTOR_ROWS)
     return vk_launch_bt_factor_rows(sl, cols, ncols, diag, diag_tile, transpose, sup_d, sub_d,
                                     inv, rhs, rmod, status, active, stream);

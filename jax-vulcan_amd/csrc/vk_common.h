@@ -193,6 +193,35 @@ __device__ inline double vk_rowsum16(double v) {
   return v;
 }
 
+// Lane KC of every 16-lane row, broadcast to the whole row (DPP row_newbcast).
+// Every lane is written, so the DPP move needs no old value.
+template <int KC>
+__device__ __forceinline__ double row_bcast(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, 0x150 + KC, 0xF, 0xF, false),
+                          __builtin_amdgcn_mov_dpp(lo, 0x150 + KC, 0xF, 0xF, false));
+}
+
+// v = max(v, v moved by a DPP pattern); rows outside ROW_MASK keep v.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_fmax(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xF, false);
+  const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xF, false);
+  return fmax(v, __hiloint2double(hi2, lo2));
+}
+
+// Max over a wave whose 16-lane rows each hold one value in every lane (no
+// NaN): row_bcast 15 and 31 carry rows 0-2 into row 3, whose lane 63 is read
+// back as a scalar.
+__device__ __forceinline__ double rows_max(double v) {
+  v = dpp_fmax<0x142, 0xA>(v);
+  v = dpp_fmax<0x143, 0xC>(v);
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+
 // Integer power for small non-negative integer exponents held as double.
 __device__ inline double vk_ipow(double y, double nu) {
   int n = (int)nu;

@@ -246,10 +246,10 @@ def test_integrate_fixed_dt_matches_oracle():
 
 @pytest.mark.parametrize("ni", [5, 21, 40, 50, 64, 69, 93])
 def test_factor_variants_agree(ni):
-    """The row-per-lane and tile block factors take the same pivots and do the
-    same per-element arithmetic: bitwise-equal inverse stacks (tile-layout and
-    natural inputs, plain and transposed); the fused sweep differs only in
-    summation order."""
+    """The row-per-lane, tile and two-wave tile block factors take the same
+    pivots and do the same per-element arithmetic: bitwise-equal inverse
+    stacks (tile-layout and natural inputs, plain and transposed); the fused
+    sweep differs only in summation order."""
     import torch
     from vulcan_amd import _lib, solver
     lib = _lib.load()
@@ -263,7 +263,7 @@ def test_factor_variants_agree(ni):
     sl = (ni + 15) // 16
     out = {}
     try:
-        for variant in (0, 1):
+        for variant in (0, 1, 3):
             _lib.check(lib.vk_tune_set(0, sl, variant), "vk_tune_set")
             for tr in (False, True):
                 f = solver.factor_block_thomas_diag_offdiag(diag, sup, sub, transpose=tr)
@@ -273,11 +273,13 @@ def test_factor_variants_agree(ni):
         lib.vk_tune_set(0, sl, 0)
     for tr in (False, True):
         inv0, x0, s0 = out[(0, tr)]
-        inv1, x1, s1 = out[(1, tr)]
-        print(f"ni={ni} transpose={tr}: max |inv diff| {float((inv0 - inv1).abs().max()):.3e}")
-        assert torch.equal(inv0, inv1)
-        assert torch.equal(s0, s1)
-        assert float((x0 - x1).abs().max() / x0.abs().max()) < 1e-12
+        for v in (1, 3):
+            inv1, x1, s1 = out[(v, tr)]
+            print(f"ni={ni} transpose={tr} variant {v}: max |inv diff| "
+                  f"{float((inv0 - inv1).abs().max()):.3e}")
+            assert torch.equal(inv0, inv1)
+            assert torch.equal(s0, s1)
+            assert float((x0 - x1).abs().max() / x0.abs().max()) < 1e-12
     dense = [_dense(diag[b], sup[b], sub[b]) for b in range(2)]
     x1 = _np(out[(1, False)][1])
     for b in range(2):
@@ -288,20 +290,47 @@ def test_factor_variants_agree(ni):
 
 
 def test_ros2_step_factor_variants_agree(cfg1):
-    """Whole Ros2 step with either block factor, well-conditioned dt: 1e-12."""
+    """Whole Ros2 step with the tile, row-per-lane or two-wave tile block
+    factor, well-conditioned dt: 1e-12 (the fused sweeps' sums differ only
+    in order)."""
     from vulcan_amd import _lib, jax_step
     lib = _lib.load()
     c = cfg1
     sl = (c.ni + 15) // 16
     sols = []
     try:
-        for variant in (0, 1):
+        for variant in (0, 1, 3):
             _lib.check(lib.vk_tune_set(0, sl, variant), "vk_tune_set")
             sol, _ = jax_step.jax_ros2_step(c.y, c.k, 1e-18, c.atm, c.net)
             sols.append(_np(sol))
     finally:
         lib.vk_tune_set(0, sl, 0)
     assert _layer_err(sols[1], sols[0]).max() < 1e-12
+    assert _layer_err(sols[2], sols[0]).max() < 1e-12
+
+
+def test_tile2_factor_matches_tile_sl7():
+    """The two-wave tile factor at SL = 7 (ni = 100, beyond the row-per-lane
+    kernel): bitwise-equal inverse stacks and status to the tile factor."""
+    import torch
+    from vulcan_amd import _lib, solver
+    lib = _lib.load()
+    rng = np.random.default_rng(7100)
+    ni, nz = 100, 8
+    diag = rng.standard_normal((2, nz, ni, ni)) + 4.0 * np.eye(ni)
+    diag[0, 4] = rng.standard_normal((ni, ni))  # needs pivoting
+    sup = rng.standard_normal((2, nz - 1, ni))
+    sub = rng.standard_normal((2, nz - 1, ni))
+    out = {}
+    try:
+        for variant in (0, 3):
+            _lib.check(lib.vk_tune_set(0, 7, variant), "vk_tune_set")
+            f = solver.factor_block_thomas_diag_offdiag(diag, sup, sub)
+            out[variant] = (f.diag_inv.clone(), f.status.clone())
+    finally:
+        lib.vk_tune_set(0, 7, 0)
+    assert torch.equal(out[0][0], out[3][0])
+    assert torch.equal(out[0][1], out[3][1])
 
 
 @pytest.mark.parametrize("ni", [5, 21, 40, 50, 64, 69, 93, 100])
