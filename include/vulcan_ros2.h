@@ -172,6 +172,10 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
 #define VK_FACTOR_ROWS 1
 #define VK_FACTOR_BLOCKED 2
 #define VK_FACTOR_TILE2 3
+/* Knob VK_TUNE_MOLPRE (sl ignored): 1 (default) has vk_ros2_step_f64 compute
+ * the molecular-diffusion factors that do not depend on y once per step;
+ * 0 forms them inside each chemistry launch.  Results are bitwise equal. */
+#define VK_TUNE_MOLPRE 1
 int vk_tune_set(int knob, int sl, int value);
 int vk_tune_get(int knob, int sl); /* -1 for an unknown knob / bucket */
 

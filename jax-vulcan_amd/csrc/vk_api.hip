@@ -760,6 +760,11 @@ int vk_timing_read(double* ms, long long* count, int n) {
   return VK_OK;
 }
 
+// VK_TUNE_MOLPRE: 1 (default) computes the y-independent molecular-diffusion
+// factors once per step (k_mol_static); 0 forms them inside each chemistry
+// launch.  The coefficients are the same bit for bit either way.
+static int g_mol_precompute = 1;
+
 size_t vk_ros2_workspace_bytes(vk_batch b) {
   if (!b) return 0;
   return align_up<double>(b->block_doubles) + 16 * align_up<double>(b->state_doubles) +
@@ -814,9 +819,13 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
     if (int rc = vk_launch_rates_layer_major(b->d_cols, b->ncols, b->max_S, b->max_nz, k, kT,
                                              active, st))
       return rc;
-    if (int rc = vk_launch_mol_static(b->d_cols, b->ncols, b->max_nz, atm, mst, mstride, active,
-                                      st))
-      return rc;
+    if (g_mol_precompute) {
+      if (int rc = vk_launch_mol_static(b->d_cols, b->ncols, b->max_nz, atm, mst, mstride,
+                                        active, st))
+        return rc;
+    } else {
+      mst = nullptr;
+    }
   }
   // The kernel sequence of one SL bucket on stream s.
   auto run_bucket = [&](int sl, hipStream_t s) -> int {
@@ -997,6 +1006,14 @@ int vk_step_error_f64(vk_batch b, const double* sol, const double* delta, double
 
 int vk_tune_set(int knob, int sl, int value) {
   g_err[0] = 0;
+  if (knob == VK_TUNE_MOLPRE) {
+    if (value != 0 && value != 1) {
+      vk_set_error("vk_tune_set: VK_TUNE_MOLPRE takes 0 or 1, not %d", value);
+      return VK_ERR_ARG;
+    }
+    g_mol_precompute = value;
+    return VK_OK;
+  }
   if (knob != VK_TUNE_FACTOR || sl < 1 || sl > 7) {
     vk_set_error("vk_tune_set: unknown knob %d or bucket %d", knob, sl);
     return VK_ERR_ARG;
@@ -1009,8 +1026,8 @@ int vk_tune_set(int knob, int sl, int value) {
   g_factor_variant[sl] = value;
   return VK_OK;
 }
-
 int vk_tune_get(int knob, int sl) {
+  if (knob == VK_TUNE_MOLPRE) return g_mol_precompute;
   if (knob != VK_TUNE_FACTOR || sl < 1 || sl > 7) return -1;
   return g_factor_variant[sl];
 }
