@@ -164,14 +164,16 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
  * 64-192 lanes per column, SL <= 6; bitwise equal to TILE) or
  * VK_FACTOR_BLOCKED (16-column panels eliminated in LDS, the rest updated
  * by rank-16 FP64 MFMA products; same pivots, rounding differs in those
- * products) or VK_FACTOR_TILE2 (TILE's algorithm with two waves per column,
- * two rows per lane per tile row; bitwise equal to TILE).  Process-wide;
- * set it between launches. */
+ * products), VK_FACTOR_TILE2 (TILE's algorithm with two waves per column,
+ * two rows per lane per tile row; bitwise equal to TILE) or VK_FACTOR_WAVE
+ * (the same with one wave per column, four rows per lane per tile row; no
+ * barrier inside a column).  Process-wide; set it between launches. */
 #define VK_TUNE_FACTOR 0
 #define VK_FACTOR_TILE 0
 #define VK_FACTOR_ROWS 1
 #define VK_FACTOR_BLOCKED 2
 #define VK_FACTOR_TILE2 3
+#define VK_FACTOR_WAVE 4
 /* Knob VK_TUNE_MOLPRE (sl ignored): 1 (default) has vk_ros2_step_f64 compute
  * the molecular-diffusion factors that do not depend on y once per step;
  * 0 forms them inside each chemistry launch.  Results are bitwise equal. */

@@ -491,18 +491,19 @@ int vk_launch_bt_factor_rows(int sl, const VkCol* cols, int ncols, const double*
                              const double* sub_d, double* inv, const double* rhs, double* rmod,
                              int* status, const int* active, hipStream_t stream);
 
-int vk_launch_bt_factor2(int sl, const VkCol* cols, int ncols, const double* diag, int diag_tile,
-                         int transpose, const double* sup_d, const double* sub_d, double* inv,
-                         const double* rhs, double* rmod, int* status, const int* active,
-                         hipStream_t stream);
+int vk_launch_bt_factor2(int sl, int rh, const VkCol* cols, int ncols, const double* diag,
+                         int diag_tile, int transpose, const double* sup_d, const double* sub_d,
+                         double* inv, const double* rhs, double* rmod, int* status,
+                         const int* active, hipStream_t stream);
 int vk_launch_bt_factor_blk(int sl, const VkCol* cols, int ncols, const double* diag,
                             int diag_tile, int transpose, const double* sup_d, const double* sub_d,
                             double* inv, int* status, const int* active, hipStream_t stream);
 
 // Factor variant per SL bucket: VK_FACTOR_TILE (256 threads, 16x16 register
 // tiles), VK_FACTOR_ROWS (half a row per lane), VK_FACTOR_BLOCKED (16-column
-// panels, MFMA trailing updates; no fused forward sweep) or VK_FACTOR_TILE2
-// (the tile kernel on 128 threads, vk_bt_tile2.hip).  Set by vk_tune_set.
+// panels, MFMA trailing updates; no fused forward sweep), VK_FACTOR_TILE2 or
+// VK_FACTOR_WAVE (the tile kernel on 128 or 64 threads, vk_bt_tile2.hip).
+// Set by vk_tune_set.
 int g_factor_variant[8] = {VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE,
                            VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE};
 
@@ -514,9 +515,11 @@ int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag
                         int transpose, const double* sup_d, const double* sub_d, double* inv,
                         const double* rhs, double* rmod, int* status, const int* active,
                         hipStream_t stream) {
-  if (sl >= 1 && sl <= 7 && g_factor_variant[sl] == VK_FACTOR_TILE2)
-    return vk_launch_bt_factor2(sl, cols, ncols, diag, diag_tile, transpose, sup_d, sub_d, inv,
-                                rhs, rmod, status, active, stream);
+  if (sl >= 1 && sl <= 7 &&
+      (g_factor_variant[sl] == VK_FACTOR_TILE2 || g_factor_variant[sl] == VK_FACTOR_WAVE))
+    return vk_launch_bt_factor2(sl, g_factor_variant[sl] == VK_FACTOR_TILE2 ? 2 : 4, cols, ncols,
+                                diag, diag_tile, transpose, sup_d, sub_d, inv, rhs, rmod, status,
+                                active, stream);
   if (sl >= 1 && sl <= 6 && g_factor_variant[sl] == VK_FACTOR_ROWS)
     return vk_launch_bt_factor_rows(sl, cols, ncols, diag, diag_tile, transpose, sup_d, sub_d,
                                     inv, rhs, rmod, status, active, stream);

@@ -1,64 +1,71 @@
-// Two-wave variant of the tile block factor (k_bt_factor in vk_bt.hip).
+// Fewer-wave variants of the tile block factor (k_bt_factor in vk_bt.hip).
 //
 // Same algorithm, pivots and per-element arithmetic (bitwise-equal inverse
 // stacks and fused forward sweep): the block-Thomas recursion of
 // solver.factor_block_thomas_diag_offdiag (solver.py:33-64), each A'_j
 // inverted in registers by swap-free Gauss-Jordan with the first-max pivot.
 //
-// One workgroup of 128 threads (two waves) owns one column.  Thread
-// (tr, tc) = (tid/16, tid%16), tr in 0..7, holds the 2SL x SL register tile
-//   a[r][t] = A[16 (r/2) + 8 (r%2) + tr][16 t + tc]      (row slot r),
-// so each pivot step's barrier joins two waves instead of four, and each
-// wave's search and fix-ups are spread over twice the rows.  A tile entry
-// sits in the same place of the tile layout as for k_bt_factor:
-//   ((r/2) SL + t) * 256 + (r%2) * 128 + tid.
+// One workgroup of 256/RH threads owns one column: RH = 2 gives two waves,
+// RH = 4 one wave (no barrier between the waves of a column at all).  Thread
+// (tr, tc) = (tid/16, tid%16), tr < 16/RH, holds the (RH SL) x SL register
+// tile
+//   a[r][t] = A[16 (r/RH) + (16/RH) (r%RH) + tr][16 t + tc]   (row slot r),
+// so each wave's search and fix-ups cover RH times the rows of the
+// four-wave kernel.  A tile entry sits in the same place of the tile layout
+// as for k_bt_factor: ((r/RH) SL + t) * 256 + (r%RH) * (256/RH) + tid.
 #include "vk_common.h"
 
 #include <utility>
 
 namespace {
 
-constexpr int NT2 = 128;          // threads per column
-constexpr int NW2 = NT2 / 64;     // waves per column
-constexpr int TR2 = NT2 / VK_G;   // lane rows (tr range)
+template <int RH>
+struct Shape {
+  static constexpr int NT = VK_NT / RH;  // threads per column
+  static constexpr int NW = NT / 64;     // waves per column
+  static constexpr int TR = VK_G / RH;   // lane rows (tr range)
+};
 
-template <int SL>
+template <int SL, int RH>
 struct Bt2Shared {
-  double cand_row[2][NW2][VK_G * SL];  // each wave's best pivot row
-  double cand_val[2][NW2];
-  int cand_idx[2][NW2];
+  static constexpr int NW = Shape<RH>::NW;
+  double cand_row[2][NW][VK_G * SL];  // each wave's best pivot row
+  double cand_val[2][NW];
+  int cand_idx[2][NW];
   double xbuf[VK_G * VK_G * SL];       // permutation staging (16 x N)
   double vec[2][VK_G * SL];            // sweep vector, double-buffered by layer
   int perm[VK_G * SL];
   int iperm[VK_G * SL];
 };
 
-// Row of A held in row slot r by lane row tr.
+// Row of A held in row slot r by lane row tr; row p sits in slot p / TR of
+// lane row p % TR.
+template <int RH>
 __device__ __forceinline__ int row_of(int r, int tr) {
-  return VK_G * (r >> 1) + TR2 * (r & 1) + tr;
+  return VK_G * (r / RH) + Shape<RH>::TR * (r % RH) + tr;
 }
 
 // Offset of entry (r, t) of this thread in one tile-layout block.
-template <int SL>
+template <int SL, int RH>
 __device__ __forceinline__ int tile_off(int r, int t, int tid) {
-  return ((r >> 1) * SL + t) * VK_NT + (r & 1) * NT2 + tid;
+  return ((r / RH) * SL + t) * VK_NT + (r % RH) * Shape<RH>::NT + tid;
 }
 
-template <int SL>
+template <int SL, int RH>
 struct Gj2Ctx {
-  Bt2Shared<SL>* sh;
+  Bt2Shared<SL, RH>* sh;
   int n, kb, bad;
-  double bias[2 * SL];  // 0 while the slot's row is a candidate, -inf after
+  double bias[RH * SL];  // 0 while the slot's row is a candidate, -inf after
 };
 
 // One Gauss-Jordan step with pivot column k = 16 kb + KC (register column 0
 // after the column rotation); the arithmetic of gj_step in vk_bt.hip.
-template <int SL, int KC>
-__device__ __forceinline__ bool gj2_step(double (&a)[2 * SL][SL], Gj2Ctx<SL>& c) {
-  constexpr int R = 2 * SL;
+template <int SL, int RH, int KC>
+__device__ __forceinline__ bool gj2_step(double (&a)[RH * SL][SL], Gj2Ctx<SL, RH>& c) {
+  constexpr int R = RH * SL, TR = Shape<RH>::TR, NW = Shape<RH>::NW;
   const int k = c.kb * VK_G + KC;
   if (k >= c.n) return false;  // uniform: the last block may be partial
-  Bt2Shared<SL>& sh = *c.sh;
+  Bt2Shared<SL, RH>& sh = *c.sh;
   const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int buf = k & 1;
@@ -78,15 +85,14 @@ __device__ __forceinline__ bool gj2_step(double (&a)[2 * SL][SL], Gj2Ctx<SL>& c)
 #pragma unroll
     for (int r = R - 1; r >= 0; --r) {
       const unsigned long long bal = __ballot(rank[r] == m);
-      if (bal) brow = row_of(r, wave * 4 + (int)(__builtin_ctzll(bal) >> 4));
+      if (bal) brow = row_of<RH>(r, wave * 4 + (int)(__builtin_ctzll(bal) >> 4));
     }
   }
   brow = __builtin_amdgcn_readfirstlane(brow);
-  // The owning row group publishes the candidate row (rotated order); its
-  // slot is brow / 8, its lane row brow % 8.
+  // The owning row group publishes the candidate row (rotated order).
   if (brow != 0x7fffffff) {
-    const unsigned wsm = __builtin_amdgcn_readfirstlane(1u << (brow >> 3));
-    if (tr == (brow & 7)) {
+    const unsigned wsm = __builtin_amdgcn_readfirstlane(1u << (brow / TR));
+    if (tr == brow % TR) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (wsm & (1u << r)) {
@@ -106,7 +112,7 @@ __device__ __forceinline__ bool gj2_step(double (&a)[2 * SL][SL], Gj2Ctx<SL>& c)
   int p = sh.cand_idx[buf][0];
   int pw = 0;
 #pragma unroll
-  for (int w = 1; w < NW2; ++w) {
+  for (int w = 1; w < NW; ++w) {
     const double v = sh.cand_val[buf][w];
     const int rr = sh.cand_idx[buf][w];
     if (v > pval || (v == pval && rr < p)) { pval = v; p = rr; pw = w; }
@@ -135,8 +141,8 @@ __device__ __forceinline__ bool gj2_step(double (&a)[2 * SL][SL], Gj2Ctx<SL>& c)
 #pragma unroll
     for (int r = 0; r < R; ++r) a[r][0] = -l[r];
   }
-  const unsigned psm = __builtin_amdgcn_readfirstlane(1u << (p >> 3));
-  if (tr == (p & 7)) {
+  const unsigned psm = __builtin_amdgcn_readfirstlane(1u << (p / TR));
+  if (tr == p % TR) {
     double pr[SL];
 #pragma unroll
     for (int t = 0; t < SL; ++t) pr[t] = u[t] * rpiv;
@@ -153,25 +159,26 @@ __device__ __forceinline__ bool gj2_step(double (&a)[2 * SL][SL], Gj2Ctx<SL>& c)
   return true;
 }
 
-template <int SL, int... KC>
-__device__ __forceinline__ void gj2_block(double (&a)[2 * SL][SL], Gj2Ctx<SL>& c,
+template <int SL, int RH, int... KC>
+__device__ __forceinline__ void gj2_block(double (&a)[RH * SL][SL], Gj2Ctx<SL, RH>& c,
                                           std::integer_sequence<int, KC...>) {
-  (void)(gj2_step<SL, KC>(a, c) && ...);
+  (void)(gj2_step<SL, RH, KC>(a, c) && ...);
 }
 
 // Invert the register-resident matrix in place (natural order on return);
 // nonzero in every thread when a zero or non-finite pivot was met.
-template <int SL>
-__device__ __forceinline__ int gj2_invert(double (&a)[2 * SL][SL], int n, Bt2Shared<SL>& sh) {
-  constexpr int R = 2 * SL, N = VK_G * SL;
+template <int SL, int RH>
+__device__ __forceinline__ int gj2_invert(double (&a)[RH * SL][SL], int n,
+                                          Bt2Shared<SL, RH>& sh) {
+  constexpr int R = RH * SL, N = VK_G * SL, NT = Shape<RH>::NT, TR = Shape<RH>::TR;
   const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
-  Gj2Ctx<SL> c{&sh, n, 0, 0, {}};
+  Gj2Ctx<SL, RH> c{&sh, n, 0, 0, {}};
 #pragma unroll
-  for (int r = 0; r < R; ++r) c.bias[r] = (row_of(r, tr) < n) ? 0.0 : -INFINITY;
+  for (int r = 0; r < R; ++r) c.bias[r] = (row_of<RH>(r, tr) < n) ? 0.0 : -INFINITY;
 #pragma unroll 1
   for (int kb = 0; kb < SL; ++kb) {
     c.kb = kb;
-    gj2_block<SL>(a, c, std::make_integer_sequence<int, VK_G>{});
+    gj2_block<SL, RH>(a, c, std::make_integer_sequence<int, VK_G>{});
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const double first = a[r][0];
@@ -181,34 +188,34 @@ __device__ __forceinline__ int gj2_invert(double (&a)[2 * SL][SL], int n, Bt2Sha
     }
   }
   const int bad = c.bad;
-  for (int i = n + tid; i < N; i += NT2) sh.perm[i] = i;
-  for (int i = tid; i < N; i += NT2) sh.iperm[i] = i;
+  for (int i = n + tid; i < N; i += NT) sh.perm[i] = i;
+  for (int i = tid; i < N; i += NT) sh.iperm[i] = i;
   __syncthreads();
-  for (int i = tid; i < N; i += NT2) sh.iperm[sh.perm[i]] = i;
+  for (int i = tid; i < N; i += NT) sh.iperm[sh.perm[i]] = i;
   __syncthreads();
   // R -> X with X[a][b] = R[pi(a)][pi^-1(b)]: the columns one tile row at a
   // time (16 rows x N in xbuf), then the rows one tile column at a time.
 #pragma unroll
   for (int s = 0; s < SL; ++s) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+    for (int h = 0; h < RH; ++h)
 #pragma unroll
       for (int t = 0; t < SL; ++t)
-        sh.xbuf[(TR2 * h + tr) * N + sh.perm[t * VK_G + tc]] = a[2 * s + h][t];
+        sh.xbuf[(TR * h + tr) * N + sh.perm[t * VK_G + tc]] = a[RH * s + h][t];
     __syncthreads();
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+    for (int h = 0; h < RH; ++h)
 #pragma unroll
-      for (int t = 0; t < SL; ++t) a[2 * s + h][t] = sh.xbuf[(TR2 * h + tr) * N + t * VK_G + tc];
+      for (int t = 0; t < SL; ++t) a[RH * s + h][t] = sh.xbuf[(TR * h + tr) * N + t * VK_G + tc];
     __syncthreads();
   }
 #pragma unroll
   for (int t = 0; t < SL; ++t) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) sh.xbuf[sh.iperm[row_of(r, tr)] * VK_G + tc] = a[r][t];
+    for (int r = 0; r < R; ++r) sh.xbuf[sh.iperm[row_of<RH>(r, tr)] * VK_G + tc] = a[r][t];
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < R; ++r) a[r][t] = sh.xbuf[row_of(r, tr) * VK_G + tc];
+    for (int r = 0; r < R; ++r) a[r][t] = sh.xbuf[row_of<RH>(r, tr) * VK_G + tc];
     __syncthreads();
   }
   return bad;
@@ -223,17 +230,21 @@ __device__ inline double nat_elem2(const double* blk, int ni, int i, int q, int 
 
 }  // namespace
 
-// k_bt_factor with two waves per column (see the top of this file): diag is
-// natural (TILE_IN false) or tile layout (TILE_IN true, may alias inv); a
-// non-null rhs fuses the forward sweep into rmod.
-template <int SL, bool TILE_IN>
-__global__ __launch_bounds__(NT2) __attribute__((amdgpu_waves_per_eu(TILE_IN && SL <= 6 ? 2 : 1))) void k_bt_factor2(
+// k_bt_factor with 256/RH threads per column (see the top of this file):
+// diag is natural (TILE_IN false) or tile layout (TILE_IN true, may alias
+// inv); a non-null rhs fuses the forward sweep into rmod.  The two-wave
+// tile-input kernel is held to 256 VGPRs for SL <= 6 (two waves per SIMD);
+// the one-wave kernel may take all 512 registers of a lane (VGPRs and
+// AGPRs), one wave per SIMD.
+template <int SL, bool TILE_IN, int RH>
+__global__ __launch_bounds__(VK_NT / RH) __attribute__((amdgpu_waves_per_eu(
+    RH == 2 && TILE_IN && SL <= 6 ? 2 : 1))) void k_bt_factor2(
     const VkCol* __restrict__ cols, int ncols, const double* diag, int transpose,
     const double* __restrict__ sup_d, const double* __restrict__ sub_d, double* inv,
     const double* __restrict__ rhs, double* __restrict__ rmod, int* __restrict__ status,
     const int* __restrict__ active) {
-  constexpr int R = 2 * SL;
-  __shared__ Bt2Shared<SL> sh;
+  constexpr int R = RH * SL, NT = Shape<RH>::NT;
+  __shared__ Bt2Shared<SL, RH> sh;
   const int c = blockIdx.x;
   if (c >= ncols) return;
   if (active && (active[c] & VK_ST_DONE)) return;
@@ -256,12 +267,12 @@ __global__ __launch_bounds__(NT2) __attribute__((amdgpu_waves_per_eu(TILE_IN && 
 #pragma unroll
         for (int t = 0; t < SL; ++t) {
           if constexpr (TILE_IN)
-            a[r][t] = Dj_tile[tile_off<SL>(r, t, tid)];
+            a[r][t] = Dj_tile[tile_off<SL, RH>(r, t, tid)];
           else
-            a[r][t] = nat_elem2(Dj_nat, ni, row_of(r, tr), t * VK_G + tc, transpose);
+            a[r][t] = nat_elem2(Dj_nat, ni, row_of<RH>(r, tr), t * VK_G + tc, transpose);
         }
       if (sweep) {
-        for (int i = tid; i < VK_G * SL; i += NT2) {
+        for (int i = tid; i < VK_G * SL; i += NT) {
           const double v = (i < ni) ? rhs[col.off_s + i] : 0.0;
           sh.vec[0][i] = v;
           if (i < ni) rmod[col.off_s + i] = v;
@@ -271,7 +282,7 @@ __global__ __launch_bounds__(NT2) __attribute__((amdgpu_waves_per_eu(TILE_IN && 
       double cv[R], bv[SL];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int i = row_of(r, tr);
+        const int i = row_of<RH>(r, tr);
         cv[r] = (i < ni) ? bsub[(int64_t)(j - 1) * ni + i] : 0.0;
       }
 #pragma unroll
@@ -296,7 +307,7 @@ __global__ __launch_bounds__(NT2) __attribute__((amdgpu_waves_per_eu(TILE_IN && 
         if (tc == 0) {
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const int i = row_of(r, tr);
+            const int i = row_of<RH>(r, tr);
             double rr = 0.0;
             if (i < ni) {
               rr = rhs[col.off_s + (int64_t)j * ni + i] - cv[r] * w[r];
@@ -313,43 +324,47 @@ __global__ __launch_bounds__(NT2) __attribute__((amdgpu_waves_per_eu(TILE_IN && 
         for (int t = 0; t < SL; ++t) {
           double d;
           if constexpr (TILE_IN)
-            d = Dj_tile[tile_off<SL>(r, t, tid)];
+            d = Dj_tile[tile_off<SL, RH>(r, t, tid)];
           else
-            d = nat_elem2(Dj_nat, ni, row_of(r, tr), t * VK_G + tc, transpose);
+            d = nat_elem2(Dj_nat, ni, row_of<RH>(r, tr), t * VK_G + tc, transpose);
           a[r][t] = d - (cv[r] * bv[t]) * a[r][t];
         }
         __builtin_amdgcn_sched_barrier(0);
       }
     }
-    bad |= gj2_invert<SL>(a, ni, sh);
+    bad |= gj2_invert<SL, RH>(a, ni, sh);
     double* dst = inv + col.off_b + (int64_t)j * BLK;
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
-      for (int t = 0; t < SL; ++t) dst[tile_off<SL>(r, t, tid)] = a[r][t];
+      for (int t = 0; t < SL; ++t) dst[tile_off<SL, RH>(r, t, tid)] = a[r][t];
   }
   if (bad && tid == 0 && status) atomicOr(&status[c], VK_ST_SINGULAR);
 }
 
-int vk_launch_bt_factor2(int sl, const VkCol* cols, int ncols, const double* diag, int diag_tile,
-                         int transpose, const double* sup_d, const double* sub_d, double* inv,
-                         const double* rhs, double* rmod, int* status, const int* active,
-                         hipStream_t stream) {
-#define VK_CASE2(n)                                                                          \
-  case n:                                                                                    \
-    if (diag_tile)                                                                           \
-      hipLaunchKernelGGL((k_bt_factor2<n, true>), dim3(ncols), dim3(NT2), 0, stream, cols,   \
-                         ncols, diag, transpose, sup_d, sub_d, inv, rhs, rmod, status, active); \
-    else                                                                                     \
-      hipLaunchKernelGGL((k_bt_factor2<n, false>), dim3(ncols), dim3(NT2), 0, stream, cols,  \
-                         ncols, diag, transpose, sup_d, sub_d, inv, rhs, rmod, status, active); \
+int vk_launch_bt_factor2(int sl, int rh, const VkCol* cols, int ncols, const double* diag,
+                         int diag_tile, int transpose, const double* sup_d, const double* sub_d,
+                         double* inv, const double* rhs, double* rmod, int* status,
+                         const int* active, hipStream_t stream) {
+#define VK_LAUNCH2(n, T, H)                                                                   \
+  hipLaunchKernelGGL((k_bt_factor2<n, T, H>), dim3(ncols), dim3(VK_NT / H), 0, stream, cols, \
+                     ncols, diag, transpose, sup_d, sub_d, inv, rhs, rmod, status, active)
+#define VK_CASE2(n)                                                    \
+  case n:                                                              \
+    if (rh == 2) {                                                     \
+      if (diag_tile) VK_LAUNCH2(n, true, 2); else VK_LAUNCH2(n, false, 2); \
+    } else {                                                           \
+      if (diag_tile) VK_LAUNCH2(n, true, 4); else VK_LAUNCH2(n, false, 4); \
+    }                                                                  \
     break;
+  if (rh != 2 && rh != 4) return VK_ERR_ARG;
   switch (sl) {
     VK_CASE2(1) VK_CASE2(2) VK_CASE2(3) VK_CASE2(4) VK_CASE2(5) VK_CASE2(6) VK_CASE2(7)
     default:
       return VK_ERR_UNSUPPORTED;
   }
 #undef VK_CASE2
+#undef VK_LAUNCH2
   VK_CHECK_HIP(hipGetLastError());
   return VK_OK;
 }
