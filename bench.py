@@ -73,7 +73,7 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-alone", type=int, default=0,
                     help="requests also solved one at a time, for the comparison")
-    ap.add_argument("--factor", choices=("auto", "tile", "rows", "blocked", "tile2", "wave"), default="auto",
+    ap.add_argument("--factor", choices=("auto", "tile", "rows", "blocked", "tile2", "wave", "tile8"), default="auto",
                     help="block-factor kernel for the nz=150 / ni=93 bucket (auto: the "
                          "library default)")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
@@ -104,7 +104,7 @@ def main() -> int:
     lib = _lib.load()
     if args.factor != "auto":
         _lib.check(lib.vk_tune_set(0, (NI + 15) // 16,
-                                   {"tile": 0, "rows": 1, "blocked": 2, "tile2": 3, "wave": 4}[args.factor]),
+                                   {"tile": 0, "rows": 1, "blocked": 2, "tile2": 3, "wave": 4, "tile8": 5}[args.factor]),
                    "vk_tune_set")
     B = args.batch
     t_gen = time.time()

@@ -167,13 +167,15 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
  * products), VK_FACTOR_TILE2 (TILE's algorithm with two waves per column,
  * two rows per lane per tile row; bitwise equal to TILE) or VK_FACTOR_WAVE
  * (the same with one wave per column, four rows per lane per tile row; no
- * barrier inside a column).  Process-wide; set it between launches. */
+ * barrier inside a column) or VK_FACTOR_TILE8 (eight waves per column, half
+ * the rows per lane; even SL).  Process-wide; set it between launches. */
 #define VK_TUNE_FACTOR 0
 #define VK_FACTOR_TILE 0
 #define VK_FACTOR_ROWS 1
 #define VK_FACTOR_BLOCKED 2
 #define VK_FACTOR_TILE2 3
 #define VK_FACTOR_WAVE 4
+#define VK_FACTOR_TILE8 5
 /* Knob VK_TUNE_MOLPRE (sl ignored): 1 (default) has vk_ros2_step_f64 compute
  * the molecular-diffusion factors that do not depend on y once per step;
  * 0 forms them inside each chemistry launch.  Results are bitwise equal. */

@@ -1019,7 +1019,8 @@ int vk_tune_set(int knob, int sl, int value) {
     return VK_ERR_ARG;
   }
   if (value != VK_FACTOR_TILE && !(value == VK_FACTOR_ROWS && sl <= 6) &&
-      value != VK_FACTOR_BLOCKED && value != VK_FACTOR_TILE2 && value != VK_FACTOR_WAVE) {
+      value != VK_FACTOR_BLOCKED && value != VK_FACTOR_TILE2 && value != VK_FACTOR_WAVE &&
+      !(value == VK_FACTOR_TILE8 && sl % 2 == 0)) {
     vk_set_error("vk_tune_set: factor variant %d not available for bucket %d", value, sl);
     return VK_ERR_UNSUPPORTED;
   }

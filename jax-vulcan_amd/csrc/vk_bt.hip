@@ -491,7 +491,7 @@ int vk_launch_bt_factor_rows(int sl, const VkCol* cols, int ncols, const double*
                              const double* sub_d, double* inv, const double* rhs, double* rmod,
                              int* status, const int* active, hipStream_t stream);
 
-int vk_launch_bt_factor2(int sl, int rh, const VkCol* cols, int ncols, const double* diag,
+int vk_launch_bt_factor2(int sl, int nt, const VkCol* cols, int ncols, const double* diag,
                          int diag_tile, int transpose, const double* sup_d, const double* sub_d,
                          double* inv, const double* rhs, double* rmod, int* status,
                          const int* active, hipStream_t stream);
@@ -501,9 +501,9 @@ int vk_launch_bt_factor_blk(int sl, const VkCol* cols, int ncols, const double* 
 
 // Factor variant per SL bucket: VK_FACTOR_TILE (256 threads, 16x16 register
 // tiles), VK_FACTOR_ROWS (half a row per lane), VK_FACTOR_BLOCKED (16-column
-// panels, MFMA trailing updates; no fused forward sweep), VK_FACTOR_TILE2 or
-// VK_FACTOR_WAVE (the tile kernel on 128 or 64 threads, vk_bt_tile2.hip).
-// Set by vk_tune_set.
+// panels, MFMA trailing updates; no fused forward sweep), VK_FACTOR_TILE2,
+// VK_FACTOR_WAVE or VK_FACTOR_TILE8 (the tile kernel on 128, 64 or 512
+// threads, vk_bt_tile2.hip).  Set by vk_tune_set.
 int g_factor_variant[8] = {VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE,
                            VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE};
 
@@ -516,10 +516,13 @@ int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag
                         const double* rhs, double* rmod, int* status, const int* active,
                         hipStream_t stream) {
   if (sl >= 1 && sl <= 7 &&
-      (g_factor_variant[sl] == VK_FACTOR_TILE2 || g_factor_variant[sl] == VK_FACTOR_WAVE))
-    return vk_launch_bt_factor2(sl, g_factor_variant[sl] == VK_FACTOR_TILE2 ? 2 : 4, cols, ncols,
-                                diag, diag_tile, transpose, sup_d, sub_d, inv, rhs, rmod, status,
-                                active, stream);
+      (g_factor_variant[sl] == VK_FACTOR_TILE2 || g_factor_variant[sl] == VK_FACTOR_WAVE ||
+       g_factor_variant[sl] == VK_FACTOR_TILE8)) {
+    const int nt = g_factor_variant[sl] == VK_FACTOR_TILE2 ? 128
+                   : g_factor_variant[sl] == VK_FACTOR_WAVE ? 64 : 512;
+    return vk_launch_bt_factor2(sl, nt, cols, ncols, diag, diag_tile, transpose, sup_d, sub_d,
+                                inv, rhs, rmod, status, active, stream);
+  }
   if (sl >= 1 && sl <= 6 && g_factor_variant[sl] == VK_FACTOR_ROWS)
     return vk_launch_bt_factor_rows(sl, cols, ncols, diag, diag_tile, transpose, sup_d, sub_d,
                                     inv, rhs, rmod, status, active, stream);

@@ -1,38 +1,39 @@
-// Fewer-wave variants of the tile block factor (k_bt_factor in vk_bt.hip).
+// Variants of the tile block factor (k_bt_factor in vk_bt.hip) with other
+// thread counts per column.
 //
 // Same algorithm, pivots and per-element arithmetic (bitwise-equal inverse
 // stacks and fused forward sweep): the block-Thomas recursion of
 // solver.factor_block_thomas_diag_offdiag (solver.py:33-64), each A'_j
 // inverted in registers by swap-free Gauss-Jordan with the first-max pivot.
 //
-// One workgroup of 256/RH threads owns one column: RH = 2 gives two waves,
-// RH = 4 one wave (no barrier between the waves of a column at all).  Thread
-// (tr, tc) = (tid/16, tid%16), tr < 16/RH, holds the (RH SL) x SL register
-// tile
-//   a[r][t] = A[16 (r/RH) + (16/RH) (r%RH) + tr][16 t + tc]   (row slot r),
-// so each wave's search and fix-ups cover RH times the rows of the
-// four-wave kernel.  A tile entry sits in the same place of the tile layout
-// as for k_bt_factor: ((r/RH) SL + t) * 256 + (r%RH) * (256/RH) + tid.
+// One workgroup of NT threads owns one column: NT = 64 (one wave, no
+// barrier between the waves of a column), 128 (two waves) or 512 (eight
+// waves, even SL only).  Thread (tr, tc) = (tid/16, tid%16), tr < NT/16,
+// holds the R x SL register tile, R = 16 SL / (NT/16),
+//   a[r][t] = A[(NT/16) r + tr][16 t + tc]                   (row slot r),
+// so each wave's search and fix-ups cover 256/NT times the rows of the
+// four-wave kernel.  Entry (i, q) sits where the tile layout of k_bt_factor
+// puts it: ((i/16) SL + q/16) * 256 + (i%16) * 16 + q%16.
 #include "vk_common.h"
 
 #include <utility>
 
 namespace {
 
-template <int RH>
+template <int NT>
 struct Shape {
-  static constexpr int NT = VK_NT / RH;  // threads per column
-  static constexpr int NW = NT / 64;     // waves per column
-  static constexpr int TR = VK_G / RH;   // lane rows (tr range)
+  static constexpr int NW = NT / 64;    // waves per column
+  static constexpr int TR = NT / VK_G;  // lane rows (tr range)
 };
 
-template <int SL, int RH>
+template <int SL, int NT>
 struct Bt2Shared {
-  static constexpr int NW = Shape<RH>::NW;
+  static constexpr int NW = Shape<NT>::NW, TR = Shape<NT>::TR;
+  static constexpr int XB = (TR > VK_G ? TR : VK_G) * VK_G * SL;
   double cand_row[2][NW][VK_G * SL];  // each wave's best pivot row
   double cand_val[2][NW];
   int cand_idx[2][NW];
-  double xbuf[VK_G * VK_G * SL];       // permutation staging (16 x N)
+  double xbuf[XB];                     // permutation staging (max(TR,16) x N)
   double vec[2][VK_G * SL];            // sweep vector, double-buffered by layer
   int perm[VK_G * SL];
   int iperm[VK_G * SL];
@@ -40,32 +41,34 @@ struct Bt2Shared {
 
 // Row of A held in row slot r by lane row tr; row p sits in slot p / TR of
 // lane row p % TR.
-template <int RH>
+template <int NT>
 __device__ __forceinline__ int row_of(int r, int tr) {
-  return VK_G * (r / RH) + Shape<RH>::TR * (r % RH) + tr;
+  return Shape<NT>::TR * r + tr;
 }
 
 // Offset of entry (r, t) of this thread in one tile-layout block.
-template <int SL, int RH>
+template <int SL, int NT>
 __device__ __forceinline__ int tile_off(int r, int t, int tid) {
-  return ((r / RH) * SL + t) * VK_NT + (r % RH) * Shape<RH>::NT + tid;
+  const int row = row_of<NT>(r, tid >> 4);
+  return ((row >> 4) * SL + t) * VK_NT + ((row & 15) << 4) + (tid & 15);
 }
 
-template <int SL, int RH>
+template <int SL, int NT>
 struct Gj2Ctx {
-  Bt2Shared<SL, RH>* sh;
+  Bt2Shared<SL, NT>* sh;
   int n, kb, bad;
-  double bias[RH * SL];  // 0 while the slot's row is a candidate, -inf after
+  double bias[VK_G * SL / Shape<NT>::TR];  // 0 while the slot's row is a candidate, -inf after
 };
 
 // One Gauss-Jordan step with pivot column k = 16 kb + KC (register column 0
 // after the column rotation); the arithmetic of gj_step in vk_bt.hip.
-template <int SL, int RH, int KC>
-__device__ __forceinline__ bool gj2_step(double (&a)[RH * SL][SL], Gj2Ctx<SL, RH>& c) {
-  constexpr int R = RH * SL, TR = Shape<RH>::TR, NW = Shape<RH>::NW;
+template <int SL, int NT, int KC>
+__device__ __forceinline__ bool gj2_step(double (&a)[VK_G * SL / Shape<NT>::TR][SL],
+                                         Gj2Ctx<SL, NT>& c) {
+  constexpr int TR = Shape<NT>::TR, NW = Shape<NT>::NW, R = VK_G * SL / TR;
   const int k = c.kb * VK_G + KC;
   if (k >= c.n) return false;  // uniform: the last block may be partial
-  Bt2Shared<SL, RH>& sh = *c.sh;
+  Bt2Shared<SL, NT>& sh = *c.sh;
   const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int buf = k & 1;
@@ -85,7 +88,7 @@ __device__ __forceinline__ bool gj2_step(double (&a)[RH * SL][SL], Gj2Ctx<SL, RH
 #pragma unroll
     for (int r = R - 1; r >= 0; --r) {
       const unsigned long long bal = __ballot(rank[r] == m);
-      if (bal) brow = row_of<RH>(r, wave * 4 + (int)(__builtin_ctzll(bal) >> 4));
+      if (bal) brow = row_of<NT>(r, wave * 4 + (int)(__builtin_ctzll(bal) >> 4));
     }
   }
   brow = __builtin_amdgcn_readfirstlane(brow);
@@ -159,26 +162,26 @@ __device__ __forceinline__ bool gj2_step(double (&a)[RH * SL][SL], Gj2Ctx<SL, RH
   return true;
 }
 
-template <int SL, int RH, int... KC>
-__device__ __forceinline__ void gj2_block(double (&a)[RH * SL][SL], Gj2Ctx<SL, RH>& c,
-                                          std::integer_sequence<int, KC...>) {
-  (void)(gj2_step<SL, RH, KC>(a, c) && ...);
+template <int SL, int NT, int... KC>
+__device__ __forceinline__ void gj2_block(double (&a)[VK_G * SL / Shape<NT>::TR][SL],
+                                          Gj2Ctx<SL, NT>& c, std::integer_sequence<int, KC...>) {
+  (void)(gj2_step<SL, NT, KC>(a, c) && ...);
 }
 
 // Invert the register-resident matrix in place (natural order on return);
 // nonzero in every thread when a zero or non-finite pivot was met.
-template <int SL, int RH>
-__device__ __forceinline__ int gj2_invert(double (&a)[RH * SL][SL], int n,
-                                          Bt2Shared<SL, RH>& sh) {
-  constexpr int R = RH * SL, N = VK_G * SL, NT = Shape<RH>::NT, TR = Shape<RH>::TR;
+template <int SL, int NT>
+__device__ __forceinline__ int gj2_invert(double (&a)[VK_G * SL / Shape<NT>::TR][SL], int n,
+                                          Bt2Shared<SL, NT>& sh) {
+  constexpr int TR = Shape<NT>::TR, R = VK_G * SL / TR, N = VK_G * SL;
   const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
-  Gj2Ctx<SL, RH> c{&sh, n, 0, 0, {}};
+  Gj2Ctx<SL, NT> c{&sh, n, 0, 0, {}};
 #pragma unroll
-  for (int r = 0; r < R; ++r) c.bias[r] = (row_of<RH>(r, tr) < n) ? 0.0 : -INFINITY;
+  for (int r = 0; r < R; ++r) c.bias[r] = (row_of<NT>(r, tr) < n) ? 0.0 : -INFINITY;
 #pragma unroll 1
   for (int kb = 0; kb < SL; ++kb) {
     c.kb = kb;
-    gj2_block<SL, RH>(a, c, std::make_integer_sequence<int, VK_G>{});
+    gj2_block<SL, NT>(a, c, std::make_integer_sequence<int, VK_G>{});
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const double first = a[r][0];
@@ -193,29 +196,24 @@ __device__ __forceinline__ int gj2_invert(double (&a)[RH * SL][SL], int n,
   __syncthreads();
   for (int i = tid; i < N; i += NT) sh.iperm[sh.perm[i]] = i;
   __syncthreads();
-  // R -> X with X[a][b] = R[pi(a)][pi^-1(b)]: the columns one tile row at a
-  // time (16 rows x N in xbuf), then the rows one tile column at a time.
+  // R -> X with X[a][b] = R[pi(a)][pi^-1(b)]: the columns one row slot at a
+  // time (TR rows x N in xbuf), then the rows one tile column at a time.
 #pragma unroll
-  for (int s = 0; s < SL; ++s) {
+  for (int r = 0; r < R; ++r) {
 #pragma unroll
-    for (int h = 0; h < RH; ++h)
-#pragma unroll
-      for (int t = 0; t < SL; ++t)
-        sh.xbuf[(TR * h + tr) * N + sh.perm[t * VK_G + tc]] = a[RH * s + h][t];
+    for (int t = 0; t < SL; ++t) sh.xbuf[tr * N + sh.perm[t * VK_G + tc]] = a[r][t];
     __syncthreads();
 #pragma unroll
-    for (int h = 0; h < RH; ++h)
-#pragma unroll
-      for (int t = 0; t < SL; ++t) a[RH * s + h][t] = sh.xbuf[(TR * h + tr) * N + t * VK_G + tc];
+    for (int t = 0; t < SL; ++t) a[r][t] = sh.xbuf[tr * N + t * VK_G + tc];
     __syncthreads();
   }
 #pragma unroll
   for (int t = 0; t < SL; ++t) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) sh.xbuf[sh.iperm[row_of<RH>(r, tr)] * VK_G + tc] = a[r][t];
+    for (int r = 0; r < R; ++r) sh.xbuf[sh.iperm[row_of<NT>(r, tr)] * VK_G + tc] = a[r][t];
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < R; ++r) a[r][t] = sh.xbuf[row_of<RH>(r, tr) * VK_G + tc];
+    for (int r = 0; r < R; ++r) a[r][t] = sh.xbuf[row_of<NT>(r, tr) * VK_G + tc];
     __syncthreads();
   }
   return bad;
@@ -230,21 +228,21 @@ __device__ inline double nat_elem2(const double* blk, int ni, int i, int q, int 
 
 }  // namespace
 
-// k_bt_factor with 256/RH threads per column (see the top of this file):
-// diag is natural (TILE_IN false) or tile layout (TILE_IN true, may alias
-// inv); a non-null rhs fuses the forward sweep into rmod.  The two-wave
-// tile-input kernel is held to 256 VGPRs for SL <= 6 (two waves per SIMD);
-// the one-wave kernel may take all 512 registers of a lane (VGPRs and
-// AGPRs), one wave per SIMD.
-template <int SL, bool TILE_IN, int RH>
-__global__ __launch_bounds__(VK_NT / RH) __attribute__((amdgpu_waves_per_eu(
-    RH == 2 && TILE_IN && SL <= 6 ? 2 : 1))) void k_bt_factor2(
+// k_bt_factor with NT threads per column (see the top of this file): diag
+// is natural (TILE_IN false) or tile layout (TILE_IN true, may alias inv);
+// a non-null rhs fuses the forward sweep into rmod.  The two-wave
+// tile-input kernel is held to 256 VGPRs for SL <= 6 (two waves per SIMD),
+// the eight-wave kernel to 128 (two columns per CU); the one-wave kernel may
+// take all 512 registers of a lane (VGPRs and AGPRs), one wave per SIMD.
+template <int SL, bool TILE_IN, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(
+    NT == 512 ? 4 : (NT == 128 && TILE_IN && SL <= 6 ? 2 : 1)))) void k_bt_factor2(
     const VkCol* __restrict__ cols, int ncols, const double* diag, int transpose,
     const double* __restrict__ sup_d, const double* __restrict__ sub_d, double* inv,
     const double* __restrict__ rhs, double* __restrict__ rmod, int* __restrict__ status,
     const int* __restrict__ active) {
-  constexpr int R = RH * SL, NT = Shape<RH>::NT;
-  __shared__ Bt2Shared<SL, RH> sh;
+  constexpr int R = VK_G * SL / Shape<NT>::TR;
+  __shared__ Bt2Shared<SL, NT> sh;
   const int c = blockIdx.x;
   if (c >= ncols) return;
   if (active && (active[c] & VK_ST_DONE)) return;
@@ -267,9 +265,9 @@ __global__ __launch_bounds__(VK_NT / RH) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
         for (int t = 0; t < SL; ++t) {
           if constexpr (TILE_IN)
-            a[r][t] = Dj_tile[tile_off<SL, RH>(r, t, tid)];
+            a[r][t] = Dj_tile[tile_off<SL, NT>(r, t, tid)];
           else
-            a[r][t] = nat_elem2(Dj_nat, ni, row_of<RH>(r, tr), t * VK_G + tc, transpose);
+            a[r][t] = nat_elem2(Dj_nat, ni, row_of<NT>(r, tr), t * VK_G + tc, transpose);
         }
       if (sweep) {
         for (int i = tid; i < VK_G * SL; i += NT) {
@@ -282,7 +280,7 @@ __global__ __launch_bounds__(VK_NT / RH) __attribute__((amdgpu_waves_per_eu(
       double cv[R], bv[SL];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int i = row_of<RH>(r, tr);
+        const int i = row_of<NT>(r, tr);
         cv[r] = (i < ni) ? bsub[(int64_t)(j - 1) * ni + i] : 0.0;
       }
 #pragma unroll
@@ -307,7 +305,7 @@ __global__ __launch_bounds__(VK_NT / RH) __attribute__((amdgpu_waves_per_eu(
         if (tc == 0) {
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const int i = row_of<RH>(r, tr);
+            const int i = row_of<NT>(r, tr);
             double rr = 0.0;
             if (i < ni) {
               rr = rhs[col.off_s + (int64_t)j * ni + i] - cv[r] * w[r];
@@ -324,46 +322,55 @@ __global__ __launch_bounds__(VK_NT / RH) __attribute__((amdgpu_waves_per_eu(
         for (int t = 0; t < SL; ++t) {
           double d;
           if constexpr (TILE_IN)
-            d = Dj_tile[tile_off<SL, RH>(r, t, tid)];
+            d = Dj_tile[tile_off<SL, NT>(r, t, tid)];
           else
-            d = nat_elem2(Dj_nat, ni, row_of<RH>(r, tr), t * VK_G + tc, transpose);
+            d = nat_elem2(Dj_nat, ni, row_of<NT>(r, tr), t * VK_G + tc, transpose);
           a[r][t] = d - (cv[r] * bv[t]) * a[r][t];
         }
         __builtin_amdgcn_sched_barrier(0);
       }
     }
-    bad |= gj2_invert<SL, RH>(a, ni, sh);
+    bad |= gj2_invert<SL, NT>(a, ni, sh);
     double* dst = inv + col.off_b + (int64_t)j * BLK;
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
-      for (int t = 0; t < SL; ++t) dst[tile_off<SL, RH>(r, t, tid)] = a[r][t];
+      for (int t = 0; t < SL; ++t) dst[tile_off<SL, NT>(r, t, tid)] = a[r][t];
   }
   if (bad && tid == 0 && status) atomicOr(&status[c], VK_ST_SINGULAR);
 }
 
-int vk_launch_bt_factor2(int sl, int rh, const VkCol* cols, int ncols, const double* diag,
+int vk_launch_bt_factor2(int sl, int nt, const VkCol* cols, int ncols, const double* diag,
                          int diag_tile, int transpose, const double* sup_d, const double* sub_d,
                          double* inv, const double* rhs, double* rmod, int* status,
                          const int* active, hipStream_t stream) {
-#define VK_LAUNCH2(n, T, H)                                                                   \
-  hipLaunchKernelGGL((k_bt_factor2<n, T, H>), dim3(ncols), dim3(VK_NT / H), 0, stream, cols, \
+#define VK_LAUNCH2(n, T, W)                                                             \
+  hipLaunchKernelGGL((k_bt_factor2<n, T, W>), dim3(ncols), dim3(W), 0, stream, cols,    \
                      ncols, diag, transpose, sup_d, sub_d, inv, rhs, rmod, status, active)
-#define VK_CASE2(n)                                                    \
-  case n:                                                              \
-    if (rh == 2) {                                                     \
-      if (diag_tile) VK_LAUNCH2(n, true, 2); else VK_LAUNCH2(n, false, 2); \
-    } else {                                                           \
-      if (diag_tile) VK_LAUNCH2(n, true, 4); else VK_LAUNCH2(n, false, 4); \
-    }                                                                  \
-    break;
-  if (rh != 2 && rh != 4) return VK_ERR_ARG;
+#define VK_NT2(n, W)                                                  \
+  if (nt == W) {                                                      \
+    if (diag_tile) VK_LAUNCH2(n, true, W); else VK_LAUNCH2(n, false, W); \
+    break;                                                            \
+  }
+#define VK_CASE2(n)  \
+  case n:            \
+    VK_NT2(n, 64)    \
+    VK_NT2(n, 128)   \
+    return VK_ERR_ARG;
+#define VK_CASE2E(n) \
+  case n:            \
+    VK_NT2(n, 64)    \
+    VK_NT2(n, 128)   \
+    VK_NT2(n, 512)   \
+    return VK_ERR_ARG;
   switch (sl) {
-    VK_CASE2(1) VK_CASE2(2) VK_CASE2(3) VK_CASE2(4) VK_CASE2(5) VK_CASE2(6) VK_CASE2(7)
+    VK_CASE2(1) VK_CASE2E(2) VK_CASE2(3) VK_CASE2E(4) VK_CASE2(5) VK_CASE2E(6) VK_CASE2(7)
     default:
       return VK_ERR_UNSUPPORTED;
   }
 #undef VK_CASE2
+#undef VK_CASE2E
+#undef VK_NT2
 #undef VK_LAUNCH2
   VK_CHECK_HIP(hipGetLastError());
   return VK_OK;

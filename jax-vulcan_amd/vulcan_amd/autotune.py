@@ -5,11 +5,11 @@ block-factor kernels the library offers for that bucket (``VK_FACTOR_TILE``:
 one 256-thread workgroup per column with 16x16 register tiles;
 ``VK_FACTOR_ROWS``: half a row per lane, 64-192 lanes per column;
 ``VK_FACTOR_BLOCKED``: 16-column panels with FP64 MFMA trailing updates;
-``VK_FACTOR_TILE2`` / ``VK_FACTOR_WAVE``: the tile kernel on two waves / one
-wave per column) on
+``VK_FACTOR_TILE2`` / ``VK_FACTOR_WAVE`` / ``VK_FACTOR_TILE8``: the tile
+kernel on two / one / eight waves per column) on
 the batch's own columns of that bucket, and keep the fastest
-(``vk_tune_set``).  All take the same pivots; TILE, ROWS, TILE2 and WAVE
-give bitwise-equal inverses, BLOCKED differs in the rounding of its rank-16
+(``vk_tune_set``).  All take the same pivots; TILE, ROWS, TILE2, WAVE and
+TILE8 give bitwise-equal inverses, BLOCKED differs in the rounding of its rank-16
 products.  Decisions are cached per
 (SL, nz rounded up to 32, column count rounded up to a power of two).
 
@@ -30,7 +30,7 @@ from . import _lib
 from .core import F64, Batch, device, device_network
 
 VK_TUNE_FACTOR = 0
-VARIANTS = {0: "tile", 1: "rows", 2: "blocked", 3: "tile2", 4: "wave"}
+VARIANTS = {0: "tile", 1: "rows", 2: "blocked", 3: "tile2", 4: "wave", 5: "tile8"}
 _CACHE: Dict[Tuple[int, int, int], int] = {}
 
 
@@ -88,7 +88,7 @@ def tune_columns(columns: Sequence, reps: int = 2, verbose: bool = False) -> Dic
         atm = b.pack_atm([c.atm for c in cols], dev)
         dt = torch.tensor([float(c.dt) for c in cols], dtype=F64, device=dev)
         times = {}
-        for v in (0, 1, 2, 3, 4):
+        for v in (0, 1, 2, 3, 4, 5):
             if lib.vk_tune_set(VK_TUNE_FACTOR, sl, v) != 0:
                 continue  # variant not offered for this bucket
             times[v] = _time_step(b, y, k, atm, dt, reps)

@@ -246,8 +246,9 @@ def test_integrate_fixed_dt_matches_oracle():
 
 @pytest.mark.parametrize("ni", [5, 21, 40, 50, 64, 69, 93])
 def test_factor_variants_agree(ni):
-    """The row-per-lane, tile, two-wave and one-wave tile block factors take
-    the same pivots and do the same per-element arithmetic: bitwise-equal inverse
+    """The row-per-lane, tile, two-wave, one-wave and eight-wave (even SL)
+    tile block factors take the same pivots and do the same per-element
+    arithmetic: bitwise-equal inverse
     stacks (tile-layout and natural inputs, plain and transposed); the fused
     sweep differs only in summation order."""
     import torch
@@ -263,7 +264,8 @@ def test_factor_variants_agree(ni):
     sl = (ni + 15) // 16
     out = {}
     try:
-        for variant in (0, 1, 3, 4):
+        variants = (0, 1, 3, 4, 5) if sl % 2 == 0 else (0, 1, 3, 4)
+        for variant in variants:
             _lib.check(lib.vk_tune_set(0, sl, variant), "vk_tune_set")
             for tr in (False, True):
                 f = solver.factor_block_thomas_diag_offdiag(diag, sup, sub, transpose=tr)
@@ -273,7 +275,7 @@ def test_factor_variants_agree(ni):
         lib.vk_tune_set(0, sl, 0)
     for tr in (False, True):
         inv0, x0, s0 = out[(0, tr)]
-        for v in (1, 3, 4):
+        for v in variants[1:]:
             inv1, x1, s1 = out[(v, tr)]
             print(f"ni={ni} transpose={tr} variant {v}: max |inv diff| "
                   f"{float((inv0 - inv1).abs().max()):.3e}")
@@ -290,22 +292,23 @@ def test_factor_variants_agree(ni):
 
 
 def test_ros2_step_factor_variants_agree(cfg1):
-    """Whole Ros2 step with the tile, row-per-lane, two-wave or one-wave tile
-    block factor, well-conditioned dt: 1e-12 (the fused sweeps' sums differ
-    only in order)."""
+    """Whole Ros2 step with the tile, row-per-lane, two-wave, one-wave or
+    eight-wave (even SL) tile block factor, well-conditioned dt: 1e-12 (the
+    fused sweeps' sums differ only in order)."""
     from vulcan_amd import _lib, jax_step
     lib = _lib.load()
     c = cfg1
     sl = (c.ni + 15) // 16
     sols = []
     try:
-        for variant in (0, 1, 3, 4):
+        variants = (0, 1, 3, 4, 5) if sl % 2 == 0 else (0, 1, 3, 4)
+        for variant in variants:
             _lib.check(lib.vk_tune_set(0, sl, variant), "vk_tune_set")
             sol, _ = jax_step.jax_ros2_step(c.y, c.k, 1e-18, c.atm, c.net)
             sols.append(_np(sol))
     finally:
         lib.vk_tune_set(0, sl, 0)
-    for v in (1, 2, 3):
+    for v in range(1, len(sols)):
         assert _layer_err(sols[v], sols[0]).max() < 1e-12
 
 
