@@ -251,6 +251,62 @@ typedef struct {
 int vk_photo_f64(vk_batch b, int parts, const vk_photo_setup* setup, const vk_photo_state* state,
                  const double* y, double* k, void* stream);
 
+/* ---- rate construction k(T, M) ----
+ * Restates the reference's host-side rate build on the device:
+ * rates.compute_forward_k (rates.py:61-122), rates.apply_lowT_caps
+ * (rates.py:138-167), gibbs.gibbs_sp_vector (gibbs.py:41-78),
+ * gibbs.K_eq_array (gibbs.py:81-126), gibbs.fill_reverse_k (gibbs.py:129-152)
+ * and rates.build_rate_array (rates.py:185-207).
+ *
+ * vk_rates_create takes the network's rows classified once on the host:
+ *   form[i]    VK_KF_* forward form of row i (VK_KF_ZERO for reverse rows,
+ *              photo/ion/condensation/radiative rows and rows the reference
+ *              leaves at zero);
+ *   lowt[i]    VK_LOWT_* low-T cap of forward row i;
+ *   has_rev[i] forward row i has a thermal reverse at i+1 (K_eq computed);
+ *   par        (nr+1, 6): a, n, E, a_inf, n_inf, E_inf;
+ *   tables     (nr+1, T) reactant/product species (PAD = ni) and stoich;
+ *   nasa9      (ni, 2, 10) NASA-9 coefficients, low-T range first (NULL:
+ *              no Gibbs / K_eq / build);
+ *   zero_rows  (nr+1) rows the build and the reverse fill zero (the remove
+ *              list, paired or not as the caller resolved it), or NULL.
+ * vk_rates_eval_f64 evaluates one column on nz layers, out (rows, nz):
+ *   VK_RATES_FORWARD  compute_forward_k (+ caps when lowt_on);
+ *   VK_RATES_CAPS     apply_lowT_caps of in_a (rows, nz);
+ *   VK_RATES_GIBBS    gibbs_sp_vector, out (ni, nz);
+ *   VK_RATES_KEQ      K_eq_array from g/RT in in_a (ni, nz);
+ *   VK_RATES_FILL     fill_reverse_k of k in_a and K_eq in_b;
+ *   VK_RATES_BUILD    forward (+ caps), reverse fill from NASA-9, zero rows.
+ * vk_batch_set_rates gives a plain batch one rates handle per network (in
+ * the batch's network order); vk_rates_build_batch_f64 then writes every
+ * column's k (S, nz) at off_k from the T and M of its atmosphere record. */
+typedef struct vk_rates_s* vk_rates;
+#define VK_KF_ZERO 0
+#define VK_KF_ARRHENIUS 1
+#define VK_KF_LINDEMANN 2
+#define VK_KF_TROE_OH_CH3 3
+#define VK_LOWT_NONE 0
+#define VK_LOWT_CH3 1  /* H + CH3 + M -> CH4 + M below 277.5 K */
+#define VK_LOWT_C2H4 2 /* H + C2H4 + M -> C2H5 + M below 300 K */
+#define VK_LOWT_C2H5 3 /* H + C2H5 + M -> C2H6 + M below 200 K */
+#define VK_RATES_FORWARD 0
+#define VK_RATES_CAPS 1
+#define VK_RATES_GIBBS 2
+#define VK_RATES_KEQ 3
+#define VK_RATES_FILL 4
+#define VK_RATES_BUILD 5
+int vk_rates_create(int nr, int ni, int T, int stop_rev_indx, const int32_t* form,
+                    const int32_t* lowt, const int32_t* has_rev, const double* par,
+                    const int64_t* reactant_idx, const double* reactant_stoich,
+                    const int64_t* product_idx, const double* product_stoich,
+                    const double* nasa9, const uint8_t* zero_rows, vk_rates* out);
+int vk_rates_destroy(vk_rates r);
+int vk_rates_eval_f64(vk_rates r, int what, int nz, const double* T, const double* M,
+                      int lowt_on, const double* in_a, const double* in_b, double* out,
+                      void* stream);
+int vk_batch_set_rates(vk_batch b, const vk_rates* per_net, int n);
+int vk_rates_build_batch_f64(vk_batch b, int lowt_on, const double* atm, double* k, void* stream);
+
 /* Kernel timing with HIP events recorded on the launch stream around each
  * kernel of vk_ros2_step_f64 (no counterpart in the reference, which has no
  * device timing).  Kernel ids: 0 chemistry+LHS, 1 block factor, 2 stage-1

@@ -136,6 +136,16 @@ struct vk_batch_s {
   std::vector<hipStream_t> aux;
   hipEvent_t ev_fork = nullptr;
   std::vector<hipEvent_t> ev_join;
+  // networks: (ni, nr) per batch network, and the rates tables bound to
+  // them by vk_batch_set_rates
+  std::vector<std::pair<int, int>> net_dims;
+  VkRatesDev* d_rates = nullptr;
+  int n_rates = 0;
+};
+
+struct vk_rates_s {
+  VkRatesDev dev{};
+  void* buf = nullptr;
 };
 
 namespace {
@@ -496,6 +506,9 @@ int vk_batch_create(int ncols, const int32_t* nz, const int32_t* ni, const int32
   b->slots.assign(sls.begin(), sls.end());
   std::vector<VkNetDev> hn(nnets);
   for (int i = 0; i < nnets; ++i) hn[i] = nets[i] ? nets[i]->dev : VkNetDev{};
+  b->net_dims.resize(nnets, {-1, -1});
+  for (int i = 0; i < nnets; ++i)
+    if (nets[i]) b->net_dims[i] = {nets[i]->ni, nets[i]->nr};
   if (hipMalloc(&b->d_cols, sizeof(VkCol) * ncols) != hipSuccess ||
       hipMalloc(&b->d_nets, sizeof(VkNetDev) * nnets) != hipSuccess) {
     vk_batch_destroy(b);
@@ -615,6 +628,7 @@ int vk_batch_destroy(vk_batch b) {
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   if (b->d_cols) (void)hipFree(b->d_cols);
   if (b->d_nets) (void)hipFree(b->d_nets);
+  if (b->d_rates) (void)hipFree(b->d_rates);
   delete b;
   return VK_OK;
 }
@@ -1034,3 +1048,169 @@ int vk_tune_get(int knob, int sl) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// rate construction (vk_rates.hip)
+// ---------------------------------------------------------------------------
+
+int vk_launch_rates_eval(const VkRatesDev& R, int what, int nz, const double* T, const double* M,
+                         int lowt_on, const double* in_a, const double* in_b, double* out,
+                         hipStream_t stream);
+int vk_launch_rates_build_batch(const VkCol* cols, int ncols, int max_S, int max_nz,
+                                const VkRatesDev* rates, const double* atm, int lowt_on,
+                                double* k, hipStream_t stream);
+
+int vk_rates_create(int nr, int ni, int T, int stop_rev_indx, const int32_t* form,
+                    const int32_t* lowt, const int32_t* has_rev, const double* par,
+                    const int64_t* reactant_idx, const double* reactant_stoich,
+                    const int64_t* product_idx, const double* product_stoich,
+                    const double* nasa9, const uint8_t* zero_rows, vk_rates* out) {
+  g_err[0] = 0;
+  if (!out || nr < 0 || ni < 1 || T < 1 || T > 4 || (nr > 0 && (!form || !lowt || !has_rev ||
+      !par || !reactant_idx || !reactant_stoich || !product_idx || !product_stoich))) {
+    vk_set_error("vk_rates_create: bad arguments (nr=%d ni=%d T=%d)", nr, ni, T);
+    return VK_ERR_ARG;
+  }
+  const int S = nr + 1;
+  std::vector<int> f(S, 0), lw(S, 0), hr(S, 0), ri(S * T, ni), pi(S * T, ni);
+  std::vector<double> rs(S * T, 0.0), ps(S * T, 0.0);
+  if (nr > 0) {
+    for (int i = 0; i < S; ++i) {
+      if (form[i] < VK_KF_ZERO || form[i] > VK_KF_TROE_OH_CH3 || lowt[i] < VK_LOWT_NONE ||
+          lowt[i] > VK_LOWT_C2H5) {
+        vk_set_error("vk_rates_create: row %d has form %d / cap %d", i, form[i], lowt[i]);
+        return VK_ERR_ARG;
+      }
+      f[i] = form[i];
+      lw[i] = lowt[i];
+      hr[i] = has_rev[i] ? 1 : 0;
+    }
+    for (int e = 0; e < S * T; ++e) {
+      if (reactant_idx[e] < 0 || reactant_idx[e] > ni || product_idx[e] < 0 ||
+          product_idx[e] > ni) {
+        vk_set_error("vk_rates_create: species index out of range at entry %d", e);
+        return VK_ERR_ARG;
+      }
+      ri[e] = (int)reactant_idx[e];
+      pi[e] = (int)product_idx[e];
+      rs[e] = reactant_stoich[e];
+      ps[e] = product_stoich[e];
+    }
+  }
+  size_t off = 0;
+  auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+  const size_t o_f = take(S * 4), o_l = take(S * 4), o_h = take(S * 4), o_p = take(S * 6 * 8),
+               o_ri = take(S * T * 4), o_rs = take(S * T * 8), o_pi = take(S * T * 4),
+               o_ps = take(S * T * 8), o_n = nasa9 ? take((size_t)ni * 20 * 8) : 0,
+               o_z = zero_rows ? take(S) : 0;
+  std::vector<char> host(off, 0);
+  auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) memcpy(&host[o], src, bytes); };
+  put(o_f, f.data(), S * 4);
+  put(o_l, lw.data(), S * 4);
+  put(o_h, hr.data(), S * 4);
+  if (nr > 0) put(o_p, par, S * 6 * 8);
+  put(o_ri, ri.data(), S * T * 4);
+  put(o_rs, rs.data(), S * T * 8);
+  put(o_pi, pi.data(), S * T * 4);
+  put(o_ps, ps.data(), S * T * 8);
+  if (nasa9) put(o_n, nasa9, (size_t)ni * 20 * 8);
+  if (zero_rows) put(o_z, zero_rows, S);
+  vk_rates_s* r = new vk_rates_s();
+  if (hipMalloc(&r->buf, off) != hipSuccess ||
+      hipMemcpy(r->buf, host.data(), off, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    if (r->buf) (void)hipFree(r->buf);
+    delete r;
+    vk_set_error("vk_rates_create: device allocation or upload of %zu bytes failed", off);
+    return VK_ERR_NOMEM;
+  }
+  char* b = (char*)r->buf;
+  VkRatesDev& d = r->dev;
+  d.nr = nr; d.ni = ni; d.T = T; d.stop_rev = stop_rev_indx;
+  d.form = (const int*)(b + o_f);
+  d.lowt = (const int*)(b + o_l);
+  d.has_rev = (const int*)(b + o_h);
+  d.par = (const double*)(b + o_p);
+  d.r_idx = (const int*)(b + o_ri);
+  d.r_st = (const double*)(b + o_rs);
+  d.p_idx = (const int*)(b + o_pi);
+  d.p_st = (const double*)(b + o_ps);
+  d.nasa9 = nasa9 ? (const double*)(b + o_n) : nullptr;
+  d.zero_rows = zero_rows ? (const uint8_t*)(b + o_z) : nullptr;
+  *out = r;
+  return VK_OK;
+}
+
+int vk_rates_destroy(vk_rates r) {
+  if (!r) return VK_OK;
+  if (r->buf) (void)hipFree(r->buf);
+  delete r;
+  return VK_OK;
+}
+
+int vk_rates_eval_f64(vk_rates r, int what, int nz, const double* T, const double* M,
+                      int lowt_on, const double* in_a, const double* in_b, double* out,
+                      void* stream) {
+  g_err[0] = 0;
+  if (!r || nz < 1 || !T || !out || what < VK_RATES_FORWARD || what > VK_RATES_BUILD) {
+    vk_set_error("vk_rates_eval_f64: bad arguments (what=%d nz=%d)", what, nz);
+    return VK_ERR_ARG;
+  }
+  const bool needs_m = (what == VK_RATES_FORWARD || what == VK_RATES_CAPS ||
+                        what == VK_RATES_BUILD);
+  const bool needs_nasa = (what == VK_RATES_GIBBS || what == VK_RATES_BUILD);
+  if ((needs_m && !M) || (needs_nasa && !r->dev.nasa9) ||
+      ((what == VK_RATES_CAPS || what == VK_RATES_KEQ || what == VK_RATES_FILL) && !in_a) ||
+      (what == VK_RATES_FILL && !in_b)) {
+    vk_set_error("vk_rates_eval_f64: mode %d needs %s", what,
+                 needs_nasa && !r->dev.nasa9 ? "NASA-9 coefficients" : "an input it was not given");
+    return VK_ERR_ARG;
+  }
+  return vk_launch_rates_eval(r->dev, what, nz, T, M, lowt_on, in_a, in_b, out,
+                              (hipStream_t)stream);
+}
+
+int vk_batch_set_rates(vk_batch b, const vk_rates* per_net, int n) {
+  if (int rc = check_plain(b, "vk_batch_set_rates")) return rc;
+  if (!per_net || n != (int)b->net_dims.size()) {
+    vk_set_error("vk_batch_set_rates: need one rates handle per batch network (%d), got %d",
+                 (int)b->net_dims.size(), n);
+    return VK_ERR_ARG;
+  }
+  std::vector<VkRatesDev> h(n);
+  for (int i = 0; i < n; ++i) {
+    if (!per_net[i] || !per_net[i]->dev.nasa9 || per_net[i]->dev.ni != b->net_dims[i].first ||
+        per_net[i]->dev.nr != b->net_dims[i].second) {
+      vk_set_error("vk_batch_set_rates: handle %d does not match network %d (ni=%d nr=%d) or "
+                   "has no NASA-9 coefficients", i, i, b->net_dims[i].first,
+                   b->net_dims[i].second);
+      return VK_ERR_ARG;
+    }
+    h[i] = per_net[i]->dev;
+  }
+  if (b->n_rates != n) {
+    if (b->d_rates) (void)hipFree(b->d_rates);
+    b->d_rates = nullptr;
+    b->n_rates = 0;
+    if (hipMalloc(&b->d_rates, sizeof(VkRatesDev) * n) != hipSuccess) {
+      (void)hipGetLastError();
+      vk_set_error("vk_batch_set_rates: device allocation failed");
+      return VK_ERR_NOMEM;
+    }
+    b->n_rates = n;
+  }
+  VK_CHECK_HIP(hipMemcpy(b->d_rates, h.data(), sizeof(VkRatesDev) * n, hipMemcpyHostToDevice));
+  return VK_OK;
+}
+
+int vk_rates_build_batch_f64(vk_batch b, int lowt_on, const double* atm, double* k,
+                             void* stream) {
+  if (int rc = check_plain(b, "vk_rates_build_batch_f64")) return rc;
+  if (!atm || !k || !b->d_rates) {
+    vk_set_error("vk_rates_build_batch_f64: %s", !b->d_rates ? "no rates bound (vk_batch_set_rates)"
+                                                               : "null pointer");
+    return VK_ERR_ARG;
+  }
+  return vk_launch_rates_build_batch(b->d_cols, b->ncols, b->max_S, b->max_nz, b->d_rates, atm,
+                                     lowt_on, k, (hipStream_t)stream);
+}

@@ -26,6 +26,38 @@ class NetworkArrays(NamedTuple):
     is_three_body: np.ndarray    # (nr+1,) bool
 
 
+class RateNetwork(NamedTuple):
+    """What the rate build reads of network.Network (network.py:44-95).
+
+    Per-row flags and Arrhenius columns are (nr+1,) arrays indexed like the
+    reaction tables; ``Rf`` maps a forward row to its equation text.  Any
+    object with these attributes (the reference's ``Network`` included) is
+    accepted where a ``RateNetwork`` is expected.
+    """
+    species: tuple
+    ni: int
+    nr: int
+    reactant_idx: np.ndarray     # (nr+1, T) int64, PAD = ni
+    product_idx: np.ndarray      # (nr+1, T) int64
+    reactant_stoich: np.ndarray  # (nr+1, T) float64
+    product_stoich: np.ndarray   # (nr+1, T) float64
+    a: np.ndarray
+    n: np.ndarray
+    E: np.ndarray
+    a_inf: np.ndarray
+    n_inf: np.ndarray
+    E_inf: np.ndarray
+    is_forward: np.ndarray
+    has_kinf: np.ndarray
+    is_special: np.ndarray
+    is_photo: np.ndarray
+    is_ion: np.ndarray
+    is_conden: np.ndarray
+    is_radiative: np.ndarray
+    stop_rev_indx: int
+    Rf: dict
+
+
 class AtmStatic(NamedTuple):
     """Per-column atmosphere held fixed through a step."""
     Kzz: np.ndarray       # (nz-1,)

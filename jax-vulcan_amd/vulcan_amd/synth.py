@@ -29,7 +29,7 @@ from typing import Optional
 
 import numpy as np
 
-from ._types import AtmStatic, NetworkArrays
+from ._types import AtmStatic, NetworkArrays, RateNetwork
 
 # Physical constants (phy_const.py:3-9).
 KB = 1.38064852e-16
@@ -240,6 +240,94 @@ def random_rates(rng, Tco: np.ndarray, n_rev: int, n_photo: int = 0, net=None,
                  consistent: bool = False):
     """k (nr+1, nz) with freshly drawn parameters."""
     return rates_from_params(random_rate_params(rng, n_rev, net, consistent), Tco, n_photo)
+
+
+# --------------------------------------------------------------------------
+# rate-construction networks
+# --------------------------------------------------------------------------
+
+SPECIAL_OH_CH3 = "OH + CH3 + M -> CH3OH + M"
+LOWT_EQUATIONS = ("H + CH3 + M -> CH4 + M", "H + C2H4 + M -> C2H5 + M",
+                  "H + C2H5 + M -> C2H6 + M")
+
+
+def random_rate_network(rng, net: NetworkArrays, n_rev: int, n_photo: int = 0,
+                        f_lindemann: float = 0.2) -> RateNetwork:
+    """Rate parameters and row flags over the reaction tables of ``net``.
+
+    Forward rows draw modified Arrhenius parameters, a fraction
+    ``f_lindemann`` a Lindemann pair (k0, k_inf); among the first rows sit
+    the hard-coded Troe reaction, a special row with and one without
+    Arrhenius columns, the three low-T cap reactions, one radiative and one
+    condensation row.  Photolysis rows follow the reversible block, which
+    ends at stop_rev_indx."""
+    nr = net.nr
+    S = nr + 1
+    z = lambda: np.zeros(S)
+    b = lambda: np.zeros(S, dtype=bool)
+    a, n, E, a_inf, n_inf, E_inf = z(), z(), z(), z(), z(), z()
+    is_forward, has_kinf, is_special = b(), b(), b()
+    is_photo, is_ion, is_conden, is_radiative = b(), b(), b(), b()
+    Rf = {}
+    for q in range(n_rev):
+        i = 2 * q + 1
+        is_forward[i] = True
+        Rf[i] = f"R{q} forward"
+        if rng.uniform() < f_lindemann:
+            has_kinf[i] = has_kinf[i + 1] = True
+            a[i] = 10.0 ** rng.uniform(-33.0, -28.0)
+            n[i] = rng.uniform(-3.0, 0.5)
+            E[i] = rng.uniform(0.0, 3.0e3)
+            a_inf[i] = 10.0 ** rng.uniform(-12.0, -9.0)
+            n_inf[i] = rng.uniform(-1.0, 1.0)
+            E_inf[i] = rng.uniform(0.0, 2.0e3)
+        else:
+            a[i] = 10.0 ** rng.uniform(-14.0, -9.0)
+            n[i] = rng.uniform(-2.0, 2.0)
+            E[i] = rng.uniform(0.0, 2.0e4)
+    special = [(1, SPECIAL_OH_CH3, 0.0), (3, "X + Y -> Z", 2.5e-11), (5, "X + Y -> W", 0.0)]
+    for i, eq, av in special:
+        if i < 2 * n_rev:
+            is_special[i] = is_special[i + 1] = True
+            has_kinf[i] = has_kinf[i + 1] = False
+            Rf[i] = "  " + eq + " "
+            a[i], n[i], E[i] = av, 0.3 if av else 0.0, 120.0 if av else 0.0
+    for j, eq in enumerate(LOWT_EQUATIONS):
+        i = 7 + 2 * j
+        if i < 2 * n_rev:
+            Rf[i] = eq
+    if 15 < 2 * n_rev:
+        is_radiative[13] = is_radiative[14] = True
+        is_conden[15] = is_conden[16] = True
+    for r in range(n_photo):
+        i = 2 * n_rev + 1 + r
+        is_photo[i] = True
+        is_forward[i] = (i % 2 == 1)
+    return RateNetwork(
+        species=tuple(f"S{j}" for j in range(net.ni)), ni=net.ni, nr=nr,
+        reactant_idx=net.reactant_idx, product_idx=net.product_idx,
+        reactant_stoich=net.reactant_stoich, product_stoich=net.product_stoich,
+        a=a, n=n, E=E, a_inf=a_inf, n_inf=n_inf, E_inf=E_inf, is_forward=is_forward,
+        has_kinf=has_kinf, is_special=is_special, is_photo=is_photo, is_ion=is_ion,
+        is_conden=is_conden, is_radiative=is_radiative,
+        stop_rev_indx=2 * n_rev + 1 if n_photo else nr + 1, Rf=Rf)
+
+
+def random_nasa9(rng, ni: int) -> np.ndarray:
+    """(ni, 2, 10) NASA-9 coefficients of plausible magnitude (low-T range
+    first); a7 unused, a8 and a9 the enthalpy and entropy constants."""
+    c = np.zeros((ni, 2, 10))
+    for r, s3 in ((0, 5e-3), (1, 1e-3)):
+        c[:, r, 0] = rng.uniform(-5e4, 5e4, ni)
+        c[:, r, 1] = rng.uniform(-1e3, 1e3, ni)
+        c[:, r, 2] = rng.uniform(2.0, 8.0, ni)
+        c[:, r, 3] = rng.uniform(-s3, s3, ni)
+        c[:, r, 4] = rng.uniform(-1e-5, 1e-5, ni)
+        c[:, r, 5] = rng.uniform(-1e-8, 1e-8, ni)
+        c[:, r, 6] = rng.uniform(-1e-12, 1e-12, ni)
+        c[:, r, 8] = rng.uniform(-6e4, 6e4, ni)
+        c[:, r, 9] = rng.uniform(-20.0, 30.0, ni)
+    return c
 
 
 # --------------------------------------------------------------------------

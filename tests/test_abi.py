@@ -45,3 +45,36 @@ def test_atm_record_layout_matches_python():
     nz, ni = 10, 7
     assert off[0] == 0 and off[5] == 5 * (nz - 1)
     assert total == 5 * (nz - 1) + 3 * nz + 6 * ni + 2 * (nz - 1) * ni + nz * ni + 4
+
+
+def test_rates_create_rejects_bad_tables_without_gpu():
+    """vk_rates_create validates its tables on the host before any device
+    call: an unknown form code or an out-of-range species index is a bad
+    argument."""
+    import ctypes as C
+    import numpy as np
+    from vulcan_amd import _lib
+    lib = _lib.load()
+    nr, ni, T = 2, 3, 2
+    form = np.array([0, 1, 0], np.int32)
+    z = np.zeros(nr + 1, np.int32)
+    par = np.zeros((nr + 1, 6))
+    idx = np.full((nr + 1, T), ni, np.int64)
+    st = np.zeros((nr + 1, T))
+    h = C.c_void_p()
+
+    def create(form, idx):
+        return lib.vk_rates_create(nr, ni, T, nr + 1, form.ctypes.data, z.ctypes.data,
+                                   z.ctypes.data, par.ctypes.data, idx.ctypes.data,
+                                   st.ctypes.data, idx.ctypes.data, st.ctypes.data, None, None,
+                                   C.byref(h))
+
+    bad_form = form.copy()
+    bad_form[1] = 7
+    assert create(bad_form, idx) == 1
+    assert b"form" in lib.vk_last_error()
+    bad_idx = idx.copy()
+    bad_idx[1, 0] = ni + 1
+    assert create(form, bad_idx) == 1
+    assert lib.vk_rates_create(nr, 0, T, 1, None, None, None, None, None, None, None, None,
+                               None, None, C.byref(h)) == 1
