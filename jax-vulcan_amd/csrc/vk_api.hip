@@ -1056,7 +1056,7 @@ int vk_tune_get(int knob, int sl) {
 int vk_launch_rates_eval(const VkRatesDev& R, int what, int nz, const double* T, const double* M,
                          int lowt_on, const double* in_a, const double* in_b, double* out,
                          hipStream_t stream);
-int vk_launch_rates_build_batch(const VkCol* cols, int ncols, int max_S, int max_nz,
+int vk_launch_rates_build_batch(const VkCol* cols, int ncols, int max_ni, int max_nz,
                                 const VkRatesDev* rates, const double* atm, int lowt_on,
                                 double* k, hipStream_t stream);
 
@@ -1066,55 +1066,66 @@ int vk_rates_create(int nr, int ni, int T, int stop_rev_indx, const int32_t* for
                     const int64_t* product_idx, const double* product_stoich,
                     const double* nasa9, const uint8_t* zero_rows, vk_rates* out) {
   g_err[0] = 0;
-  if (!out || nr < 0 || ni < 1 || T < 1 || T > 4 || (nr > 0 && (!form || !lowt || !has_rev ||
-      !par || !reactant_idx || !reactant_stoich || !product_idx || !product_stoich))) {
+  if (!out || nr < 0 || ni < 1 || ni > 255 || T < 1 || T > 4 ||
+      (nr > 0 && (!form || !lowt || !has_rev || !par || !reactant_idx || !reactant_stoich ||
+                  !product_idx || !product_stoich))) {
     vk_set_error("vk_rates_create: bad arguments (nr=%d ni=%d T=%d)", nr, ni, T);
     return VK_ERR_ARG;
   }
-  const int S = nr + 1;
-  std::vector<int> f(S, 0), lw(S, 0), hr(S, 0), ri(S * T, ni), pi(S * T, ni);
-  std::vector<double> rs(S * T, 0.0), ps(S * T, 0.0);
-  if (nr > 0) {
-    for (int i = 0; i < S; ++i) {
-      if (form[i] < VK_KF_ZERO || form[i] > VK_KF_TROE_OH_CH3 || lowt[i] < VK_LOWT_NONE ||
-          lowt[i] > VK_LOWT_C2H5) {
-        vk_set_error("vk_rates_create: row %d has form %d / cap %d", i, form[i], lowt[i]);
+  // Row i of the full build is held at zero: the host mask, row 0, rows past
+  // stop_rev_indx at odd offsets (gibbs.py:149-150); a reverse row also at
+  // or past stop_rev_indx (gibbs.py:141).
+  auto zero = [&](int i) {
+    if (zero_rows && zero_rows[i]) return true;
+    if (i == 0) return true;
+    return i >= stop_rev_indx + 1 && ((i - stop_rev_indx - 1) & 1) == 0;
+  };
+  const int npairs = (nr + 1) / 2;
+  std::vector<VkRatePair> pairs(std::max(npairs, 1));
+  for (int p = 0; p < npairs; ++p) {
+    const int i = 2 * p + 1;
+    VkRatePair& q = pairs[p];
+    memset(&q, 0, sizeof(q));
+    if (form[i] < VK_KF_ZERO || form[i] > VK_KF_TROE_OH_CH3 || lowt[i] < VK_LOWT_NONE ||
+        lowt[i] > VK_LOWT_C2H5) {
+      vk_set_error("vk_rates_create: row %d has form %d / cap %d", i, form[i], lowt[i]);
+      return VK_ERR_ARG;
+    }
+    q.form = form[i];
+    q.lowt = lowt[i];
+    for (int k = 0; k < 6; ++k) q.par[k] = par[6 * i + k];
+    double sr = 0.0, sp = 0.0;
+    for (int s = 0; s < 4; ++s) q.sp[s] = q.sp[4 + s] = (uint8_t)ni;
+    for (int s = 0; s < T; ++s) {
+      const int64_t e = (int64_t)i * T + s;
+      const int64_t ri = reactant_idx[e], pi = product_idx[e];
+      const double rs = reactant_stoich[e], ps = product_stoich[e];
+      if (ri < 0 || ri > ni || pi < 0 || pi > ni || (double)(float)rs != rs ||
+          (double)(float)ps != ps) {
+        vk_set_error("vk_rates_create: row %d slot %d: species index out of range or "
+                     "stoichiometry not exact in fp32", i, s);
         return VK_ERR_ARG;
       }
-      f[i] = form[i];
-      lw[i] = lowt[i];
-      hr[i] = has_rev[i] ? 1 : 0;
+      q.sp[s] = (uint8_t)ri;
+      q.st[s] = (float)rs;
+      q.sp[4 + s] = (uint8_t)pi;
+      q.st[4 + s] = (float)ps;
+      sr += rs;
     }
-    for (int e = 0; e < S * T; ++e) {
-      if (reactant_idx[e] < 0 || reactant_idx[e] > ni || product_idx[e] < 0 ||
-          product_idx[e] > ni) {
-        vk_set_error("vk_rates_create: species index out of range at entry %d", e);
-        return VK_ERR_ARG;
-      }
-      ri[e] = (int)reactant_idx[e];
-      pi[e] = (int)product_idx[e];
-      rs[e] = reactant_stoich[e];
-      ps[e] = product_stoich[e];
-    }
+    for (int s = 0; s < T; ++s) sp += product_stoich[(int64_t)i * T + s];
+    q.dn = sr - sp;
+    q.flags = (zero(i) ? VK_PAIR_ZERO_FWD : 0) |
+              ((i + 1 > nr || zero(i + 1) || i + 1 >= stop_rev_indx) ? VK_PAIR_ZERO_REV : 0) |
+              (has_rev[i] ? VK_PAIR_HAS_REV : 0);
   }
   size_t off = 0;
   auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
-  const size_t o_f = take(S * 4), o_l = take(S * 4), o_h = take(S * 4), o_p = take(S * 6 * 8),
-               o_ri = take(S * T * 4), o_rs = take(S * T * 8), o_pi = take(S * T * 4),
-               o_ps = take(S * T * 8), o_n = nasa9 ? take((size_t)ni * 20 * 8) : 0,
-               o_z = zero_rows ? take(S) : 0;
+  const size_t o_p = take(sizeof(VkRatePair) * pairs.size()),
+               o_n = nasa9 ? take((size_t)ni * 20 * 8) : 0, o_z = zero_rows ? take(nr + 1) : 0;
   std::vector<char> host(off, 0);
-  auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) memcpy(&host[o], src, bytes); };
-  put(o_f, f.data(), S * 4);
-  put(o_l, lw.data(), S * 4);
-  put(o_h, hr.data(), S * 4);
-  if (nr > 0) put(o_p, par, S * 6 * 8);
-  put(o_ri, ri.data(), S * T * 4);
-  put(o_rs, rs.data(), S * T * 8);
-  put(o_pi, pi.data(), S * T * 4);
-  put(o_ps, ps.data(), S * T * 8);
-  if (nasa9) put(o_n, nasa9, (size_t)ni * 20 * 8);
-  if (zero_rows) put(o_z, zero_rows, S);
+  memcpy(&host[o_p], pairs.data(), sizeof(VkRatePair) * pairs.size());
+  if (nasa9) memcpy(&host[o_n], nasa9, (size_t)ni * 20 * 8);
+  if (zero_rows) memcpy(&host[o_z], zero_rows, nr + 1);
   vk_rates_s* r = new vk_rates_s();
   if (hipMalloc(&r->buf, off) != hipSuccess ||
       hipMemcpy(r->buf, host.data(), off, hipMemcpyHostToDevice) != hipSuccess) {
@@ -1126,15 +1137,11 @@ int vk_rates_create(int nr, int ni, int T, int stop_rev_indx, const int32_t* for
   }
   char* b = (char*)r->buf;
   VkRatesDev& d = r->dev;
-  d.nr = nr; d.ni = ni; d.T = T; d.stop_rev = stop_rev_indx;
-  d.form = (const int*)(b + o_f);
-  d.lowt = (const int*)(b + o_l);
-  d.has_rev = (const int*)(b + o_h);
-  d.par = (const double*)(b + o_p);
-  d.r_idx = (const int*)(b + o_ri);
-  d.r_st = (const double*)(b + o_rs);
-  d.p_idx = (const int*)(b + o_pi);
-  d.p_st = (const double*)(b + o_ps);
+  d.nr = nr;
+  d.ni = ni;
+  d.npairs = npairs;
+  d.stop_rev = stop_rev_indx;
+  d.pairs = (const VkRatePair*)(b + o_p);
   d.nasa9 = nasa9 ? (const double*)(b + o_n) : nullptr;
   d.zero_rows = zero_rows ? (const uint8_t*)(b + o_z) : nullptr;
   *out = r;
@@ -1211,6 +1218,6 @@ int vk_rates_build_batch_f64(vk_batch b, int lowt_on, const double* atm, double*
                                                                : "null pointer");
     return VK_ERR_ARG;
   }
-  return vk_launch_rates_build_batch(b->d_cols, b->ncols, b->max_S, b->max_nz, b->d_rates, atm,
+  return vk_launch_rates_build_batch(b->d_cols, b->ncols, b->max_ni, b->max_nz, b->d_rates, atm,
                                      lowt_on, k, (hipStream_t)stream);
 }

@@ -126,17 +126,30 @@ struct VkNetDev {
   const double* jac_c;   // signed out stoich
 };
 
+// Everything the rate build reads of one (forward, reverse) row pair: row
+// i = 2p + 1 and its reverse i + 1, in one 128-byte record so a wave fetches
+// a pair's tables in one go (vk_rates_create fills it).
+enum : int {
+  VK_PAIR_ZERO_FWD = 1,  // forward row held at zero in the full build
+  VK_PAIR_ZERO_REV = 2,  // reverse row held at zero (or absent) in the full build
+  VK_PAIR_HAS_REV = 4,   // K_eq is computed (else the sentinel 1)
+};
+struct alignas(16) VkRatePair {
+  double par[6];      // a, n, E, a_inf, n_inf, E_inf
+  double dn;          // sum of reactant minus sum of product stoichiometries
+  float st[8];        // reactant stoich of slots 0..3, product stoich at 4..7
+  uint8_t sp[8];      // species of those slots, PAD = ni
+  int32_t form;       // VK_KF_*
+  int32_t lowt;       // VK_LOWT_*
+  int32_t flags;      // VK_PAIR_*
+  int32_t pad[5];
+};
+static_assert(sizeof(VkRatePair) == 128, "VkRatePair is one 128-byte line");
+
 // Device copy of one network's rate-construction tables (vk_rates_create).
 struct VkRatesDev {
-  int nr, ni, T, stop_rev;
-  const int* form;          // (nr+1) VK_KF_*
-  const int* lowt;          // (nr+1) VK_LOWT_*
-  const int* has_rev;       // (nr+1) forward row with a thermal reverse
-  const double* par;        // (nr+1, 6) a, n, E, a_inf, n_inf, E_inf
-  const int* r_idx;         // (nr+1, T), PAD = ni
-  const double* r_st;
-  const int* p_idx;
-  const double* p_st;
+  int nr, ni, npairs, stop_rev;
+  const VkRatePair* pairs;  // (npairs)
   const double* nasa9;      // (ni, 2, 10) or null
   const uint8_t* zero_rows; // (nr+1) or null
 };

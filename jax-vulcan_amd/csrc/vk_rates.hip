@@ -2,21 +2,32 @@
 // NASA-9 Gibbs energies, equilibrium constants and the reverse fill.
 //
 // Restates rates.compute_forward_k (rates.py:61-122), rates.apply_lowT_caps
-// (rates.py:138-167), gibbs.gibbs_sp_vector (gibbs.py:41-78),
-// gibbs.K_eq_array (gibbs.py:81-126), gibbs.fill_reverse_k
-// (gibbs.py:129-152) and rates.build_rate_array (rates.py:185-207).  The
-// host classifies each row once (vk_rates_create): the forward form
-// (modified Arrhenius, Lindemann falloff, the hard-coded Troe reaction),
-// the low-T cap it takes, and whether its reverse is filled from K_eq.
-// Every output entry (row or species, layer) is one thread; a reverse row
-// recomputes its forward rate and K_eq itself, so a full build is one
-// launch with no intermediate arrays.
+// (rates.py:142-167), gibbs.gibbs_sp_vector (gibbs.py:41-74),
+// gibbs.K_eq_array (gibbs.py:77-128), gibbs.fill_reverse_k
+// (gibbs.py:131-154) and rates.build_rate_array (rates.py:185-208).  The
+// host classifies each (forward, reverse) row pair once into a 128-byte
+// record (vk_rates_create): the forward form (modified Arrhenius,
+// Lindemann falloff, the hard-coded Troe reaction), the low-T cap it
+// takes, whether its reverse is filled from K_eq, which of its rows the
+// full build holds at zero, and the stoichiometry K_eq reads.
+//
+// The separate stages (forward, caps, Gibbs, K_eq, fill) are one thread per
+// output entry.  The full build is one launch with no intermediate arrays
+// in HBM: a block stages the Gibbs energies of all species over a tile of
+// 64 layers in LDS; each wave then takes one row pair at a time, its 64
+// lanes on the 64 layers, so the pair's record is a wave-uniform fetch and
+// each row's 64 stores are contiguous; the forward rate is evaluated once
+// for both rows of the pair.
 #include "vk_common.h"
 
 namespace {
 
-__device__ inline double arrh(double a, double n, double E, double T) {
-  return a * pow(T, n) * exp(-E / T);
+// a T^n exp(-E/T) as a exp(n ln T - E/T) with ln T shared by the layer: one
+// exponential instead of pow and exp, within 3e-14 relative of the
+// two-function form for |E/T| <= 700 (rates.py:36-38).
+__device__ inline double arrh(double a, double n, double E, double T, double lnT) {
+#pragma clang fp contract(off)
+  return a * exp(n * lnT - E / T);
 }
 
 // OH + CH3 + M -> CH3OH + M (rates.py:41-58).
@@ -33,20 +44,20 @@ __device__ inline double troe_oh_ch3(double T, double M) {
   return k0 / (1.0 + k0 * M / kinf) * ff;
 }
 
-// Forward rate of row i (0 for rows the reference leaves at zero), with the
-// low-T cap applied when requested (rates.py:108-120, 149-165).
-__device__ inline double forward_rate(const VkRatesDev& R, int i, double T, double M,
+// Forward rate of a pair's forward row (0 for rows the reference leaves at
+// zero), with the low-T cap applied when requested (rates.py:108-120,
+// 149-165).
+__device__ inline double forward_rate(const VkRatePair& q, double T, double lnT, double M,
                                       int lowt_on) {
 #pragma clang fp contract(off)
-  const double* p = R.par + 6 * i;
   double k = 0.0;
-  switch (R.form[i]) {
+  switch (q.form) {
     case VK_KF_ARRHENIUS:
-      k = arrh(p[0], p[1], p[2], T);
+      k = arrh(q.par[0], q.par[1], q.par[2], T, lnT);
       break;
     case VK_KF_LINDEMANN: {
-      const double k0 = arrh(p[0], p[1], p[2], T);
-      const double kinf = arrh(p[3], p[4], p[5], T);
+      const double k0 = arrh(q.par[0], q.par[1], q.par[2], T, lnT);
+      const double kinf = arrh(q.par[3], q.par[4], q.par[5], T, lnT);
       k = k0 / (1.0 + k0 * M / kinf);
     } break;
     case VK_KF_TROE_OH_CH3:
@@ -55,15 +66,25 @@ __device__ inline double forward_rate(const VkRatesDev& R, int i, double T, doub
     default:
       break;
   }
-  return lowt_on ? vk_lowt_cap(R.lowt[i], k, T, M) : k;
+  return lowt_on ? vk_lowt_cap(q.lowt, k, T, M) : k;
+}
+
+// Powers of T shared by every species' Gibbs fit at one layer.
+struct TPow {
+  double T, T2, T3, T4, lT;
+};
+
+__device__ inline TPow tpow(double T) {
+#pragma clang fp contract(off)
+  return TPow{T, T * T, pow(T, 3.0), pow(T, 4.0), log(T)};
 }
 
 // g/(RT) of species j from its NASA-9 fit: H/RT - S/R, the low range below
 // 1000 K (gibbs.py:47-76).
-__device__ inline double gibbs_rt(const double* c, double T) {
+__device__ inline double gibbs_rt(const double* c, const TPow& p) {
 #pragma clang fp contract(off)
+  const double T = p.T, T2 = p.T2, T3 = p.T3, T4 = p.T4, lT = p.lT;
   const double* a = c + (T < 1000.0 ? 0 : 10);
-  const double T2 = T * T, T3 = pow(T, 3.0), T4 = pow(T, 4.0), lT = log(T);
   const double h = -a[0] / T2 + a[1] * lT / T + a[2] + a[3] * T / 2.0 + a[4] * T2 / 3.0 +
                    a[5] * T3 / 4.0 + a[6] * T4 / 5.0 + a[8] / T;
   const double s = -a[0] / T2 / 2.0 - a[1] / T + a[2] * lT + a[3] * T + a[4] * T2 / 2.0 +
@@ -71,50 +92,58 @@ __device__ inline double gibbs_rt(const double* c, double T) {
   return h - s;
 }
 
-// Equilibrium constant of forward row i; g(j) gives g/RT of species j, 0
-// for the padding species (gibbs.py:100-124).
+// Equilibrium constant of a pair's forward row; g(j) gives g/RT of species
+// j (the padding species counts 0); lct = ln(CORR T) of the layer
+// (gibbs.py:107-126).
 template <class G>
-__device__ inline double keq(const VkRatesDev& R, int i, double T, G g) {
+__device__ inline double keq(const VkRatePair& q, double T, double lct, int ni, G g) {
 #pragma clang fp contract(off)
-  if (!R.has_rev[i]) return 1.0;
-  double rs = 0.0, ps = 0.0, nr = 0.0, np = 0.0;
-  for (int s = 0; s < R.T; ++s) {
-    const double st = R.r_st[i * R.T + s];
-    if (st != 0.0) rs = rs + st * g(R.r_idx[i * R.T + s]);
+  if (!(q.flags & VK_PAIR_HAS_REV)) return 1.0;
+  double rs = 0.0, ps = 0.0;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const double st = q.st[s];
+    if (st != 0.0) rs = rs + st * (q.sp[s] < ni ? g(q.sp[s]) : 0.0);
   }
-  for (int s = 0; s < R.T; ++s) {
-    const double st = R.p_st[i * R.T + s];
-    if (st != 0.0) ps = ps + st * g(R.p_idx[i * R.T + s]);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const double st = q.st[4 + s];
+    if (st != 0.0) ps = ps + st * (q.sp[4 + s] < ni ? g(q.sp[4 + s]) : 0.0);
   }
-  for (int s = 0; s < R.T; ++s) nr = nr + R.r_st[i * R.T + s];
-  for (int s = 0; s < R.T; ++s) np = np + R.p_st[i * R.T + s];
-  const double dn = nr - np;
   double K = exp(rs - ps);
-  if (dn != 0.0) K = K * pow(VK_GIBBS_CORR * T, dn);
+  // (CORR T)^dn: exact for dn = +-1, exp(dn ln(CORR T)) otherwise.
+  if (q.dn == 1.0) {
+    K = K * (VK_GIBBS_CORR * T);
+  } else if (q.dn == -1.0) {
+    K = K * (1.0 / (VK_GIBBS_CORR * T));
+  } else if (q.dn != 0.0) {
+    K = K * exp(q.dn * lct);
+  }
   return K;
 }
 
-// Row i of the full build at one layer (rates.py:185-207): forward rows,
-// reverse rows k_f / K_eq below stop_rev_indx, the rows past it zeroed
-// (gibbs.py:149-150), then the host's zero mask.
-__device__ inline double build_entry(const VkRatesDev& R, int i, double T, double M,
-                                     int lowt_on) {
+// Both rows of a pair in the full build at one layer (rates.py:185-207):
+// the forward rate, and k_f / K_eq for a reverse row below stop_rev_indx
+// (0 where K <= 0, gibbs.py:141-149); the rows the record marks are zero.
+template <class G>
+__device__ inline void build_pair(const VkRatePair& q, double T, double lnT, double lct,
+                                  double M, int lowt_on, int ni, G g, double& k_fwd,
+                                  double& k_rev) {
 #pragma clang fp contract(off)
-  if (R.zero_rows && R.zero_rows[i]) return 0.0;
-  if (i == 0) return 0.0;
-  if (i >= R.stop_rev + 1 && ((i - R.stop_rev - 1) & 1) == 0) return 0.0;
-  if ((i & 1) == 0) {
-    if (i >= R.stop_rev) return 0.0;
-    const double f = forward_rate(R, i - 1, T, M, lowt_on);
-    const double K = keq(R, i - 1, T, [&](int j) {
-      return j < R.ni ? gibbs_rt(R.nasa9 + 20 * j, T) : 0.0;
-    });
-    return K > 0.0 ? f / K : 0.0;
+  const bool zf = q.flags & VK_PAIR_ZERO_FWD;
+  const bool zr = q.flags & VK_PAIR_ZERO_REV;
+  double f = 0.0;
+  if (!zf || !zr) f = forward_rate(q, T, lnT, M, lowt_on);
+  k_fwd = zf ? 0.0 : f;
+  k_rev = 0.0;
+  if (!zr) {
+    const double K = keq(q, T, lct, ni, g);
+    k_rev = K > 0.0 ? f / K : 0.0;
   }
-  return forward_rate(R, i, T, M, lowt_on);
 }
 
-// One column: out (rows or species, nz), entry (i, z) at i * nz + z.
+// One column, one stage: out (rows or species, nz), entry (i, z) at
+// i * nz + z.
 __global__ __launch_bounds__(256) void k_rates_eval(VkRatesDev R, int what, int nz,
                                                     const double* __restrict__ Tz,
                                                     const double* __restrict__ Mz, int lowt_on,
@@ -128,19 +157,21 @@ __global__ __launch_bounds__(256) void k_rates_eval(VkRatesDev R, int what, int 
   const int i = (int)(e / nz), z = (int)(e % nz);
   const double T = Tz[z];
   const double M = Mz ? Mz[z] : 0.0;
+  const bool fwd_row = (what != VK_RATES_GIBBS) && (i & 1);
   double v = 0.0;
   switch (what) {
     case VK_RATES_FORWARD:
-      v = (i & 1) ? forward_rate(R, i, T, M, lowt_on) : 0.0;
+      v = fwd_row ? forward_rate(R.pairs[i >> 1], T, log(T), M, lowt_on) : 0.0;
       break;
     case VK_RATES_CAPS:
-      v = (i & 1) ? vk_lowt_cap(R.lowt[i], in_a[e], T, M) : in_a[e];
+      v = fwd_row ? vk_lowt_cap(R.pairs[i >> 1].lowt, in_a[e], T, M) : in_a[e];
       break;
     case VK_RATES_GIBBS:
-      v = gibbs_rt(R.nasa9 + 20 * i, T);
+      v = gibbs_rt(R.nasa9 + 20 * i, tpow(T));
       break;
     case VK_RATES_KEQ:
-      v = (i & 1) ? keq(R, i, T, [&](int j) { return j < R.ni ? in_a[(int64_t)j * nz + z] : 0.0; })
+      v = fwd_row ? keq(R.pairs[i >> 1], T, log(VK_GIBBS_CORR * T), R.ni,
+                        [&](int j) { return in_a[(int64_t)j * nz + z]; })
                   : 1.0;
       break;
     case VK_RATES_FILL: {
@@ -154,31 +185,89 @@ __global__ __launch_bounds__(256) void k_rates_eval(VkRatesDev R, int what, int 
         v = 0.0;
       }
     } break;
-    default:  // VK_RATES_BUILD
-      v = build_entry(R, i, T, M, lowt_on);
+    default:
       break;
   }
   out[e] = v;
 }
 
+constexpr int VK_RATES_ZT = 64;   // layers per block, one per lane
+constexpr int VK_RATES_NT = 512;  // threads per block
+
+// The full build over layers z0 .. z0+63 of one column.
+__device__ inline void build_tile(const VkRatesDev& R, const double* __restrict__ Tz,
+                                  const double* __restrict__ Mz, int nz, int z0, int lowt_on,
+                                  double* __restrict__ out, double* lds) {
+#pragma clang fp contract(off)
+  double* lT = lds;                       // per-layer T, powers, logs, M
+  double* lT2 = lT + VK_RATES_ZT;
+  double* lT3 = lT2 + VK_RATES_ZT;
+  double* lT4 = lT3 + VK_RATES_ZT;
+  double* llog = lT4 + VK_RATES_ZT;
+  double* lM = llog + VK_RATES_ZT;
+  double* lct = lM + VK_RATES_ZT;         // ln(CORR T)
+  double* gl = lct + VK_RATES_ZT;         // (ni, ZT) g/RT
+  const int tid = threadIdx.x;
+  const int zn = min(VK_RATES_ZT, nz - z0);
+  if (tid < VK_RATES_ZT) {
+    const double T = tid < zn ? Tz[z0 + tid] : 300.0;
+    const TPow p = tpow(T);
+    lT[tid] = p.T;
+    lT2[tid] = p.T2;
+    lT3[tid] = p.T3;
+    lT4[tid] = p.T4;
+    llog[tid] = p.lT;
+    lM[tid] = tid < zn ? Mz[z0 + tid] : 0.0;
+    lct[tid] = log(VK_GIBBS_CORR * T);
+  }
+  __syncthreads();
+  for (int e = tid; e < R.ni * VK_RATES_ZT; e += VK_RATES_NT) {
+    const int j = e / VK_RATES_ZT, z = e % VK_RATES_ZT;
+    const TPow p{lT[z], lT2[z], lT3[z], lT4[z], llog[z]};
+    gl[e] = gibbs_rt(R.nasa9 + 20 * j, p);
+  }
+  __syncthreads();
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (lane >= zn) return;
+  const double T = lT[lane], M = lM[lane], lnT = llog[lane], lc = lct[lane];
+  auto g = [&](int j) { return gl[j * VK_RATES_ZT + lane]; };
+  double* o = out + z0 + lane;
+  if (wave == 0) o[0] = 0.0;               // row 0
+  const VkRatePair* __restrict__ pairs = R.pairs;
+  for (int p = wave; p < R.npairs; p += VK_RATES_NT / 64) {
+    const VkRatePair q = pairs[p];
+    double kf, kr;
+    build_pair(q, T, lnT, lc, M, lowt_on, R.ni, g, kf, kr);
+    const int i = 2 * p + 1;
+    o[(int64_t)i * nz] = kf;
+    if (i + 1 <= R.nr) o[(int64_t)(i + 1) * nz] = kr;
+  }
+}
+
+// One column, T and M given as arrays; out (nr+1, nz).
+__global__ __launch_bounds__(VK_RATES_NT) void k_rates_build_one(
+    VkRatesDev R, int nz, const double* __restrict__ Tz, const double* __restrict__ Mz,
+    int lowt_on, double* __restrict__ out) {
+  extern __shared__ double lds[];
+  build_tile(R, Tz, Mz, nz, blockIdx.x * VK_RATES_ZT, lowt_on, out, lds);
+}
+
 // A batch: column c's rates (its network's S rows x nz layers) at off_k,
 // T and M read from its packed atmosphere record.
-__global__ __launch_bounds__(256) void k_rates_build_batch(const VkCol* __restrict__ cols,
-                                                           int ncols,
-                                                           const VkRatesDev* __restrict__ rates,
-                                                           const double* __restrict__ atm,
-                                                           int lowt_on, double* __restrict__ k) {
+__global__ __launch_bounds__(VK_RATES_NT) void k_rates_build_batch(
+    const VkCol* __restrict__ cols, int ncols, const VkRatesDev* __restrict__ rates,
+    const double* __restrict__ atm, int lowt_on, double* __restrict__ k) {
+  extern __shared__ double lds[];
   const int c = blockIdx.y;
   if (c >= ncols) return;
   const VkCol col = cols[c];
-  const VkRatesDev& R = rates[col.net];
-  const int nz = col.nz;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)(R.nr + 1) * nz) return;
-  const int i = (int)(e / nz), z = (int)(e % nz);
-  const VkAtmOff o = vk_atm_offsets(nz, col.ni);
+  const int z0 = blockIdx.x * VK_RATES_ZT;
+  if (z0 >= col.nz) return;
+  const VkAtmOff o = vk_atm_offsets(col.nz, col.ni);
   const double* rec = atm + col.off_a;
-  k[col.off_k + e] = build_entry(R, i, rec[o.Tco + z], rec[o.M + z], lowt_on);
+  const VkRatesDev R = rates[col.net];
+  build_tile(R, rec + o.Tco, rec + o.M, col.nz, z0, lowt_on, k + col.off_k, lds);
 }
 
 }  // namespace
@@ -186,6 +275,13 @@ __global__ __launch_bounds__(256) void k_rates_build_batch(const VkCol* __restri
 int vk_launch_rates_eval(const VkRatesDev& R, int what, int nz, const double* T, const double* M,
                          int lowt_on, const double* in_a, const double* in_b, double* out,
                          hipStream_t stream) {
+  if (what == VK_RATES_BUILD) {
+    const size_t shm = sizeof(double) * VK_RATES_ZT * (7 + R.ni);
+    hipLaunchKernelGGL(k_rates_build_one, dim3((nz + VK_RATES_ZT - 1) / VK_RATES_ZT),
+                       dim3(VK_RATES_NT), shm, stream, R, nz, T, M, lowt_on, out);
+    VK_CHECK_HIP(hipGetLastError());
+    return VK_OK;
+  }
   const int rows = (what == VK_RATES_GIBBS) ? R.ni : R.nr + 1;
   const int64_t n = (int64_t)rows * nz;
   if (n <= 0) return VK_OK;
@@ -195,13 +291,13 @@ int vk_launch_rates_eval(const VkRatesDev& R, int what, int nz, const double* T,
   return VK_OK;
 }
 
-int vk_launch_rates_build_batch(const VkCol* cols, int ncols, int max_S, int max_nz,
+int vk_launch_rates_build_batch(const VkCol* cols, int ncols, int max_ni, int max_nz,
                                 const VkRatesDev* rates, const double* atm, int lowt_on,
                                 double* k, hipStream_t stream) {
-  const int64_t n = (int64_t)max_S * max_nz;
-  if (n <= 0 || ncols <= 0) return VK_OK;
-  hipLaunchKernelGGL(k_rates_build_batch, dim3((unsigned)((n + 255) / 256), ncols), dim3(256), 0,
-                     stream, cols, ncols, rates, atm, lowt_on, k);
+  if (max_nz <= 0 || ncols <= 0) return VK_OK;
+  const size_t shm = sizeof(double) * VK_RATES_ZT * (7 + max_ni);
+  hipLaunchKernelGGL(k_rates_build_batch, dim3((max_nz + VK_RATES_ZT - 1) / VK_RATES_ZT, ncols),
+                     dim3(VK_RATES_NT), shm, stream, cols, ncols, rates, atm, lowt_on, k);
   VK_CHECK_HIP(hipGetLastError());
   return VK_OK;
 }

@@ -1,5 +1,5 @@
 """Copy the NASA-9 fits of ten species from the reference's thermo/NASA9
-data folder into tests/golden/nasa9/ (data files, unchanged).
+data folder into tests/golden/NASA9/ (data files, unchanged).
 
 Run once in the build container:  python tests/golden/copy_nasa9.py
 """
@@ -7,7 +7,7 @@ import shutil
 from pathlib import Path
 
 SRC = Path("/root/reference/thermo/NASA9")
-DST = Path(__file__).resolve().parent / "nasa9"
+DST = Path(__file__).resolve().parent / "NASA9"
 SPECIES = ("H", "H2", "O", "O2", "OH", "H2O", "CO", "CO2", "CH3", "CH4")
 
 if __name__ == "__main__":

@@ -5,7 +5,7 @@ not shipped (test_rates.py:27-33, test_gibbs.py:24-31), so the oracle is
 pinned here by: the known-answer cases of test_read_rate.py (the three
 low-T caps, the literal remove list), limits of the falloff forms,
 standard Gibbs energies of formation (JANAF tables) reproduced from the
-reference's own NASA-9 data files (tests/golden/nasa9, copied unchanged by
+reference's own NASA-9 data files (tests/golden/NASA9, copied unchanged by
 tests/golden/copy_nasa9.py), and equilibrium constants of two reactions in
 both unit conventions (dn = 0 and dn = 1).  Each is chosen so that a
 dropped term, a wrong sign or a wrong branch fails it.
@@ -21,7 +21,7 @@ from oracle import rates_np as R
 from vulcan_amd import synth
 from vulcan_amd._types import RateNetwork
 
-GOLD = Path(__file__).resolve().parent / "golden" / "nasa9"
+GOLD = Path(__file__).resolve().parent / "golden" / "NASA9"
 SPECIES = ("H", "H2", "O", "O2", "OH", "H2O", "CO", "CO2", "CH3", "CH4")
 RGAS = 8.314462618e-3     # kJ/(mol K)
 
