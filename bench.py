@@ -52,6 +52,55 @@ def factor_traffic(B: int):
     return None
 
 
+def bench_rates(B: int, net, atm, reps: int = 20, cpu_cols: int = 20, no_cpu: bool = False):
+    """Batched rate construction k(T, M) (forward rates, NASA-9 K_eq, reverse
+    fill) for B columns of the step's network, T and M from each column's
+    packed atmosphere record, timed with HIP events on the launch stream;
+    the NumPy oracle times a few columns on the host."""
+    import numpy as np
+    import torch
+    from vulcan_amd import core, rates, synth
+    rng = np.random.default_rng(77)
+    rnet = synth.random_rate_network(rng, net, NREV, NPHOTO)
+    c9 = synth.random_nasa9(rng, NI)
+    b = core.uniform_batch(B, NZ, net)
+    out = torch.empty(b.k_len, dtype=torch.float64, device="cuda")
+    rates.build_rate_batch(b, [rnet], [c9], atm, False, None, out=out)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        rates.build_rate_batch(b, [rnet], [c9], atm, False, None, out=out)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = b.k_len * 8
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    res = {"workload": f"build_rate_array for {B} columns, nz={NZ}, ni={NI}, "
+                       f"{net.nr + 1} rate rows (random Arrhenius/Lindemann/Troe/cap rows, "
+                       f"random NASA-9 fits)",
+           "ms_per_batch": round(ms, 4), "column_builds_per_s": round(B / (ms * 1e-3), 1),
+           "bytes_written": nbytes, "write_GBps": round(gbs, 1),
+           "hbm_frac": round(gbs / PEAK_HBM_GBS, 4)}
+    if not no_cpu:
+        from oracle import rates_np as O
+        rec_len = atm.numel() // B
+        _, off = core.atm_record_layout(NZ, NI)
+        r0 = atm[:rec_len].cpu().numpy()
+        T = r0[off[5]:off[5] + NZ]
+        M = r0[off[7]:off[7] + NZ]
+        O.build_rate_array(rnet, T, M, c9)
+        t0 = time.perf_counter()
+        with np.errstate(all="ignore"):
+            for _ in range(cpu_cols):
+                O.build_rate_array(rnet, T, M, c9)
+        cpu_s = (time.perf_counter() - t0) / cpu_cols
+        res["cpu_oracle_column_builds_per_s"] = round(1.0 / cpu_s, 2)
+        res["cpu_sample"] = f"{cpu_cols} columns, NumPy oracle, 1 thread"
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +125,10 @@ def main() -> int:
     ap.add_argument("--factor", choices=("auto", "tile", "rows", "blocked", "tile2", "wave", "tile8"), default="auto",
                     help="block-factor kernel for the nz=150 / ni=93 bucket (auto: the "
                          "library default)")
+    ap.add_argument("--no-rates", action="store_true",
+                    help="skip the rate-construction measurement")
+    ap.add_argument("--rates-only", action="store_true",
+                    help="only measure the batched rate construction k(T, M)")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only run a plain elementwise copy of GIB GiB (reads GIB, writes "
                          "GIB): the reference kernel that scales the HBM byte counters")
@@ -116,6 +169,13 @@ def main() -> int:
     atm = torch.cat([core.pack_atm_record(c.atm) for c in cols]).to(dev)
     dt = torch.full((B,), 1e-10, dtype=torch.float64, device=dev)
     print(f"[bench] generated {B} columns in {time.time() - t_gen:.1f}s", file=sys.stderr, flush=True)
+
+    if args.rates_only:
+        rb = bench_rates(B, net, atm, reps=max(args.steps, 1), no_cpu=args.no_cpu)
+        print(json.dumps({"metric": "rate-construction column-builds/s",
+                          "value": rb["column_builds_per_s"], "unit": "column-builds/s",
+                          "n_gpus": 1, "dtype": "f64", "data": "synthetic", "rate_build": rb}))
+        return 0
 
     def make(Bn):
         b = core.uniform_batch(Bn, NZ, net)
@@ -246,6 +306,11 @@ def main() -> int:
                             "change < 1e-4 or 5000 accepted steps"}
         print(f"[bench] serving: {serving}", file=sys.stderr, flush=True)
 
+    rate_build = None
+    if not args.no_rates:
+        rate_build = bench_rates(B, net, atm, no_cpu=args.no_cpu)
+        print(f"[bench] rate build: {rate_build}", file=sys.stderr, flush=True)
+
     # ---- roofline of the dominant kernel (block factor) ---------------------
     fac_ms = kern_ms[1]
     fac_flops = B * factor_flops(NZ, NI)
@@ -309,6 +374,7 @@ def main() -> int:
         "singular_columns": n_singular,
         "uneven_batch": uneven,
         "serving": serving,
+        "rate_build": rate_build,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
