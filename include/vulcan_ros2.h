@@ -254,9 +254,9 @@ int vk_photo_f64(vk_batch b, int parts, const vk_photo_setup* setup, const vk_ph
 /* ---- rate construction k(T, M) ----
  * Restates the reference's host-side rate build on the device:
  * rates.compute_forward_k (rates.py:61-122), rates.apply_lowT_caps
- * (rates.py:138-167), gibbs.gibbs_sp_vector (gibbs.py:41-78),
- * gibbs.K_eq_array (gibbs.py:81-126), gibbs.fill_reverse_k (gibbs.py:129-152)
- * and rates.build_rate_array (rates.py:185-207).
+ * (rates.py:142-167), gibbs.gibbs_sp_vector (gibbs.py:41-74),
+ * gibbs.K_eq_array (gibbs.py:77-128), gibbs.fill_reverse_k (gibbs.py:131-154)
+ * and rates.build_rate_array (rates.py:185-208).
  *
  * vk_rates_create takes the network's rows classified once on the host:
  *   form[i]    VK_KF_* forward form of row i (VK_KF_ZERO for reverse rows,
@@ -265,7 +265,8 @@ int vk_photo_f64(vk_batch b, int parts, const vk_photo_setup* setup, const vk_ph
  *   lowt[i]    VK_LOWT_* low-T cap of forward row i;
  *   has_rev[i] forward row i has a thermal reverse at i+1 (K_eq computed);
  *   par        (nr+1, 6): a, n, E, a_inf, n_inf, E_inf;
- *   tables     (nr+1, T) reactant/product species (PAD = ni) and stoich;
+ *   tables     (nr+1, T) reactant/product species (PAD = ni) and stoich,
+ *              T <= 4, stoich exact in fp32, ni <= 255;
  *   nasa9      (ni, 2, 10) NASA-9 coefficients, low-T range first (NULL:
  *              no Gibbs / K_eq / build);
  *   zero_rows  (nr+1) rows the build and the reverse fill zero (the remove
@@ -304,6 +305,15 @@ int vk_rates_destroy(vk_rates r);
 int vk_rates_eval_f64(vk_rates r, int what, int nz, const double* T, const double* M,
                       int lowt_on, const double* in_a, const double* in_b, double* out,
                       void* stream);
+/* vk_rates_vjp_f64: the cotangents of the full build's inputs,
+ * T_bar[z] = sum_i k_bar[i, z] dk[i, z]/dT[z] and M_bar likewise, for
+ * k = VK_RATES_BUILD(T, M).  No counterpart in the reference, whose NumPy
+ * rate build sits outside its autodiff; it is for callers that need
+ * gradients through k(T) and M(T) (the steady-state sensitivity to T(z)).
+ * k_bar (nr+1, nz); either output may be NULL; the sums over rows are in a
+ * fixed order. */
+int vk_rates_vjp_f64(vk_rates r, int nz, const double* T, const double* M, int lowt_on,
+                     const double* k_bar, double* T_bar, double* M_bar, void* stream);
 int vk_batch_set_rates(vk_batch b, const vk_rates* per_net, int n);
 int vk_rates_build_batch_f64(vk_batch b, int lowt_on, const double* atm, double* k, void* stream);
 

@@ -1056,6 +1056,9 @@ int vk_tune_get(int knob, int sl) {
 int vk_launch_rates_eval(const VkRatesDev& R, int what, int nz, const double* T, const double* M,
                          int lowt_on, const double* in_a, const double* in_b, double* out,
                          hipStream_t stream);
+int vk_launch_rates_vjp(const VkRatesDev& R, int nz, const double* T, const double* M,
+                        int lowt_on, const double* kbar, double* Tbar, double* Mbar,
+                        hipStream_t stream);
 int vk_launch_rates_build_batch(const VkCol* cols, int ncols, int max_ni, int max_nz,
                                 const VkRatesDev* rates, const double* atm, int lowt_on,
                                 double* k, hipStream_t stream);
@@ -1175,6 +1178,18 @@ int vk_rates_eval_f64(vk_rates r, int what, int nz, const double* T, const doubl
   }
   return vk_launch_rates_eval(r->dev, what, nz, T, M, lowt_on, in_a, in_b, out,
                               (hipStream_t)stream);
+}
+
+int vk_rates_vjp_f64(vk_rates r, int nz, const double* T, const double* M, int lowt_on,
+                     const double* k_bar, double* T_bar, double* M_bar, void* stream) {
+  g_err[0] = 0;
+  if (!r || nz < 1 || !T || !M || !k_bar || (!T_bar && !M_bar) || !r->dev.nasa9) {
+    vk_set_error("vk_rates_vjp_f64: bad arguments (nz=%d)%s", nz,
+                 r && !r->dev.nasa9 ? ", handle has no NASA-9 coefficients" : "");
+    return VK_ERR_ARG;
+  }
+  return vk_launch_rates_vjp(r->dev, nz, T, M, lowt_on, k_bar, T_bar, M_bar,
+                             (hipStream_t)stream);
 }
 
 int vk_batch_set_rates(vk_batch b, const vk_rates* per_net, int n) {

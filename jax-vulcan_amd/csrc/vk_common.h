@@ -154,29 +154,6 @@ struct VkRatesDev {
   const uint8_t* zero_rows; // (nr+1) or null
 };
 
-// kb / P0, the concentration-unit correction of K_eq (gibbs.py:11-12).
-constexpr double VK_GIBBS_CORR = 1.38064852e-16 / 1.0e6;
-
-// The low-T caps of rates.apply_lowT_caps (rates.py:149-165).
-__device__ inline double vk_lowt_cap(int code, double k, double T, double M) {
-#pragma clang fp contract(off)
-  switch (code) {
-    case VK_LOWT_CH3:
-      if (T <= 277.5) {
-        const double k0 = 6.0e-29;
-        const double kinf = 2.06e-10 * pow(T, -0.4);
-        return k0 / (1.0 + k0 * M / kinf);
-      }
-      return k;
-    case VK_LOWT_C2H4:
-      return T <= 300.0 ? 3.7e-30 : k;
-    case VK_LOWT_C2H5:
-      return T <= 200.0 ? 2.49e-27 : k;
-    default:
-      return k;
-  }
-}
-
 // One column of a batch.
 struct VkCol {
   int32_t nz, ni, S, net;

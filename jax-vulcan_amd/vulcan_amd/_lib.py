@@ -56,6 +56,7 @@ _SIGS = {
     "vk_rates_create": (i32, [i32, i32, i32, i32, p, p, p, p, p, p, p, p, p, p, C.POINTER(p)]),
     "vk_rates_destroy": (i32, [p]),
     "vk_rates_eval_f64": (i32, [p, i32, i32, p, p, i32, p, p, p, p]),
+    "vk_rates_vjp_f64": (i32, [p, i32, p, p, i32, p, p, p, p]),
     "vk_batch_set_rates": (i32, [p, p, i32]),
     "vk_rates_build_batch_f64": (i32, [p, i32, p, p, p]),
 }

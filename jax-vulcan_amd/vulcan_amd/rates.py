@@ -11,9 +11,11 @@ The host reads the network once (``rate_tables``): the forward form of
 each row (modified Arrhenius, Lindemann falloff, the hard-coded Troe
 reaction, or zero), which of the three low-T caps it takes, and whether
 its reverse is filled from the equilibrium constant.  The device does the
-arithmetic, one thread per (row, layer); ``build_rate_array`` is a single
-launch, and ``build_rate_batch`` builds the rates of a whole batch of
-columns (each with its own T(z), M(z)) in one launch.
+arithmetic: ``build_rate_array`` is a single launch, ``build_rate_batch``
+builds the rates of a whole batch of columns (each with its own T(z),
+M(z)) in one launch, and ``build_rate_array_differentiable`` carries
+gradients with respect to T and M through the device's vector-Jacobian
+product.
 """
 from __future__ import annotations
 
@@ -29,8 +31,8 @@ from ._types import RateNetwork
 from .core import F64, Batch, device
 
 __all__ = ["RateNetwork", "rate_tables", "DeviceRates", "compute_forward_k", "apply_lowT_caps",
-           "apply_remove_list", "build_rate_array", "build_rate_batch", "k_dict_from_array",
-           "k_array_from_dict"]
+           "apply_remove_list", "build_rate_array", "build_rate_array_differentiable",
+           "build_rate_batch", "k_dict_from_array", "k_array_from_dict"]
 
 # forms, caps and modes of include/vulcan_ros2.h
 KF_ZERO, KF_ARRHENIUS, KF_LINDEMANN, KF_TROE_OH_CH3 = 0, 1, 2, 3
@@ -172,6 +174,40 @@ class DeviceRates:
             "vk_rates_eval_f64")
         return out
 
+    def vjp(self, T, M, k_bar, lowt_on=False):
+        """(T_bar, M_bar) = k_bar . dk/dT, k_bar . dk/dM of the full build."""
+        dev = device()
+        T = torch.as_tensor(T, dtype=F64, device=dev).contiguous()
+        M = torch.as_tensor(M, dtype=F64, device=dev).contiguous()
+        k_bar = torch.as_tensor(k_bar, dtype=F64, device=dev).contiguous()
+        nz = int(T.shape[0])
+        if tuple(k_bar.shape) != (self.nr + 1, nz):
+            raise ValueError(f"k_bar has shape {tuple(k_bar.shape)}, expected "
+                             f"({self.nr + 1}, {nz})")
+        Tb = torch.empty(nz, dtype=F64, device=dev)
+        Mb = torch.empty(nz, dtype=F64, device=dev)
+        _lib.check(_lib.load().vk_rates_vjp_f64(
+            self.handle, nz, T.data_ptr(), M.data_ptr(), int(bool(lowt_on)), k_bar.data_ptr(),
+            Tb.data_ptr(), Mb.data_ptr(), _lib.stream_ptr()), "vk_rates_vjp_f64")
+        return Tb, Mb
+
+
+class _RateBuild(torch.autograd.Function):
+    """k = build(T, M) with its T and M cotangents from vk_rates_vjp_f64."""
+
+    @staticmethod
+    def forward(ctx, T, M, rates_handle, lowt_on):
+        ctx.rates = rates_handle
+        ctx.lowt = lowt_on
+        ctx.save_for_backward(T, M)
+        return rates_handle.eval(RATES_BUILD, T, M, lowt_on=lowt_on)
+
+    @staticmethod
+    def backward(ctx, k_bar):
+        T, M = ctx.saved_tensors
+        Tb, Mb = ctx.rates.vjp(T, M, k_bar, ctx.lowt)
+        return Tb.to(T.dtype), Mb.to(M.dtype), None, None
+
 
 # Handles are cached per (network object, coefficients, mask) so repeated
 # builds (an atmosphere refresh, every column of a batch) upload nothing.
@@ -241,6 +277,23 @@ def build_rate_array(cfg, net, atm, nasa9_coeffs) -> torch.Tensor:
     mask = remove_mask(net, getattr(cfg, "remove_list", None), paired=False)
     lowt = bool(getattr(cfg, "use_lowT_limit_rates", False))
     return device_rates(net, nasa9_coeffs, mask).eval(RATES_BUILD, T, M, lowt_on=lowt)
+
+
+def build_rate_array_differentiable(net, T, M, nasa9_coeffs, use_lowT_limit_rates=False,
+                                    remove_list=None) -> torch.Tensor:
+    """build_rate_array on torch tensors T, M (nz,) that carries gradients:
+    backward returns k_bar . dk/dT and k_bar . dk/dM from the device, exact
+    derivatives of the same expressions (no counterpart in the reference,
+    whose NumPy rate build is outside its autodiff)."""
+    dev = device()
+    T = torch.as_tensor(T, dtype=F64, device=dev)
+    M = torch.as_tensor(M, dtype=F64, device=dev)
+    if T.shape != M.shape or T.dim() != 1:
+        raise ValueError(f"T and M must have the same 1-D shape; got {tuple(T.shape)} and "
+                         f"{tuple(M.shape)}")
+    mask = remove_mask(net, remove_list, paired=False)
+    d = device_rates(net, nasa9_coeffs, mask)
+    return _RateBuild.apply(T, M, d, bool(use_lowT_limit_rates))
 
 
 def build_rate_batch(batch: Batch, nets: Sequence, nasa9: Sequence, atm_packed: torch.Tensor,

@@ -17,8 +17,8 @@ from __future__ import annotations
 import numpy as np
 
 KB = 1.38064852e-16         # phy_const.py:3
-P0 = 1.0e6                  # gibbs.py:11
-CORR = KB / P0              # gibbs.py:12
+P0 = 1.0e6                  # gibbs.py:12
+CORR = KB / P0              # gibbs.py:13
 
 SPECIAL_OH_CH3 = "OH + CH3 + M -> CH3OH + M"      # rates.py:27
 LOWT_CH3 = "H + CH3 + M -> CH4 + M"               # rates.py:31
@@ -70,7 +70,7 @@ def compute_forward_k(net, T, M):
 
 
 def apply_lowT_caps(net, k, T, M):
-    """The three low-T caps on forward rows (rates.py:138-167)."""
+    """The three low-T caps on forward rows (rates.py:142-167)."""
     k = np.array(k, np.float64)
     T = np.asarray(T, np.float64)
     M = np.asarray(M, np.float64)
@@ -91,7 +91,7 @@ def apply_lowT_caps(net, k, T, M):
 
 def load_nasa9(paths):
     """(ni, 2, 10) coefficients from NASA-9 text files, low-T range first
-    (gibbs.py:15-38); a missing file gives zeros."""
+    (gibbs.py:16-38); a missing file gives zeros."""
     out = np.zeros((len(paths), 2, 10))
     for j, p in enumerate(paths):
         if p is None:
@@ -104,7 +104,7 @@ def load_nasa9(paths):
 
 def gibbs_sp_vector(coeffs, T):
     """g/(RT) per species and layer from the NASA-9 fits: H/RT - S/R, low
-    range below 1000 K (gibbs.py:41-78)."""
+    range below 1000 K (gibbs.py:41-74)."""
     T = np.asarray(T, np.float64)[None, :]
 
     def branch(a):
@@ -121,7 +121,7 @@ def gibbs_sp_vector(coeffs, T):
 
 def K_eq_array(net, g_sp, T):
     """Equilibrium constants of the forward rows; 1 where there is no thermal
-    reverse (gibbs.py:81-126)."""
+    reverse (gibbs.py:77-128)."""
     T = np.asarray(T, np.float64)
     nz = T.shape[0]
     K = np.ones((net.nr + 1, nz))
@@ -148,7 +148,7 @@ def K_eq_array(net, g_sp, T):
 
 def fill_reverse_k(net, k, K_eq, remove_list=None):
     """Reverse rows k_f / K; a removed row takes its partner with it; rows past
-    stop_rev_indx (odd offsets) are zeroed (gibbs.py:129-152)."""
+    stop_rev_indx (odd offsets) are zeroed (gibbs.py:131-154)."""
     k = np.array(k, np.float64)
     remove = set(remove_list or [])
     for i in range(2, net.stop_rev_indx, 2):
@@ -175,7 +175,7 @@ def apply_remove_list(net, k, remove_list):
 
 def build_rate_array(net, T, M, coeffs, use_lowT_limit_rates=False, remove_list=None):
     """Forward, optional low-T caps, reverse fill, then the remove list
-    without pairing (rates.py:185-207)."""
+    without pairing (rates.py:185-208)."""
     k = compute_forward_k(net, T, M)
     if use_lowT_limit_rates:
         k = apply_lowT_caps(net, k, T, M)
@@ -186,7 +186,7 @@ def build_rate_array(net, T, M, coeffs, use_lowT_limit_rates=False, remove_list=
 
 def compute_all_k(net, T, M, coeffs, remove_list=None):
     """Forward rates plus the reverse fill with paired removal
-    (gibbs.py:155-170)."""
+    (gibbs.py:157-170)."""
     k = compute_forward_k(net, T, M)
     K = K_eq_array(net, gibbs_sp_vector(coeffs, T), T)
     return fill_reverse_k(net, k, K, remove_list=remove_list)

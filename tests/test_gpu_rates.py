@@ -188,3 +188,79 @@ def test_rates_errors():
     d = rates.DeviceRates(net)               # no NASA-9 table
     with pytest.raises(_lib.VulcanError):
         d.eval(rates.RATES_BUILD, T, M)
+
+
+def _cap_free_grid(nz, lo, hi, seed):
+    """Layer temperatures at least 0.5 K away from the cap thresholds and
+    the NASA-9 range switch, where k(T) has a jump."""
+    rng = np.random.default_rng(seed)
+    T = np.sort(rng.uniform(lo, hi, nz))
+    for edge in (200.0, 277.5, 300.0, 1000.0):
+        near = np.abs(T - edge) < 0.5
+        T[near] += 1.0
+    return T
+
+
+@pytest.mark.parametrize("lowt", [False, True])
+def test_rates_vjp_matches_oracle_central_differences(lowt):
+    """T_bar and M_bar of the device VJP against five-point central
+    differences of the oracle's build_rate_array in T and M (h = 1e-5 of T,
+    1e-4 of M: truncation O(h^4), rounding ~1e-11 of each term), with k_bar
+    weighted by 1 / |k| so every row counts: the VJP is then
+    sum_i w_i d ln k_i / dT.  Agreement to 1e-6 of the sum of |terms|."""
+    from oracle import rates_np as O
+    from vulcan_amd import rates
+    _, net, coeffs, _, _ = _setup(seed=21, ni=50, n_rev=300, n_photo=10, nz=40)
+    nz = 40
+    T = _cap_free_grid(nz, 130.0, 2600.0, 5)
+    M = np.logspace(19.0, 9.0, nz)
+    with np.errstate(all="ignore"):
+        k0 = O.build_rate_array(net, T, M, coeffs, use_lowT_limit_rates=lowt)
+    rng = np.random.default_rng(6)
+    ok = np.isfinite(k0) & (np.abs(k0) > 1e-290) & (np.abs(k0) < 1e290)
+    w = np.where(ok, rng.uniform(-1.0, 1.0, k0.shape) / np.where(ok, np.abs(k0), 1.0), 0.0)
+    d = rates.device_rates(net, coeffs, None)
+    Tb, Mb = d.vjp(T, M, w, lowt)
+    Tb, Mb = _np(Tb), _np(Mb)
+    for name, x, g in (("T", T, Tb), ("M", M, Mb)):
+        worst = 0.0
+        for z in range(nz):
+            h = (1e-5 if name == "T" else 1e-4) * x[z]
+            ks = []
+            for step in (2.0, 1.0, -1.0, -2.0):
+                xs = x.copy()
+                xs[z] += step * h
+                args = (xs, M) if name == "T" else (T, xs)
+                with np.errstate(all="ignore"):
+                    ks.append(O.build_rate_array(net, *args, coeffs,
+                                                 use_lowT_limit_rates=lowt)[:, z])
+            d = (-ks[0] + 8.0 * ks[1] - 8.0 * ks[2] + ks[3]) / (12.0 * h)
+            terms = w[:, z] * d
+            fd = terms.sum()
+            scale = np.abs(terms).sum() + 1e-300
+            worst = max(worst, abs(g[z] - fd) / scale)
+        print(f"lowt={lowt} {name}_bar: worst |vjp - fd| / sum|terms| = {worst:.2e}")
+        assert worst < 1e-6
+
+
+def test_differentiable_build_autograd():
+    """torch autograd through build_rate_array_differentiable returns the
+    VJP of the device build, and the forward equals build_rate_array."""
+    import torch
+    from types import SimpleNamespace
+    from vulcan_amd import rates
+    _, net, coeffs, _, _ = _setup(seed=22, ni=30, n_rev=80, n_photo=0, nz=16)
+    T = torch.tensor(_cap_free_grid(16, 400.0, 1800.0, 8), dtype=torch.float64,
+                     device="cuda", requires_grad=True)
+    M = torch.tensor(np.logspace(18.0, 12.0, 16), dtype=torch.float64, device="cuda",
+                     requires_grad=True)
+    k = rates.build_rate_array_differentiable(net, T, M, coeffs)
+    ref = rates.build_rate_array(SimpleNamespace(use_lowT_limit_rates=False, remove_list=None),
+                                 net, SimpleNamespace(Tco=T.detach(), M=M.detach()), coeffs)
+    assert torch.equal(k.detach(), ref)
+    w = torch.where(k.detach() != 0, 1.0 / k.detach().abs().clamp_min(1e-300),
+                    torch.zeros_like(k))
+    (k * w).sum().backward()
+    Tb, Mb = rates.device_rates(net, coeffs, None).vjp(T.detach(), M.detach(), w)
+    assert torch.equal(T.grad, Tb) and torch.equal(M.grad, Mb)
+    assert torch.isfinite(T.grad).all() and T.grad.abs().max() > 0
