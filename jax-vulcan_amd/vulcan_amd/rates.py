@@ -160,14 +160,19 @@ class DeviceRates:
         self.handle = h
         self._fin = weakref.finalize(self, lib.vk_rates_destroy, h)
 
-    def eval(self, what: int, T, M=None, lowt_on=False, in_a=None, in_b=None) -> torch.Tensor:
+    def eval(self, what: int, T, M=None, lowt_on=False, in_a=None, in_b=None,
+             out: torch.Tensor | None = None) -> torch.Tensor:
         dev = device()
         T = torch.as_tensor(T, dtype=F64, device=dev).contiguous()
         nz = int(T.shape[0])
         if M is not None:
             M = torch.as_tensor(M, dtype=F64, device=dev).contiguous()
         rows = self.ni if what == RATES_GIBBS else self.nr + 1
-        out = torch.empty((rows, nz), dtype=F64, device=dev)
+        if out is None:
+            out = torch.empty((rows, nz), dtype=F64, device=dev)
+        elif (tuple(out.shape) != (rows, nz) or out.dtype != F64 or not out.is_contiguous()
+              or out.device != T.device):
+            raise ValueError(f"out must be a contiguous float64 ({rows}, {nz}) tensor on {dev}")
         _lib.check(_lib.load().vk_rates_eval_f64(
             self.handle, what, nz, T.data_ptr(), _lib.ptr(M), int(bool(lowt_on)),
             _lib.ptr(in_a), _lib.ptr(in_b), out.data_ptr(), _lib.stream_ptr()),

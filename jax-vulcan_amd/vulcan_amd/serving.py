@@ -2,7 +2,10 @@
 
 Requests are columns (a ``synth.Column``-like object: nz, ni, y, k, atm,
 net, dt) to be integrated to steady state with the device step control of
-``outer_loop.SteadyStateRunner``.  The reference runs one column per
+``outer_loop.SteadyStateRunner``.  A request may carry a ``rate_source``
+(``RateSource``: its rate network and NASA-9 table) instead of k: its rates
+are then built on the device, in its slot, from the T and M of its own
+atmosphere (the reference's build_rate_array, rates.py:185-208).  The reference runs one column per
 ``OuterLoop`` call (outer_loop.py:1112-1240) or a vmapped uniform batch; the
 service keeps many columns of different sizes in flight at once:
 
@@ -27,7 +30,7 @@ import collections
 import ctypes as C
 import time
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional, Sequence
+from typing import Dict, List, NamedTuple, Optional, Sequence
 
 import numpy as np
 import torch
@@ -35,6 +38,14 @@ import torch
 from . import _lib
 from .core import F64, DeviceNetwork, atm_record_layout, block_doubles, device, pack_atm_record
 from .outer_loop import (ST_CONVERGED, ST_DONE, ST_MAXSTEPS, VK_MAX_ATOMS, RunConfig, SsState)
+
+class RateSource(NamedTuple):
+    """What a request needs to have its k built on the device."""
+    net: object                      # rate network (RateNetwork or network.Network)
+    nasa9: np.ndarray                # (ni, 2, 10)
+    use_lowT_limit_rates: bool = False
+    remove_list: Optional[Sequence[int]] = None
+
 
 ST_IDLE = 16
 IDLE_STATUS = ST_IDLE | ST_CONVERGED  # DONE bits set: every kernel skips the slot
@@ -133,10 +144,22 @@ class SlotPool:
             o = slot * self.cs
             self.y[o:o + nz * ni].copy_(torch.as_tensor(np.ascontiguousarray(col.y, np.float64)).reshape(-1))
             ok = slot * self.ck
-            self.k[ok:ok + dn.S * nz].copy_(torch.as_tensor(np.ascontiguousarray(col.k, np.float64)).reshape(-1))
             oa = slot * self.ca
             rec = pack_atm_record(col.atm)
             self.atm[oa:oa + rec.numel()].copy_(rec)
+            src = getattr(col, "rate_source", None)
+            if src is not None:
+                from . import rates
+                _, off = atm_record_layout(nz, ni)
+                d = rates.device_rates(src.net, src.nasa9,
+                                       rates.remove_mask(src.net, src.remove_list, paired=False))
+                d.eval(rates.RATES_BUILD, self.atm[oa + off[5]:oa + off[5] + nz],
+                       self.atm[oa + off[7]:oa + off[7] + nz],
+                       lowt_on=src.use_lowT_limit_rates,
+                       out=self.k[ok:ok + dn.S * nz].view(dn.S, nz))
+            else:
+                self.k[ok:ok + dn.S * nz].copy_(
+                    torch.as_tensor(np.ascontiguousarray(col.k, np.float64)).reshape(-1))
             self.dt[slot:slot + 1].fill_(float(col.dt))
             _lib.check(lib.vk_batch_set_column(self.handle, slot, nz, dn.handle, st),
                        "vk_batch_set_column")

@@ -62,3 +62,48 @@ def test_single_slot_budget_queues_in_order():
     ref = [_alone(c) for c in reqs]
     for r in out:
         assert np.array_equal(r.y, ref[r.index][0])
+
+
+def _rate_source_requests():
+    """Requests that carry their rate network and NASA-9 fits instead of k,
+    and the same requests with k built beforehand by build_rate_array."""
+    from types import SimpleNamespace
+    from vulcan_amd import rates, synth
+    from vulcan_amd.serving import RateSource
+    with_src, with_k = [], []
+    for i, col in enumerate(_requests()[:4]):
+        rng = np.random.default_rng(900 + i)
+        n_rev = (col.net.nr) // 2
+        rnet = synth.random_rate_network(rng, col.net, n_rev, 0)
+        c9 = np.zeros((col.ni, 2, 10))        # g/RT = 2.5 (1 - ln T) + a8 / T - a9
+        c9[:, :, 2] = 2.5
+        c9[:, :, 8] = rng.uniform(-2000.0, 2000.0, col.ni)[:, None]
+        c9[:, :, 9] = rng.uniform(-2.0, 2.0, col.ni)[:, None]
+        src = RateSource(rnet, c9, use_lowT_limit_rates=bool(i % 2), remove_list=[3] if i else None)
+        cfg = SimpleNamespace(use_lowT_limit_rates=src.use_lowT_limit_rates,
+                              remove_list=src.remove_list)
+        k = rates.build_rate_array(cfg, rnet, col.atm, c9).cpu().numpy()
+        a = synth.Column(**{**col.__dict__, "k": None})
+        a.rate_source = src
+        with_src.append(a)
+        with_k.append(synth.Column(**{**col.__dict__, "k": k}))
+    return with_src, with_k
+
+
+def test_requests_carrying_rate_sources():
+    """A request that carries its rate network has its k built on the device
+    in its slot from its own T and M; it is served exactly like the same
+    request with k built beforehand."""
+    from vulcan_amd.outer_loop import RunConfig
+    from vulcan_amd.serving import ServingEngine
+    with_src, with_k = _rate_source_requests()
+    ref = [_alone(c) for c in with_k]
+    eng = ServingEngine(hbm_budget_bytes=1e9, max_slots=2, cfg=RunConfig(count_max=1500),
+                        chunk=25, use_graph=True)
+    out = eng.run(with_src, realtime=False)
+    assert sorted(r.index for r in out) == list(range(len(with_src)))
+    for r in out:
+        y0, att0, st0 = ref[r.index]
+        print(f"req {r.index}: attempts {r.attempts} (k given: {att0}), status {r.status} ({st0})")
+        assert r.attempts == att0 and r.status == st0
+        assert np.array_equal(r.y, y0)
