@@ -194,6 +194,8 @@ typedef struct {
   double rtol, dt_var_min, dt_var_max, dt_min, dt_max, atol, mtol, pos_cut, nega_cut, loss_eps;
   double trun_min, conv_tol, flux_cri;
   int32_t batch_max_retries, count_min, count_max, na, zero_bot_row, photo_frq;
+  int32_t update_frq; /* > 0: flag an atmosphere refresh after accepted steps whose
+                         pre-increment count is a multiple (outer_loop.py:813-820) */
 } vk_ss_params;
 
 /* Device state of a batch of runs.  State-shaped arrays at off_s; compo is
@@ -208,6 +210,7 @@ typedef struct {
   double *dt, *t, *delta, *rel;
   int32_t *accept, *retry, *nega, *loss, *dcount, *status, *attempts, *photo_due;
   double* aflux_change;
+  int32_t* refresh_due; /* (ncols) or NULL: set by the controller (update_frq) */
 } vk_ss_state;
 
 /* ymix, y_prev, atom_ini, zeroed counters and status; y and dt must be set. */
@@ -221,6 +224,37 @@ int vk_ss_init_range_f64(vk_batch b, int first, int count, const double* atm,
                          const vk_ss_params* p, const vk_ss_state* s, void* stream);
 int vk_ss_attempt_f64(vk_batch b, const double* k, const double* atm, const vk_ss_params* p,
                       const vk_ss_state* s, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- atmosphere refresh (atm_refresh.py) -----------------------------------
+ * Replaces atm_refresh.update_mu_dz_jax (atm_refresh.py:36-98): mean
+ * molecular mass, the hydrostatic column integrated up and down from the
+ * reference layer with g = gs (Rp / (Rp + z))^2, and the interface averages;
+ * and update_phi_esc_jax (atm_refresh.py:101-122): diffusion-limited escape
+ * at the top floored at -max_flux.  Each column has a refresh record
+ * (layout from vk_refresh_record_layout: pico (nz+1), mol_mass (ni), escape
+ * mask (ni, 1.0 / 0.0), then pref_indx, zco_pref, gs, Rp, max_flux, kb,
+ * Navo) at off_r[c], and geometry outputs at off_g[c]: mu, g, Hp, dz (nz
+ * each) and zco (nz+1).  The atmosphere record's g, dzi, Hpi and the
+ * escaping species' top_flux are rewritten in place (the next step reads
+ * them, outer_loop.py:215-216).
+ * mode bit 1: geometry from the mixing ratios; bit 2: escape fluxes (from
+ * the geometry's top g and Hp).  With sol non-NULL (the in-loop form) the
+ * mixing ratios and the top-layer densities are those of the controller's
+ * clip of the step output sol against ymix (outer_loop.py:196-224, p's cuts),
+ * as the reference refreshes from the clipped state; otherwise ymix and y
+ * are used as given.  With r->due non-NULL only flagged columns refresh and
+ * their flags are cleared. */
+typedef struct {
+  const double* rec;
+  const int64_t* off_r; /* (ncols) device */
+  double* geo;
+  const int64_t* off_g; /* (ncols) device */
+  int32_t* due;         /* (ncols) or NULL */
+} vk_refresh_state;
+int64_t vk_refresh_record_layout(int nz, int ni, int64_t* off /* 4 */);
+int vk_atm_refresh_f64(vk_batch b, int mode, const double* ymix, const double* y,
+                       const double* sol, const vk_ss_params* p, double* atm,
+                       const vk_refresh_state* r, void* stream);
 
 /* ---- photolysis (photo.py) -----------------------------------------------
  * One cross-section / stellar-flux setup shared by every column of a call.

@@ -1056,6 +1056,9 @@ int vk_tune_get(int knob, int sl) {
 int vk_launch_rates_eval(const VkRatesDev& R, int what, int nz, const double* T, const double* M,
                          int lowt_on, const double* in_a, const double* in_b, double* out,
                          hipStream_t stream);
+int vk_launch_atm_refresh(const VkCol* cols, int ncols, int max_nz, int mode, const double* ymix,
+                          const double* y, const double* sol, const vk_ss_params& p, double* atm,
+                          const vk_refresh_state& r, hipStream_t stream);
 int vk_launch_rates_vjp(const VkRatesDev& R, int nz, const double* T, const double* M,
                         int lowt_on, const double* kbar, double* Tbar, double* Mbar,
                         hipStream_t stream);
@@ -1235,4 +1238,40 @@ int vk_rates_build_batch_f64(vk_batch b, int lowt_on, const double* atm, double*
   }
   return vk_launch_rates_build_batch(b->d_cols, b->ncols, b->max_ni, b->max_nz, b->d_rates, atm,
                                      lowt_on, k, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// atmosphere refresh (vk_refresh.hip)
+// ---------------------------------------------------------------------------
+
+int64_t vk_refresh_record_layout(int nz, int ni, int64_t* off) {
+  if (nz < 2 || ni < 1) return -1;
+  const VkRefOff o = vk_refresh_offsets(nz, ni);
+  if (off) {
+    off[0] = o.pico;
+    off[1] = o.mass;
+    off[2] = o.esc;
+    off[3] = o.scal;
+  }
+  return o.total;
+}
+
+int vk_atm_refresh_f64(vk_batch b, int mode, const double* ymix, const double* y,
+                       const double* sol, const vk_ss_params* p, double* atm,
+                       const vk_refresh_state* r, void* stream) {
+  g_err[0] = 0;
+  if (!b || mode < 1 || mode > 3 || !atm || !r || !r->rec || !r->off_r || !r->geo ||
+      !r->off_g) {
+    vk_set_error("vk_atm_refresh_f64: bad arguments (mode=%d)", mode);
+    return VK_ERR_ARG;
+  }
+  if (sol ? (!ymix || !p) : (((mode & 1) && !ymix) || ((mode & 2) && !y))) {
+    vk_set_error("vk_atm_refresh_f64: %s", sol ? "the in-loop form needs ymix and the params"
+                                               : "mode needs ymix (bit 1) / y (bit 2)");
+    return VK_ERR_ARG;
+  }
+  vk_ss_params pp{};
+  if (p) pp = *p;
+  return vk_launch_atm_refresh(b->d_cols, b->ncols, b->max_nz, mode, ymix, y, sol, pp, atm, *r,
+                               (hipStream_t)stream);
 }

@@ -154,6 +154,27 @@ struct VkRatesDev {
   const uint8_t* zero_rows; // (nr+1) or null
 };
 
+// Refresh-record offsets of one column (vk_refresh_record_layout).
+struct VkRefOff {
+  int64_t pico, mass, esc, scal, total;
+};
+__host__ __device__ inline VkRefOff vk_refresh_offsets(int nz, int ni) {
+  VkRefOff o;
+  o.pico = 0;
+  o.mass = nz + 1;
+  o.esc = o.mass + ni;
+  o.scal = o.esc + ni;
+  o.total = o.scal + 7;
+  return o;
+}
+
+// The controller's clip of a step output (outer_loop.py:196-224).
+__device__ inline double vk_clip_val(double v, double ymix_old, const vk_ss_params& p) {
+  double c = (v < p.pos_cut && v >= p.nega_cut) ? 0.0 : v;
+  if (ymix_old < p.mtol && c < 0.0) c = 0.0;
+  return c;
+}
+
 // One column of a batch.
 struct VkCol {
   int32_t nz, ni, S, net;

@@ -20,9 +20,7 @@ namespace {
 constexpr int NT = VK_NT;
 
 __device__ inline double clip_val(double v, double ymix_old, const vk_ss_params& p) {
-  double c = (v < p.pos_cut && v >= p.nega_cut) ? 0.0 : v;
-  if (ymix_old < p.mtol && c < 0.0) c = 0.0;
-  return c;
+  return vk_clip_val(v, ymix_old, p);
 }
 
 template <typename T, typename Op>
@@ -214,6 +212,11 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
     if (acc >= p.count_max) st |= VK_ST_MAXSTEPS;
     if (flags & 2) st |= VK_ST_NONFINITE;
     s.status[c] = st;
+    // atmosphere refresh after this attempt (outer_loop.py:813-820): accepted
+    // or forced, counted before the increment
+    if (s.refresh_due)
+      s.refresh_due[c] = (do_accept && p.update_frq > 0 &&
+                          ((acc - 1) % p.update_frq) == 0) ? 1 : 0;
     // photolysis due before the next attempt (outer_loop.py:253-257)
     if (s.photo_due) s.photo_due[c] = (s.retry[c] == 0 && p.photo_frq > 0 &&
                                        (acc % p.photo_frq) == 0) ? 1 : 0;

@@ -58,6 +58,8 @@ _SIGS = {
     "vk_rates_eval_f64": (i32, [p, i32, i32, p, p, i32, p, p, p, p]),
     "vk_rates_vjp_f64": (i32, [p, i32, p, p, i32, p, p, p, p]),
     "vk_batch_set_rates": (i32, [p, p, i32]),
+    "vk_refresh_record_layout": (i64, [i32, i32, p]),
+    "vk_atm_refresh_f64": (i32, [p, i32, p, p, p, p, p, p, p]),
     "vk_rates_build_batch_f64": (i32, [p, i32, p, p, p]),
 }
 

@@ -33,14 +33,15 @@ class SsParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "rtol", "dt_var_min", "dt_var_max", "dt_min", "dt_max", "atol", "mtol", "pos_cut",
         "nega_cut", "loss_eps", "trun_min", "conv_tol", "flux_cri")] + [(n, C.c_int32) for n in (
-            "batch_max_retries", "count_min", "count_max", "na", "zero_bot_row", "photo_frq")]
+            "batch_max_retries", "count_min", "count_max", "na", "zero_bot_row", "photo_frq",
+            "update_frq")]
 
 
 class SsState(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in (
         "y", "y_prev", "ymix", "sol", "delta_arr", "compo", "atom_ini", "atom_loss",
         "atom_loss_prev", "dt", "t", "delta", "rel", "accept", "retry", "nega", "loss",
-        "dcount", "status", "attempts", "photo_due", "aflux_change")]
+        "dcount", "status", "attempts", "photo_due", "aflux_change", "refresh_due")]
 
 
 @dataclass
@@ -63,6 +64,7 @@ class RunConfig:
     conv_tol: float = 1e-4
     flux_cri: float = 0.1
     photo_frq: int = 5
+    update_frq: int = 0          # atmosphere refresh cadence (vulcan_cfg.update_frq); 0: off
 
     def c_params(self, na: int, zero_bot_row: bool = False) -> SsParams:
         p = SsParams()
@@ -75,6 +77,7 @@ class RunConfig:
         p.na = int(na)
         p.zero_bot_row = int(bool(zero_bot_row))
         p.photo_frq = int(self.photo_frq)
+        p.update_frq = int(self.update_frq)
         return p
 
 
@@ -82,7 +85,11 @@ class SteadyStateRunner:
     """Batch of adaptive Ros2 runs, uneven nz / ni / networks allowed."""
 
     def __init__(self, columns: Sequence, cfg: RunConfig = RunConfig(), compo=None,
-                 photo: Optional[object] = None):
+                 photo: Optional[object] = None, refresh: Optional[Sequence] = None):
+        """refresh: one ``atm_refresh.AtmRefreshStatic`` per column; with
+        cfg.update_frq > 0 each column's geometry (g, dzi, Hpi) and escape
+        fluxes are refreshed in the loop after every update_frq-th accepted
+        step (outer_loop.py:809-843)."""
         self.dev = device()
         self.cfg = cfg
         cols = list(columns)
@@ -126,12 +133,17 @@ class SteadyStateRunner:
         if photo is not None or any(getattr(c, "photo", None) is not None for c in cols):
             from .photo import PhotoBatch
             self.photo = PhotoBatch(self, cols)
+        self.refresh = None
+        if refresh is not None and cfg.update_frq > 0:
+            from .atm_refresh import RefreshSet
+            self.refresh = RefreshSet(b, refresh)
         self.state = SsState(*[t.data_ptr() for t in (
             self.y, self.y_prev, self.ymix, self.sol, self.delta_arr, self.compo, self.atom_ini,
             self.atom_loss, self.atom_loss_prev, self.dt, self.t, self.delta, self.rel,
             self.accept, self.retry, self.nega, self.loss, self.dcount, self.status,
             self.attempts, self.photo_due)], None if self.photo is None
-            else self.photo.state.aflux_change.data_ptr())
+            else self.photo.state.aflux_change.data_ptr(),
+            None if self.refresh is None else self.refresh.due.data_ptr())
         self.graph = None
         self._init()
 
@@ -150,6 +162,11 @@ class SteadyStateRunner:
                                          C.byref(self.params), C.byref(self.state),
                                          self.ws.data_ptr(), self.ws.numel(), _lib.stream_ptr()),
                    "vk_ss_attempt_f64")
+        if self.refresh is not None:
+            # the columns the controller flagged: geometry and escape from the
+            # clipped step output, read by the next attempt
+            self.refresh.run(self.atm, mode=3, ymix=self.ymix, sol=self.sol, params=self.params,
+                             use_due=True)
 
     def capture(self, n_attempts: int):
         """Capture n attempts in one HIP graph (replayed by run())."""
@@ -172,6 +189,9 @@ class SteadyStateRunner:
         self.y.copy_(b.pack_state([c.y for c in self.columns], self.dev))
         self.dt.copy_(torch.tensor([float(c.dt) for c in self.columns], dtype=F64))
         self.k.copy_(b.pack_k([c.k for c in self.columns], self.dev))
+        if self.refresh is not None:
+            self.atm.copy_(b.pack_atm([c.atm for c in self.columns], self.dev))
+            self.refresh.due.zero_()
         if self.photo is not None:
             self.photo.reset()
         self._init()
