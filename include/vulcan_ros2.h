@@ -211,7 +211,16 @@ typedef struct {
   int32_t *accept, *retry, *nega, *loss, *dcount, *status, *attempts, *photo_due;
   double* aflux_change;
   int32_t* refresh_due; /* (ncols) or NULL: set by the controller (update_frq) */
+  /* Per-column constraints on accepted states after the hydrostatic balance
+   * (outer_loop.py:855-869), or NULL.  Record at off_pin[c] (device int64):
+   * flags (VK_PIN_*), e_idx, charge (ni; 0 at e_idx), bottom densities (ni),
+   * bottom mixing ratios of pinned species (ni; NaN where not pinned). */
+  const double* pin;
+  const int64_t* off_pin;
 } vk_ss_state;
+#define VK_PIN_CHARGE 1  /* e = -sum_i charge_i y_i in every layer */
+#define VK_PIN_ALL_BOT 2 /* bottom layer = the bottom densities */
+#define VK_PIN_SP_BOT 4  /* bottom layer of pinned species = mix * n0[0] */
 
 /* ymix, y_prev, atom_ini, zeroed counters and status; y and dt must be set. */
 int vk_ss_init_f64(vk_batch b, const double* atm, const vk_ss_params* p, const vk_ss_state* s,

@@ -118,6 +118,9 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
   double relmax = 0.0;
   double at_b[VK_MAX_ATOMS];
   for (int a = 0; a < VK_MAX_ATOMS; ++a) at_b[a] = 0.0;
+  // per-column constraints on accepted states (outer_loop.py:855-869)
+  const double* pin = s.pin ? s.pin + s.off_pin[c] : nullptr;
+  const int pflags = pin ? (int)pin[0] : 0;
   for (int z = tid; z < nz; z += NT) {
     double ys = 0.0;
     for (int i = 0; i < ni; ++i) {
@@ -127,12 +130,32 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
     }
     double ysb = 0.0;
     if (accept) {
+      // hydrostatic balance of the gas species (outer_loop.py:847-853); the
+      // other species keep their clipped value.  Written to y first so the
+      // constraints can act on the balanced layer.
       for (int i = 0; i < ni; ++i) {
         const int64_t e = (int64_t)z * ni + i;
         const double sc = clip_val(sol[e], ymix[e], p);
-        const double sb = n0[z] * (sc / ys);
-        if (gas[i] != 0.0) ysb += sb;
+        y[e] = (gas[i] != 0.0) ? n0[z] * (sc / ys) : sc;
       }
+      if (pflags) {
+        const int64_t row = (int64_t)z * ni;
+        if (pflags & VK_PIN_CHARGE) {   // e = -sum_i charge_i y_i (outer_loop.py:855-860)
+          const int eidx = (int)pin[1];
+          double q = 0.0;
+          for (int i = 0; i < ni; ++i) q = q + y[row + i] * pin[2 + i];
+          if (eidx >= 0 && eidx < ni) y[row + eidx] = -q;
+        }
+        if (z == 0 && (pflags & VK_PIN_ALL_BOT))   // outer_loop.py:864-865
+          for (int i = 0; i < ni; ++i) y[i] = pin[2 + ni + i];
+        if (z == 0 && (pflags & VK_PIN_SP_BOT))    // outer_loop.py:866-869
+          for (int i = 0; i < ni; ++i) {
+            const double m = pin[2 + 2 * ni + i];
+            if (m == m) y[i] = m * n0[0];
+          }
+      }
+      for (int i = 0; i < ni; ++i)
+        if (gas[i] != 0.0) ysb += y[(int64_t)z * ni + i];
     }
     for (int i = 0; i < ni; ++i) {
       const int64_t e = (int64_t)z * ni + i;
@@ -141,8 +164,7 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
       if (force) {
         ynew = yprev < 0.0 ? 0.0 : yprev;
       } else if (accept) {
-        const double sc = clip_val(sol[e], ymix[e], p);
-        ynew = n0[z] * (sc / ys);
+        ynew = y[e];
         mixnew = ynew / ysb;
         for (int a = 0; a < na; ++a) at_b[a] += compo[i * na + a] * ynew;
       } else {

@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass, field
-from typing import Optional, Sequence
+from typing import NamedTuple, Optional, Sequence
 
 import numpy as np
 import torch
@@ -41,7 +41,49 @@ class SsState(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in (
         "y", "y_prev", "ymix", "sol", "delta_arr", "compo", "atom_ini", "atom_loss",
         "atom_loss_prev", "dt", "t", "delta", "rel", "accept", "retry", "nega", "loss",
-        "dcount", "status", "attempts", "photo_due", "aflux_change", "refresh_due")]
+        "dcount", "status", "attempts", "photo_due", "aflux_change", "refresh_due", "pin",
+        "off_pin")]
+
+
+PIN_CHARGE, PIN_ALL_BOT, PIN_SP_BOT = 1, 2, 4
+
+
+class ColumnConstraints(NamedTuple):
+    """Constraints on every accepted state of one column, applied after the
+    hydrostatic balance (outer_loop.py:855-869): ion charge balance
+    (charge (ni) with 0 at e_idx, e = -sum_i charge_i y_i), the whole bottom
+    layer fixed (bottom_n (ni) densities), and pinned bottom mixing ratios
+    (fix_sp_bot {species index: mixing ratio}, times the bottom density)."""
+    charge: Optional[np.ndarray] = None
+    e_idx: int = -1
+    bottom_n: Optional[np.ndarray] = None
+    fix_sp_bot: Optional[dict] = None
+
+    def record(self, ni: int) -> np.ndarray:
+        flags = 0
+        rec = np.zeros(2 + 3 * ni)
+        rec[2 + 2 * ni:] = np.nan
+        if self.charge is not None:
+            if not 0 <= int(self.e_idx) < ni:
+                raise ValueError(f"e_idx {self.e_idx} outside 0..{ni - 1}")
+            q = np.asarray(self.charge, np.float64).reshape(-1)
+            if q.shape[0] != ni:
+                raise ValueError("charge needs one entry per species")
+            rec[2:2 + ni] = q
+            rec[2 + int(self.e_idx)] = 0.0
+            rec[1] = int(self.e_idx)
+            flags |= PIN_CHARGE
+        if self.bottom_n is not None:
+            rec[2 + ni:2 + 2 * ni] = np.asarray(self.bottom_n, np.float64).reshape(-1)
+            flags |= PIN_ALL_BOT
+        if self.fix_sp_bot:
+            for i, m in self.fix_sp_bot.items():
+                if not 0 <= int(i) < ni:
+                    raise ValueError(f"pinned species {i} outside 0..{ni - 1}")
+                rec[2 + 2 * ni + int(i)] = float(m)
+            flags |= PIN_SP_BOT
+        rec[0] = flags
+        return rec
 
 
 @dataclass
@@ -85,11 +127,13 @@ class SteadyStateRunner:
     """Batch of adaptive Ros2 runs, uneven nz / ni / networks allowed."""
 
     def __init__(self, columns: Sequence, cfg: RunConfig = RunConfig(), compo=None,
-                 photo: Optional[object] = None, refresh: Optional[Sequence] = None):
+                 photo: Optional[object] = None, refresh: Optional[Sequence] = None,
+                 constraints: Optional[Sequence] = None):
         """refresh: one ``atm_refresh.AtmRefreshStatic`` per column; with
         cfg.update_frq > 0 each column's geometry (g, dzi, Hpi) and escape
         fluxes are refreshed in the loop after every update_frq-th accepted
-        step (outer_loop.py:809-843)."""
+        step (outer_loop.py:809-843).  constraints: one ``ColumnConstraints``
+        (or None) per column."""
         self.dev = device()
         self.cfg = cfg
         cols = list(columns)
@@ -137,13 +181,24 @@ class SteadyStateRunner:
         if refresh is not None and cfg.update_frq > 0:
             from .atm_refresh import RefreshSet
             self.refresh = RefreshSet(b, refresh)
+        self.pin = self.off_pin = None
+        if constraints is not None:
+            if len(constraints) != b.ncols:
+                raise ValueError(f"need one constraints entry (or None) per column ({b.ncols})")
+            recs = [(cc or ColumnConstraints()).record(int(b.ni[i]))
+                    for i, cc in enumerate(constraints)]
+            offs = np.concatenate([[0], np.cumsum([r.size for r in recs])[:-1]]).astype(np.int64)
+            self.pin = torch.tensor(np.concatenate(recs), dtype=F64, device=dev)
+            self.off_pin = torch.tensor(offs, dtype=torch.int64, device=dev)
         self.state = SsState(*[t.data_ptr() for t in (
             self.y, self.y_prev, self.ymix, self.sol, self.delta_arr, self.compo, self.atom_ini,
             self.atom_loss, self.atom_loss_prev, self.dt, self.t, self.delta, self.rel,
             self.accept, self.retry, self.nega, self.loss, self.dcount, self.status,
             self.attempts, self.photo_due)], None if self.photo is None
             else self.photo.state.aflux_change.data_ptr(),
-            None if self.refresh is None else self.refresh.due.data_ptr())
+            None if self.refresh is None else self.refresh.due.data_ptr(),
+            None if self.pin is None else self.pin.data_ptr(),
+            None if self.off_pin is None else self.off_pin.data_ptr())
         self.graph = None
         self._init()
 

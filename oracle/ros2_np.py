@@ -456,6 +456,27 @@ def rel_change(y, y_prev, ymix, mtol, atol):
     return float(np.max(dy[pos] / y[pos])) if np.any(pos) else 0.0
 
 
+def balance_and_constrain(sol_clip, ymix_new, n0, gas_mask=None, charge=None, e_idx=None,
+                          bottom_n=None, fix_sp_bot=None):
+    """The accepted state: hydrostatic balance n0 * ymix_new of the gas
+    species, the others kept clipped (outer_loop.py:847-853); then the ion
+    charge balance e = -sum_i charge_i y_i (charge 0 at e_idx,
+    outer_loop.py:855-860), the fixed bottom layer (:864-865) and pinned
+    bottom mixing ratios times n0[0] (:866-869)."""
+    y = n0[:, None] * ymix_new
+    if gas_mask is not None:
+        y = np.where(gas_mask[None, :], y, sol_clip)
+    y = np.array(y, np.float64)
+    if charge is not None:
+        y[:, e_idx] = -(y * charge[None, :]).sum(axis=1)
+    if bottom_n is not None:
+        y[0] = bottom_n
+    if fix_sp_bot:
+        for i, m in fix_sp_bot.items():
+            y[0, i] = m * n0[0]
+    return y
+
+
 class ControlState(NamedTuple):
     y: np.ndarray
     y_prev: np.ndarray
