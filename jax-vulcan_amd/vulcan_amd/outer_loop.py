@@ -283,6 +283,50 @@ class SteadyStateRunner:
         return done_attempts
 
     # ------------------------------------------------------------------
+    # checkpoint / resume (the run state the reference packs and unpacks,
+    # outer_loop.py:1795-2013)
+    _STATE = ("y", "y_prev", "ymix", "sol", "delta_arr", "atom_ini", "atom_loss",
+              "atom_loss_prev", "dt", "t", "delta", "rel", "accept", "retry", "nega", "loss",
+              "dcount", "status", "attempts", "photo_due", "k", "atm")
+
+    def snapshot(self) -> dict:
+        """Every device array the next attempt reads, copied to the host
+        (NumPy), after the stream's pending work."""
+        torch.cuda.current_stream().synchronize()
+        snap = {f: getattr(self, f).detach().cpu().numpy().copy() for f in self._STATE}
+        if self.refresh is not None:
+            snap["refresh_due"] = self.refresh.due.cpu().numpy().copy()
+            snap["refresh_geo"] = self.refresh.geo.cpu().numpy().copy()
+        if self.photo is not None:
+            for name, t in self.photo.state_tensors().items():
+                snap["photo_" + name] = t.detach().cpu().numpy().copy()
+        return snap
+
+    def restore(self, snap: dict):
+        """Load a snapshot of a runner built on the same columns; the next
+        attempt continues exactly where the snapshot was taken."""
+        for f in self._STATE:
+            t = getattr(self, f)
+            a = np.asarray(snap[f])
+            if a.shape != tuple(t.shape):
+                raise ValueError(f"snapshot field {f} has shape {a.shape}, expected "
+                                 f"{tuple(t.shape)}")
+            t.copy_(torch.as_tensor(a))
+        if self.refresh is not None:
+            self.refresh.due.copy_(torch.as_tensor(snap["refresh_due"]))
+            self.refresh.geo.copy_(torch.as_tensor(snap["refresh_geo"]))
+        if self.photo is not None:
+            for name, t in self.photo.state_tensors().items():
+                t.copy_(torch.as_tensor(snap["photo_" + name]))
+
+    def save_checkpoint(self, path):
+        """snapshot() into an .npz file (no pickled objects)."""
+        np.savez(path, **self.snapshot())
+
+    def load_checkpoint(self, path):
+        with np.load(path, allow_pickle=False) as z:
+            self.restore({k: z[k] for k in z.files})
+
     def results(self):
         b = self.batch
         ys = [t.detach().cpu().numpy() for t in b.unpack_state(self.y)]

@@ -141,6 +141,11 @@ class PhotoBatch:
                   self.state.dflux_u, self.state.aflux, self.state.J, self.state.aflux_change):
             t.zero_()
 
+    def state_tensors(self) -> dict:
+        """The device arrays the next photolysis update reads or keeps."""
+        return {f: getattr(self.state, f) for f in ("tau", "w0", "sflux", "dflux_d", "dflux_u",
+                                                     "aflux", "J", "aflux_change")}
+
     def update(self):
         r = self.runner
         _lib.check(_lib.load().vk_photo_f64(r.batch.handle, 7, C.byref(self.setup.c),
