@@ -212,15 +212,19 @@ typedef struct {
   double* aflux_change;
   int32_t* refresh_due; /* (ncols) or NULL: set by the controller (update_frq) */
   /* Per-column constraints on accepted states after the hydrostatic balance
-   * (outer_loop.py:855-869), or NULL.  Record at off_pin[c] (device int64):
+   * (outer_loop.py:855-895), or NULL.  Record at off_pin[c] (device int64):
    * flags (VK_PIN_*), e_idx, charge (ni; 0 at e_idx), bottom densities (ni),
-   * bottom mixing ratios of pinned species (ni; NaN where not pinned). */
-  const double* pin;
+   * bottom mixing ratios of pinned species (ni; NaN where not pinned), then
+   * the H2/He pin: h2_idx, he_idx, pin_time, and its state (pinned 0/1,
+   * H2 and He bottom mixing ratios), which every attempt updates. */
+  double* pin;
   const int64_t* off_pin;
 } vk_ss_state;
 #define VK_PIN_CHARGE 1  /* e = -sum_i charge_i y_i in every layer */
 #define VK_PIN_ALL_BOT 2 /* bottom layer = the bottom densities */
 #define VK_PIN_SP_BOT 4  /* bottom layer of pinned species = mix * n0[0] */
+#define VK_PIN_H2HE 8    /* once t > pin_time: bottom H2 and He held at the mixing
+                            ratios of that moment (outer_loop.py:871-895) */
 
 /* ymix, y_prev, atom_ini, zeroed counters and status; y and dt must be set. */
 int vk_ss_init_f64(vk_batch b, const double* atm, const vk_ss_params* p, const vk_ss_state* s,

@@ -119,9 +119,30 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
   double at_b[VK_MAX_ATOMS];
   for (int a = 0; a < VK_MAX_ATOMS; ++a) at_b[a] = 0.0;
   // per-column constraints on accepted states (outer_loop.py:855-869)
-  const double* pin = s.pin ? s.pin + s.off_pin[c] : nullptr;
+  double* pin = s.pin ? s.pin + s.off_pin[c] : nullptr;
   const int pflags = pin ? (int)pin[0] : 0;
+  const double t_old = s.t[c];
   for (int z = tid; z < nz; z += NT) {
+    // H2/He pin (outer_loop.py:877-882): the snapshot trips on any attempt
+    // with t > pin_time, from the bottom layer's pre-step ymix
+    bool h2he_apply = false;
+    double h2m = 0.0, hem = 0.0;
+    int h2 = -1, he = -1;
+    if (z == 0 && (pflags & VK_PIN_H2HE)) {
+      double* hp = pin + 2 + 3 * ni;
+      h2 = (int)hp[0];
+      he = (int)hp[1];
+      if (h2 >= 0 && h2 < ni && he >= 0 && he < ni) {
+        const bool pinned = hp[3] != 0.0;
+        const bool trip = !pinned && t_old > hp[2];
+        h2m = trip ? ymix[h2] : hp[4];
+        hem = trip ? ymix[he] : hp[5];
+        hp[3] = (pinned || trip) ? 1.0 : 0.0;
+        hp[4] = h2m;
+        hp[5] = hem;
+        h2he_apply = pinned || trip;
+      }
+    }
     double ys = 0.0;
     for (int i = 0; i < ni; ++i) {
       const int64_t e = (int64_t)z * ni + i;
@@ -153,6 +174,10 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
             const double m = pin[2 + 2 * ni + i];
             if (m == m) y[i] = m * n0[0];
           }
+        if (h2he_apply) {                            // outer_loop.py:884-892
+          y[h2] = h2m * n0[0];
+          y[he] = hem * n0[0];
+        }
       }
       for (int i = 0; i < ni; ++i)
         if (gas[i] != 0.0) ysb += y[(int64_t)z * ni + i];

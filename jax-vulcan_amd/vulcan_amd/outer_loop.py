@@ -45,7 +45,7 @@ class SsState(C.Structure):
         "off_pin")]
 
 
-PIN_CHARGE, PIN_ALL_BOT, PIN_SP_BOT = 1, 2, 4
+PIN_CHARGE, PIN_ALL_BOT, PIN_SP_BOT, PIN_H2HE = 1, 2, 4, 8
 
 
 class ColumnConstraints(NamedTuple):
@@ -53,16 +53,26 @@ class ColumnConstraints(NamedTuple):
     hydrostatic balance (outer_loop.py:855-869): ion charge balance
     (charge (ni) with 0 at e_idx, e = -sum_i charge_i y_i), the whole bottom
     layer fixed (bottom_n (ni) densities), and pinned bottom mixing ratios
-    (fix_sp_bot {species index: mixing ratio}, times the bottom density)."""
+    (fix_sp_bot {species index: mixing ratio}, times the bottom density), and
+    the Hycean H2/He pin (h2he = (h2_idx, he_idx, pin_time): once t exceeds
+    pin_time the bottom H2 and He keep that moment's mixing ratios,
+    outer_loop.py:871-895)."""
     charge: Optional[np.ndarray] = None
     e_idx: int = -1
     bottom_n: Optional[np.ndarray] = None
     fix_sp_bot: Optional[dict] = None
+    h2he: Optional[tuple] = None
 
     def record(self, ni: int) -> np.ndarray:
         flags = 0
-        rec = np.zeros(2 + 3 * ni)
-        rec[2 + 2 * ni:] = np.nan
+        rec = np.zeros(2 + 3 * ni + 6)
+        rec[2 + 2 * ni:2 + 3 * ni] = np.nan
+        if self.h2he is not None:
+            h2, he, t_pin = self.h2he
+            if not (0 <= int(h2) < ni and 0 <= int(he) < ni):
+                raise ValueError("h2he species outside 0..ni-1")
+            rec[2 + 3 * ni:2 + 3 * ni + 3] = (int(h2), int(he), float(t_pin))
+            flags |= PIN_H2HE
         if self.charge is not None:
             if not 0 <= int(self.e_idx) < ni:
                 raise ValueError(f"e_idx {self.e_idx} outside 0..{ni - 1}")
@@ -188,7 +198,8 @@ class SteadyStateRunner:
             recs = [(cc or ColumnConstraints()).record(int(b.ni[i]))
                     for i, cc in enumerate(constraints)]
             offs = np.concatenate([[0], np.cumsum([r.size for r in recs])[:-1]]).astype(np.int64)
-            self.pin = torch.tensor(np.concatenate(recs), dtype=F64, device=dev)
+            self._pin_host = np.concatenate(recs)
+            self.pin = torch.tensor(self._pin_host, dtype=F64, device=dev)
             self.off_pin = torch.tensor(offs, dtype=torch.int64, device=dev)
         self.state = SsState(*[t.data_ptr() for t in (
             self.y, self.y_prev, self.ymix, self.sol, self.delta_arr, self.compo, self.atom_ini,
@@ -247,6 +258,8 @@ class SteadyStateRunner:
         if self.refresh is not None:
             self.atm.copy_(b.pack_atm([c.atm for c in self.columns], self.dev))
             self.refresh.due.zero_()
+        if self.pin is not None:     # the H2/He pin state starts unpinned
+            self.pin.copy_(torch.as_tensor(self._pin_host))
         if self.photo is not None:
             self.photo.reset()
         self._init()
@@ -300,6 +313,8 @@ class SteadyStateRunner:
         if self.photo is not None:
             for name, t in self.photo.state_tensors().items():
                 snap["photo_" + name] = t.detach().cpu().numpy().copy()
+        if self.pin is not None:
+            snap["pin"] = self.pin.cpu().numpy().copy()
         return snap
 
     def restore(self, snap: dict):
@@ -318,6 +333,8 @@ class SteadyStateRunner:
         if self.photo is not None:
             for name, t in self.photo.state_tensors().items():
                 t.copy_(torch.as_tensor(snap["photo_" + name]))
+        if self.pin is not None:
+            self.pin.copy_(torch.as_tensor(snap["pin"]))
 
     def save_checkpoint(self, path):
         """snapshot() into an .npz file (no pickled objects)."""

@@ -456,13 +456,24 @@ def rel_change(y, y_prev, ymix, mtol, atol):
     return float(np.max(dy[pos] / y[pos])) if np.any(pos) else 0.0
 
 
+def h2he_pin_step(pinned, mixes, t_old, ymix_old, h2_idx, he_idx, pin_time):
+    """The H2/He pin state after one attempt (outer_loop.py:877-882): the
+    snapshot trips on any attempt with t > pin_time, from the bottom layer's
+    pre-step mixing ratios.  Returns (pinned, (h2_mix, he_mix))."""
+    trip = (not pinned) and t_old > pin_time
+    if trip:
+        mixes = (float(ymix_old[0, h2_idx]), float(ymix_old[0, he_idx]))
+    return pinned or trip, mixes
+
+
 def balance_and_constrain(sol_clip, ymix_new, n0, gas_mask=None, charge=None, e_idx=None,
-                          bottom_n=None, fix_sp_bot=None):
+                          bottom_n=None, fix_sp_bot=None, h2he=None):
     """The accepted state: hydrostatic balance n0 * ymix_new of the gas
     species, the others kept clipped (outer_loop.py:847-853); then the ion
     charge balance e = -sum_i charge_i y_i (charge 0 at e_idx,
-    outer_loop.py:855-860), the fixed bottom layer (:864-865) and pinned
-    bottom mixing ratios times n0[0] (:866-869)."""
+    outer_loop.py:855-860), the fixed bottom layer (:864-865), pinned
+    bottom mixing ratios times n0[0] (:866-869) and, once pinned, the H2/He
+    bottom mixing ratios (:884-892)."""
     y = n0[:, None] * ymix_new
     if gas_mask is not None:
         y = np.where(gas_mask[None, :], y, sol_clip)
@@ -474,6 +485,10 @@ def balance_and_constrain(sol_clip, ymix_new, n0, gas_mask=None, charge=None, e_
     if fix_sp_bot:
         for i, m in fix_sp_bot.items():
             y[0, i] = m * n0[0]
+    if h2he is not None:            # (h2_idx, he_idx, h2_mix, he_mix) once pinned
+        h2, he, mh2, mhe = h2he
+        y[0, h2] = mh2 * n0[0]
+        y[0, he] = mhe * n0[0]
     return y
 
 
