@@ -188,9 +188,22 @@ class SteadyStateRunner:
             from .photo import PhotoBatch
             self.photo = PhotoBatch(self, cols)
         self.refresh = None
+        self._dz_map = None
         if refresh is not None and cfg.update_frq > 0:
             from .atm_refresh import RefreshSet
             self.refresh = RefreshSet(b, refresh)
+            if self.photo is not None:
+                # the photolysis layer thicknesses follow the refreshed
+                # geometry (the reference's photo branch reads the carried
+                # dz, outer_loop.py:285, 841)
+                st = self.photo.setup
+                idx_l = np.concatenate([st.off_l_np[c] + np.arange(int(b.nz[c]))
+                                        for c in range(n)])
+                idx_g = np.concatenate([self.refresh.off_g_h[c] + 3 * int(b.nz[c]) +
+                                        np.arange(int(b.nz[c])) for c in range(n)])
+                self._dz_map = (torch.as_tensor(idx_l, device=dev),
+                                torch.as_tensor(idx_g, device=dev))
+                self.refresh.geo[self._dz_map[1]] = st.dz[self._dz_map[0]]
         self.pin = self.off_pin = None
         if constraints is not None:
             if len(constraints) != b.ncols:
@@ -233,6 +246,9 @@ class SteadyStateRunner:
             # clipped step output, read by the next attempt
             self.refresh.run(self.atm, mode=3, ymix=self.ymix, sol=self.sol, params=self.params,
                              use_due=True)
+            if self._dz_map is not None:
+                st = self.photo.setup
+                st.dz[self._dz_map[0]] = self.refresh.geo[self._dz_map[1]]
 
     def capture(self, n_attempts: int):
         """Capture n attempts in one HIP graph (replayed by run())."""
@@ -258,6 +274,11 @@ class SteadyStateRunner:
         if self.refresh is not None:
             self.atm.copy_(b.pack_atm([c.atm for c in self.columns], self.dev))
             self.refresh.due.zero_()
+            if self._dz_map is not None:
+                st = self.photo.setup
+                st.dz.copy_(torch.as_tensor(np.concatenate(
+                    [np.asarray(c.dz, np.float64) for c in self.columns])))
+                self.refresh.geo[self._dz_map[1]] = st.dz[self._dz_map[0]]
         if self.pin is not None:     # the H2/He pin state starts unpinned
             self.pin.copy_(torch.as_tensor(self._pin_host))
         if self.photo is not None:
@@ -315,6 +336,8 @@ class SteadyStateRunner:
                 snap["photo_" + name] = t.detach().cpu().numpy().copy()
         if self.pin is not None:
             snap["pin"] = self.pin.cpu().numpy().copy()
+        if self._dz_map is not None:
+            snap["photo_dz"] = self.photo.setup.dz.cpu().numpy().copy()
         return snap
 
     def restore(self, snap: dict):
@@ -335,6 +358,8 @@ class SteadyStateRunner:
                 t.copy_(torch.as_tensor(snap["photo_" + name]))
         if self.pin is not None:
             self.pin.copy_(torch.as_tensor(snap["pin"]))
+        if self._dz_map is not None:
+            self.photo.setup.dz.copy_(torch.as_tensor(snap["photo_dz"]))
 
     def save_checkpoint(self, path):
         """snapshot() into an .npz file (no pickled objects)."""

@@ -141,3 +141,32 @@ def test_in_loop_refresh_matches_oracle_and_graph_replay():
     assert torch.equal(r.accept, eager[3])
     assert torch.equal(r.y, eager[0]) and torch.equal(r.atm, eager[1])
     assert torch.equal(r.dt, eager[2])
+
+
+def test_photolysis_reads_the_refreshed_layer_thickness():
+    """With photolysis and the refresh both on, the photolysis dz follows
+    the refreshed geometry (the reference's photo branch reads the carried
+    dz, outer_loop.py:285, 841), and the photolysis rows then differ from a
+    run without the refresh."""
+    import torch
+    from test_gpu_photo import _photo_col
+    from vulcan_amd import synth
+    from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
+    c = _photo_col(seed=43, nz=20, ni=14, n_rev=40, n_photo=5, nbin=200)
+    rng = np.random.default_rng(4)
+    st = _static(rng, c.atm.Tco, synth.compute_pico(c.P), c.ni, c.atm.ms, np.zeros(c.ni), [],
+                 5)
+    r = SteadyStateRunner([c], RunConfig(update_frq=2, photo_frq=1), refresh=[st])
+    plain = SteadyStateRunner([c], RunConfig(photo_frq=1))
+    assert torch.equal(r.photo.setup.dz, plain.photo.setup.dz)
+    for _ in range(12):
+        r.attempt()
+        plain.attempt()
+    torch.cuda.synchronize()
+    geo_dz = r.refresh.geometry(0)["dz"]
+    assert torch.equal(r.photo.setup.dz, geo_dz)
+    assert not torch.equal(r.photo.setup.dz, plain.photo.setup.dz)
+    rows = torch.as_tensor(np.asarray(c.photo.branch_re_idx), device="cuda")
+    kr = r.k.view(c.k.shape[0], -1)[rows]
+    kp = plain.k.view(c.k.shape[0], -1)[rows]
+    assert not torch.equal(kr, kp)
