@@ -128,3 +128,32 @@ def test_step_conserves_elements_on_the_hd189_network():
           f"y moved by {np.max(np.abs(sol - y) / np.abs(y).max(axis=1, keepdims=True)):.2e}")
     assert np.all(np.isfinite(sol))
     assert rel.max() < 1e-9
+
+
+def test_low_temperature_caps_on_the_hd189_network():
+    """A cold column (120-420 K) with use_lowT_limit_rates: the device build
+    equals the oracle's (rtol 1e-11), and the three cap reactions (slots 665,
+    671, 673) carry the capped values where T is below each threshold
+    (rates.py:149-165)."""
+    from oracle import rates_np as R
+    from vulcan_amd import network as N, rates
+    from vulcan_amd.gibbs import load_nasa9
+    net = N.parse_network(NET, extra_species=["CH4", "O2", "He", "N2", "H2"])
+    c9, _ = load_nasa9(net.species, GOLD)
+    nz = 40
+    T = np.linspace(120.0, 420.0, nz)
+    M = np.logspace(19.0, 14.0, nz)
+    cfg = SimpleNamespace(use_lowT_limit_rates=True, remove_list=None)
+    k = _np(rates.build_rate_array(cfg, net, SimpleNamespace(Tco=T, M=M), c9))
+    with np.errstate(all="ignore"):
+        ko = R.build_rate_array(net, T, M, c9, use_lowT_limit_rates=True)
+    assert np.array_equal(k == 0.0, ko == 0.0)
+    nzr = (ko != 0.0) & np.isfinite(ko)
+    assert np.max(np.abs(k[nzr] - ko[nzr]) / np.abs(ko[nzr])) < 1e-11
+    cold = T <= 300.0
+    assert np.all(k[671][cold] == 3.7e-30) and np.all(k[671][~cold] != 3.7e-30)
+    assert np.all(k[673][T <= 200.0] == 2.49e-27)
+    kinf = 2.06e-10 * T ** -0.4
+    cap = 6.0e-29 / (1.0 + 6.0e-29 * M / kinf)
+    m = T <= 277.5
+    assert np.max(np.abs(k[665][m] - cap[m]) / cap[m]) < 1e-13

@@ -109,6 +109,10 @@ def test_to_arrays_and_rate_classification(net):
     assert t["form"][783] == rates.KF_ZERO and t["has_rev"][783] == 0
     assert t["has_rev"][1] == 1 and t["has_rev"][781] == 1
     assert t["stop_rev"] == 783
+    # the three low-T cap reactions are slots 665, 671, 673 of this file
+    assert (t["lowt"][665], t["lowt"][671], t["lowt"][673]) == (rates.LOWT_CH3, rates.LOWT_C2H4,
+                                                                 rates.LOWT_C2H5)
+    assert int((t["lowt"] != 0).sum()) == 3
 
 
 def test_malformed_equation_raises(tmp_path):
