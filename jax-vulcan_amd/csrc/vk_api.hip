@@ -932,6 +932,10 @@ static int check_ss(vk_batch b, const double* atm, const vk_ss_params* p, const 
     vk_set_error("steady-state call: na=%d outside 1..%d", p->na, VK_MAX_ATOMS);
     return VK_ERR_ARG;
   }
+  if ((s->pin == nullptr) != (s->off_pin == nullptr) || p->update_frq < 0) {
+    vk_set_error("steady-state call: pin and off_pin must be given together; update_frq >= 0");
+    return VK_ERR_ARG;
+  }
   return VK_OK;
 }
 
