@@ -196,6 +196,15 @@ typedef struct {
   int32_t batch_max_retries, count_min, count_max, na, zero_bot_row, photo_frq;
   int32_t update_frq; /* > 0: flag an atmosphere refresh after accepted steps whose
                          pre-increment count is a multiple (outer_loop.py:813-820) */
+  /* conv_mode 0: stop when the last accepted step's largest masked relative
+   * change is below conv_tol (_f_dy, outer_loop.py:2163-2184).  1: the
+   * reference's stopping rule (_conv_jax and cond_fn, outer_loop.py:636-713,
+   * 968-1003): longdy against a ring of the last conv_step accepted states at
+   * the entry nearest t * st_factor, longdydt, the strict and loose
+   * criteria with slope_min from Kzz and Hp, the stall fallback, runtime and
+   * count_max; needs the ring and conv fields of vk_ss_state. */
+  int32_t conv_mode, conv_step, conv_stall_window;
+  double st_factor, yconv_cri, slope_cri, yconv_min, mtol_conv, runtime;
 } vk_ss_params;
 
 /* Device state of a batch of runs.  State-shaped arrays at off_s; compo is
@@ -219,6 +228,20 @@ typedef struct {
    * H2 and He bottom mixing ratios), which every attempt updates. */
   double* pin;
   const int64_t* off_pin;
+  /* conv_mode 1 only: ring_y (conv_step states of each column at
+   * conv_step * off_s), ring_t (ncols, conv_step), conv (ncols, 4: longdy,
+   * longdydt, smallest longdy seen, accepted steps since it last dropped 5 %;
+   * start at 0, 0, +inf, 0), the layer scale heights Hp (nz) of column c at
+   * hp + off_hp[c] (slope_min; may point at a refresh geometry array), and the
+   * mask of species the criterion ignores (ni, nonzero = ignored;
+   * conver_ignore) at conv_ign + off_ign[c]. */
+  double* ring_y;
+  double* ring_t;
+  double* conv;
+  const double* hp;
+  const int64_t* off_hp;
+  const double* conv_ign;
+  const int64_t* off_ign;
 } vk_ss_state;
 #define VK_PIN_CHARGE 1  /* e = -sum_i charge_i y_i in every layer */
 #define VK_PIN_ALL_BOT 2 /* bottom layer = the bottom densities */

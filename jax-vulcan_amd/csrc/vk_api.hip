@@ -936,6 +936,13 @@ static int check_ss(vk_batch b, const double* atm, const vk_ss_params* p, const 
     vk_set_error("steady-state call: pin and off_pin must be given together; update_frq >= 0");
     return VK_ERR_ARG;
   }
+  if (p->conv_mode != 0 && (p->conv_mode != 1 || p->conv_step < 2 || !s->ring_y || !s->ring_t ||
+                            !s->conv || !s->hp || !s->off_hp || !s->conv_ign ||
+                            !s->off_ign)) {
+    vk_set_error("steady-state call: conv_mode %d needs conv_step >= 2 and the ring and conv "
+                 "arrays", p->conv_mode);
+    return VK_ERR_ARG;
+  }
   return VK_OK;
 }
 

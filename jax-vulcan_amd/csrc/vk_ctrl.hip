@@ -214,6 +214,65 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
   if (accept)
     for (int a = 0; a < na; ++a) at_b[a] = block_reduce(at_b[a], redd, AddD());
 
+  // ---- pass 3 (conv_mode 1): the reference's longdy (outer_loop.py:636-670)
+  __shared__ double cv_d[NT];
+  __shared__ int cv_i[NT];
+  double longdy = 0.0, slope_min = 0.0;
+  int indx = 0, slot = 0;
+  const bool ring = p.conv_mode == 1 && do_accept;
+  if (ring) {
+    const double t_new = s.t[c] + (force ? p.dt_min : dt);
+    const int acc_next = s.accept[c] + 1;
+    slot = (acc_next - 1) % p.conv_step;
+    double* rt = s.ring_t + (int64_t)c * p.conv_step;
+    // the ring entry nearest t * st_factor, never the one this step fills
+    const double target = t_new * p.st_factor;
+    double bd = INFINITY;
+    int bi = 0x7fffffff;
+    for (int i = tid; i < p.conv_step; i += NT) {
+      const double d = (i == slot) ? INFINITY : fabs(rt[i] - target);
+      if (d < bd || (d == bd && i < bi)) { bd = d; bi = i; }
+    }
+    cv_d[tid] = bd;
+    cv_i[tid] = bi;
+    __syncthreads();
+    if (tid == 0) {
+      double d0 = cv_d[0];
+      int i0 = cv_i[0];
+      for (int w = 1; w < NT; ++w)
+        if (cv_d[w] < d0 || (cv_d[w] == d0 && cv_i[w] < i0)) { d0 = cv_d[w]; i0 = cv_i[w]; }
+      cv_i[0] = (i0 == 0x7fffffff) ? 0 : i0;
+    }
+    __syncthreads();
+    indx = cv_i[0];
+    const double* Hp = s.hp + s.off_hp[c];
+    const double* ign = s.conv_ign + s.off_ign[c];
+    const double* Kzz = atm + col.off_a + ao.Kzz;
+    double* ry = s.ring_y + (int64_t)p.conv_step * col.off_s;
+    const double* yold = ry + (int64_t)indx * nz * ni;
+    double* ynew_slot = ry + (int64_t)slot * nz * ni;
+    double lmax = 0.0, smin = INFINITY;
+    for (int z = tid; z < nz; z += NT) {
+      for (int i = 0; i < ni; ++i) {
+        const int64_t e = (int64_t)z * ni + i;
+        const double yv = y[e], mv = ymix[e];
+        double ld = fabs((yv - yold[e]) / n0[z]);
+        if (mv < p.mtol_conv || yv < p.atol || ign[i] != 0.0) ld = 0.0;
+        const double r = (mv > 0.0) ? ld / fmax(mv, 1e-300) : 0.0;
+        lmax = r > lmax ? r : lmax;
+        ynew_slot[e] = yv;
+      }
+      if (z + 1 < nz) {
+        const double h = 0.1 * Hp[z];
+        const double q = Kzz[z] / (h * h);
+        smin = q < smin ? q : smin;
+      }
+    }
+    longdy = block_reduce(lmax, redd, MaxD());
+    smin = -block_reduce(-smin, redd, MaxD());
+    slope_min = fmax(fmin(smin, 1e-8), 1e-10);   // outer_loop.py:679-683
+  }
+
   if (tid == 0) {
     // atom loss bookkeeping (outer_loop.py:898, 914-917)
     if (accept) {
@@ -250,13 +309,42 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
     }
     s.dt[c] = dt_new;
     int st = s.status[c];
-    if (do_accept) {
-      s.rel[c] = relmax;
+    const bool flux_ok = !s.aflux_change || s.aflux_change[c] < p.flux_cri;
+    if (p.conv_mode == 1) {
+      double* cv = s.conv + (int64_t)c * 4;
+      if (ring) {
+        double* rt = s.ring_t + (int64_t)c * p.conv_step;
+        const double ldt = longdy / fmax(t_new - rt[indx], 1e-300);
+        rt[slot] = t_new;
+        cv[0] = longdy;
+        cv[1] = ldt;
+        // stall bookkeeping: a 5 % drop is a new minimum (outer_loop.py:986-1003)
+        if (longdy < cv[2] * 0.95) {
+          cv[2] = longdy;
+          cv[3] = 0.0;
+        } else {
+          cv[3] += 1.0;
+        }
+        s.rel[c] = longdy;
+      }
+      // cond_fn (outer_loop.py:672-713), on the carried longdy / longdydt
+      const double ld = cv[0], ldt = cv[1];
+      bool conv = ((ld < p.yconv_cri) && (ldt < p.slope_cri)) ||
+                  ((ld < p.yconv_min) && (ldt < slope_min));
+      conv = conv && flux_ok;
+      const bool stalled = (cv[3] > (double)p.conv_stall_window) && (cv[2] < p.yconv_min) &&
+                           (ld < p.yconv_min) && flux_ok;
       const bool ready = (t_new > p.trun_min) && (acc > p.count_min);
-      const bool flux_ok = !s.aflux_change || s.aflux_change[c] < p.flux_cri;
-      if (ready && relmax < p.conv_tol && flux_ok) st |= VK_ST_CONVERGED;
+      if (do_accept && ready && (conv || stalled)) st |= VK_ST_CONVERGED;
+      if (acc > p.count_max || t_new > p.runtime) st |= VK_ST_MAXSTEPS;
+    } else {
+      if (do_accept) {
+        s.rel[c] = relmax;
+        const bool ready = (t_new > p.trun_min) && (acc > p.count_min);
+        if (ready && relmax < p.conv_tol && flux_ok) st |= VK_ST_CONVERGED;
+      }
+      if (acc >= p.count_max) st |= VK_ST_MAXSTEPS;
     }
-    if (acc >= p.count_max) st |= VK_ST_MAXSTEPS;
     if (flags & 2) st |= VK_ST_NONFINITE;
     s.status[c] = st;
     // atmosphere refresh after this attempt (outer_loop.py:813-820): accepted
@@ -299,6 +387,20 @@ __global__ __launch_bounds__(NT) void k_ss_init(const VkCol* __restrict__ cols, 
     }
   }
   for (int a = 0; a < na; ++a) at[a] = block_reduce(at[a], redd, AddD());
+  if (p.conv_mode == 1) {
+    // empty ring and longdy state (outer_loop.py:1777-1786)
+    double* ry = s.ring_y + (int64_t)p.conv_step * col.off_s;
+    const int64_t n = (int64_t)p.conv_step * nz * ni;
+    for (int64_t e = tid; e < n; e += NT) ry[e] = 0.0;
+    for (int i = tid; i < p.conv_step; i += NT) s.ring_t[(int64_t)c * p.conv_step + i] = 0.0;
+    if (tid == 0) {
+      double* cv = s.conv + (int64_t)c * 4;
+      cv[0] = 0.0;
+      cv[1] = 0.0;
+      cv[2] = INFINITY;
+      cv[3] = 0.0;
+    }
+  }
   if (tid == 0) {
     for (int a = 0; a < VK_MAX_ATOMS; ++a) {
       const double v = a < na ? at[a] : 1.0;

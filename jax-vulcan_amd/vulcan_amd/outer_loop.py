@@ -34,7 +34,9 @@ class SsParams(C.Structure):
         "rtol", "dt_var_min", "dt_var_max", "dt_min", "dt_max", "atol", "mtol", "pos_cut",
         "nega_cut", "loss_eps", "trun_min", "conv_tol", "flux_cri")] + [(n, C.c_int32) for n in (
             "batch_max_retries", "count_min", "count_max", "na", "zero_bot_row", "photo_frq",
-            "update_frq")]
+            "update_frq", "conv_mode", "conv_step", "conv_stall_window")] + [
+                (n, C.c_double) for n in ("st_factor", "yconv_cri", "slope_cri", "yconv_min",
+                                          "mtol_conv", "runtime")]
 
 
 class SsState(C.Structure):
@@ -42,7 +44,7 @@ class SsState(C.Structure):
         "y", "y_prev", "ymix", "sol", "delta_arr", "compo", "atom_ini", "atom_loss",
         "atom_loss_prev", "dt", "t", "delta", "rel", "accept", "retry", "nega", "loss",
         "dcount", "status", "attempts", "photo_due", "aflux_change", "refresh_due", "pin",
-        "off_pin")]
+        "off_pin", "ring_y", "ring_t", "conv", "hp", "off_hp", "conv_ign", "off_ign")]
 
 
 PIN_CHARGE, PIN_ALL_BOT, PIN_SP_BOT, PIN_H2HE = 1, 2, 4, 8
@@ -96,9 +98,26 @@ class ColumnConstraints(NamedTuple):
         return rec
 
 
+class ColumnConv(NamedTuple):
+    """What the reference's stopping rule reads of one column besides its
+    state: the layer scale heights Hp (nz; slope_min, outer_loop.py:676-683)
+    and the species longdy ignores (conver_ignore, indices)."""
+    Hp: np.ndarray
+    ignore: Sequence[int] = ()
+
+
 @dataclass
 class RunConfig:
-    """Solver tolerances (vulcan_cfg.py:125-152) and the stopping rule."""
+    """Solver tolerances (vulcan_cfg.py:125-152) and the stopping rule.
+
+    conv_mode "relmax": a column is done when the largest masked relative
+    change of an accepted step falls below conv_tol.  "longdy": the
+    reference's rule (outer_loop.py:636-713, 918-1003): longdy against a
+    ring of the last conv_step accepted states at the entry nearest
+    t * st_factor, the strict (yconv_cri, slope_cri) and loose (yconv_min,
+    slope_min) criteria, the stall fallback after conv_stall_window steps,
+    and the runtime / count_max caps (count_max as accept_count > count_max).
+    """
     rtol: float = 0.2
     dt_var_min: float = 0.5
     dt_var_max: float = 2.0
@@ -117,6 +136,15 @@ class RunConfig:
     flux_cri: float = 0.1
     photo_frq: int = 5
     update_frq: int = 0          # atmosphere refresh cadence (vulcan_cfg.update_frq); 0: off
+    conv_mode: str = "relmax"
+    conv_step: int = 500         # vulcan_cfg.py:120-122, 131-147
+    st_factor: float = 0.5
+    conv_stall_window: int = 200
+    yconv_cri: float = 0.01
+    slope_cri: float = 1e-4
+    yconv_min: float = 0.1
+    mtol_conv: float = 1e-20
+    runtime: float = 1e22
 
     def c_params(self, na: int, zero_bot_row: bool = False) -> SsParams:
         p = SsParams()
@@ -130,6 +158,13 @@ class RunConfig:
         p.zero_bot_row = int(bool(zero_bot_row))
         p.photo_frq = int(self.photo_frq)
         p.update_frq = int(self.update_frq)
+        if self.conv_mode not in ("relmax", "longdy"):
+            raise ValueError(f"conv_mode must be 'relmax' or 'longdy', not {self.conv_mode!r}")
+        p.conv_mode = 1 if self.conv_mode == "longdy" else 0
+        p.conv_step = int(self.conv_step)
+        p.conv_stall_window = int(self.conv_stall_window)
+        for f in ("st_factor", "yconv_cri", "slope_cri", "yconv_min", "mtol_conv", "runtime"):
+            setattr(p, f, float(getattr(self, f)))
         return p
 
 
@@ -138,12 +173,14 @@ class SteadyStateRunner:
 
     def __init__(self, columns: Sequence, cfg: RunConfig = RunConfig(), compo=None,
                  photo: Optional[object] = None, refresh: Optional[Sequence] = None,
-                 constraints: Optional[Sequence] = None):
+                 constraints: Optional[Sequence] = None, conv: Optional[Sequence] = None):
         """refresh: one ``atm_refresh.AtmRefreshStatic`` per column; with
         cfg.update_frq > 0 each column's geometry (g, dzi, Hpi) and escape
         fluxes are refreshed in the loop after every update_frq-th accepted
         step (outer_loop.py:809-843).  constraints: one ``ColumnConstraints``
-        (or None) per column."""
+        (or None) per column.  conv: one ``ColumnConv`` per column, needed
+        by cfg.conv_mode "longdy" (with the refresh on, slope_min follows
+        the refreshed Hp)."""
         self.dev = device()
         self.cfg = cfg
         cols = list(columns)
@@ -214,6 +251,10 @@ class SteadyStateRunner:
             self._pin_host = np.concatenate(recs)
             self.pin = torch.tensor(self._pin_host, dtype=F64, device=dev)
             self.off_pin = torch.tensor(offs, dtype=torch.int64, device=dev)
+        self.ring_y = self.ring_t = self.conv = None
+        self._conv_ptrs = (None,) * 7
+        if cfg.conv_mode == "longdy":
+            self._setup_conv(conv)
         self.state = SsState(*[t.data_ptr() for t in (
             self.y, self.y_prev, self.ymix, self.sol, self.delta_arr, self.compo, self.atom_ini,
             self.atom_loss, self.atom_loss_prev, self.dt, self.t, self.delta, self.rel,
@@ -222,9 +263,72 @@ class SteadyStateRunner:
             else self.photo.state.aflux_change.data_ptr(),
             None if self.refresh is None else self.refresh.due.data_ptr(),
             None if self.pin is None else self.pin.data_ptr(),
-            None if self.off_pin is None else self.off_pin.data_ptr())
+            None if self.off_pin is None else self.off_pin.data_ptr(), *self._conv_ptrs)
         self.graph = None
         self._init()
+
+    def _setup_conv(self, conv):
+        b, dev, cfg = self.batch, self.dev, self.cfg
+        if conv is None or len(conv) != b.ncols:
+            raise ValueError(f"conv_mode 'longdy' needs one ColumnConv per column ({b.ncols})")
+        if cfg.conv_step < 2:
+            raise ValueError("conv_step must be at least 2")
+        hp, ign = [], []
+        for c, cc in enumerate(conv):
+            h = np.asarray(cc.Hp, np.float64).reshape(-1)
+            if h.shape[0] != int(b.nz[c]):
+                raise ValueError(f"column {c}: Hp has {h.shape[0]} entries, expected {b.nz[c]}")
+            m = np.zeros(int(b.ni[c]))
+            idx = np.asarray(list(cc.ignore), np.int64)
+            if idx.size and (idx.min() < 0 or idx.max() >= b.ni[c]):
+                raise ValueError(f"column {c}: conver_ignore index out of range")
+            m[idx] = 1.0
+            hp.append(h)
+            ign.append(m)
+        ex = lambda a: np.concatenate([[0], np.cumsum([x.size for x in a])[:-1]]).astype(np.int64)
+        self._hp_host = hp
+        self.conv_ign = torch.tensor(np.concatenate(ign), dtype=F64, device=dev)
+        self.off_ign = torch.tensor(ex(ign), dtype=torch.int64, device=dev)
+        if self.refresh is not None:
+            # Hp lives in the refresh geometry (mu, g, Hp, dz, zco per column)
+            self.hp = self.refresh.geo
+            off = self.refresh.off_g_h + 2 * b.nz.astype(np.int64)
+            self._hp_idx = torch.as_tensor(np.concatenate(
+                [off[c] + np.arange(int(b.nz[c])) for c in range(b.ncols)]), device=dev)
+            self.hp[self._hp_idx] = torch.as_tensor(np.concatenate(hp), device=dev)
+        else:
+            self.hp = torch.tensor(np.concatenate(hp), dtype=F64, device=dev)
+            off = ex(hp)
+            self._hp_idx = None
+        self.off_hp = torch.tensor(off, dtype=torch.int64, device=dev)
+        n = int(cfg.conv_step)
+        self.ring_y = torch.zeros(n * b.state_len, dtype=F64, device=dev)
+        self.ring_t = torch.zeros(b.ncols * n, dtype=F64, device=dev)
+        self.conv = torch.zeros(b.ncols * 4, dtype=F64, device=dev)
+        self._conv_ptrs = tuple(t.data_ptr() for t in (
+            self.ring_y, self.ring_t, self.conv, self.hp, self.off_hp, self.conv_ign,
+            self.off_ign))
+
+    def history(self, c: int):
+        """Column c's ring in time order, (y_time (n, nz, ni), t_time (n,)),
+        n = min(accept_count, conv_step): the reference's ring unpacking."""
+        if self.ring_y is None:
+            raise RuntimeError("history() needs conv_mode 'longdy'")
+        b, n = self.batch, int(self.cfg.conv_step)
+        acc = int(self.accept[c].item())
+        nz, ni = int(b.nz[c]), int(b.ni[c])
+        ry = self.ring_y[n * int(b.off_s[c]):n * int(b.off_s[c]) + n * nz * ni].view(n, nz, ni)
+        rt = self.ring_t[c * n:(c + 1) * n]
+        start = acc % n if acc >= n else 0
+        order = torch.as_tensor([(start + i) % n for i in range(min(acc, n))], device=self.dev)
+        return ry[order].cpu().numpy(), rt[order].cpu().numpy()
+
+    def conv_state(self):
+        """(ncols, 4): longdy, longdydt, smallest longdy seen, accepted
+        steps since it last dropped by 5 %."""
+        if self.conv is None:
+            raise RuntimeError("conv_state() needs conv_mode 'longdy'")
+        return self.conv.view(-1, 4).cpu().numpy()
 
     # ------------------------------------------------------------------
     def _init(self):
@@ -281,6 +385,8 @@ class SteadyStateRunner:
                 self.refresh.geo[self._dz_map[1]] = st.dz[self._dz_map[0]]
         if self.pin is not None:     # the H2/He pin state starts unpinned
             self.pin.copy_(torch.as_tensor(self._pin_host))
+        if self.ring_y is not None and self._hp_idx is not None:
+            self.hp[self._hp_idx] = torch.as_tensor(np.concatenate(self._hp_host), device=self.dev)
         if self.photo is not None:
             self.photo.reset()
         self._init()
@@ -338,6 +444,9 @@ class SteadyStateRunner:
             snap["pin"] = self.pin.cpu().numpy().copy()
         if self._dz_map is not None:
             snap["photo_dz"] = self.photo.setup.dz.cpu().numpy().copy()
+        if self.ring_y is not None:
+            for f in ("ring_y", "ring_t", "conv"):
+                snap[f] = getattr(self, f).cpu().numpy().copy()
         return snap
 
     def restore(self, snap: dict):
@@ -360,6 +469,9 @@ class SteadyStateRunner:
             self.pin.copy_(torch.as_tensor(snap["pin"]))
         if self._dz_map is not None:
             self.photo.setup.dz.copy_(torch.as_tensor(snap["photo_dz"]))
+        if self.ring_y is not None:
+            for f in ("ring_y", "ring_t", "conv"):
+                getattr(self, f).copy_(torch.as_tensor(snap[f]))
 
     def save_checkpoint(self, path):
         """snapshot() into an .npz file (no pickled objects)."""

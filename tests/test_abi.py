@@ -1,8 +1,11 @@
 """The C-ABI library loads and exports every symbol include/*.h declares."""
 from __future__ import annotations
 
+import ctypes as C
 import re
 from pathlib import Path
+
+import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -78,3 +81,37 @@ def test_rates_create_rejects_bad_tables_without_gpu():
     assert create(form, bad_idx) == 1
     assert lib.vk_rates_create(nr, 0, T, 1, None, None, None, None, None, None, None, None,
                                None, None, C.byref(h)) == 1
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every structure the Python side mirrors has the header's size and
+    field offsets (compiled with the host C compiler)."""
+    import shutil
+    import subprocess
+    from vulcan_amd import atm_refresh, outer_loop, photo
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    mirrors = {"vk_ss_params": outer_loop.SsParams, "vk_ss_state": outer_loop.SsState,
+               "vk_refresh_state": atm_refresh.RefreshState,
+               "vk_photo_setup": photo.PhotoSetupC, "vk_photo_state": photo.PhotoStateC}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vulcan_ros2.h"',
+             'int main(void) {']
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                               text=True).stdout.splitlines():
+        cname, f, v = line.split()
+        got[(cname, f)] = int(v)
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)

@@ -1,0 +1,168 @@
+"""GPU: the reference's stopping rule (conv_mode "longdy", outer_loop.py:
+636-713, 918-1003) in the device controller, against oracle/conv_np.py.
+
+* In the loop, after every attempt, the host feeds the oracle's tracker the
+  state the device accepted (or kept) and compares longdy, longdydt, the
+  smallest longdy seen and the steps since it last dropped 5 % (1e-15
+  relative; the counter exactly), and the converged / capped status bits
+  against the oracle's cond_fn on the same numbers; the ring unpacked in
+  time order equals the oracle's ring.
+* The reference's own property run (tests/test_outer_loop_conv.py:1-25):
+  with count_max = 50 and count_min = 1 the column stops at accept count
+  51 with the count-cap status, the ring holds 51 states in strictly
+  increasing time with the last equal to the final state, and longdy and
+  longdydt are finite and positive.
+* Replaying the attempts from a HIP graph gives the eager ring and
+  longdy state bit for bit.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _cols(n=4):
+    from vulcan_amd import synth
+    out = []
+    for c, (nz, ni, n_rev) in enumerate(((18, 14, 40), (24, 20, 55), (16, 14, 40),
+                                         (30, 20, 55))[:n]):
+        out.append(synth.make_column(500 + c, nz=nz, ni=ni, n_rev=n_rev, dt=1e-10, p_bot=1e5,
+                                     p_top=1e-1, consistent=True))
+    return out
+
+
+def _conv(cols, ignore_first=False):
+    from vulcan_amd import synth
+    from vulcan_amd.outer_loop import ColumnConv
+    return [ColumnConv(Hp=synth.compute_geometry(c.P, c.atm.Tco)["Hp"],
+                       ignore=(0, 3) if (ignore_first and i == 0) else ())
+            for i, c in enumerate(cols)]
+
+
+def test_longdy_rule_matches_oracle_in_the_loop():
+    import torch
+    from oracle import conv_np as V
+    from vulcan_amd.outer_loop import (ST_CONVERGED, ST_DONE, ST_MAXSTEPS, RunConfig,
+                                       SteadyStateRunner)
+    cols = _cols()
+    conv = _conv(cols, ignore_first=True)
+    cfg = RunConfig(conv_mode="longdy", conv_step=8, count_min=10, trun_min=1e-9,
+                    count_max=600, yconv_cri=0.05, slope_cri=1e3)
+    r = SteadyStateRunner(cols, cfg, conv=conv)
+    b = r.batch
+    tr = []
+    for c, col in enumerate(cols):
+        ign = np.zeros(col.ni, bool)
+        ign[list(conv[c].ignore)] = True
+        tr.append((V.ConvTracker(col.nz, col.ni, cfg.conv_step, st_factor=cfg.st_factor,
+                                 yconv_cri=cfg.yconv_cri, slope_cri=cfg.slope_cri,
+                                 yconv_min=cfg.yconv_min, mtol_conv=cfg.mtol_conv,
+                                 atol=cfg.atol, conv_stall_window=cfg.conv_stall_window,
+                                 flux_cri=cfg.flux_cri, trun_min=cfg.trun_min,
+                                 count_min=cfg.count_min, count_max=cfg.count_max,
+                                 runtime=cfg.runtime), ign))
+    checked = 0
+    decided = np.zeros(len(cols), bool)
+    for it in range(700):
+        st_before = r.status.cpu().numpy().copy()
+        acc_before = r.accept.cpu().numpy().copy()
+        r.attempt()
+        torch.cuda.synchronize()
+        st = r.status.cpu().numpy()
+        acc = r.accept.cpu().numpy()
+        t = r.t.cpu().numpy()
+        ys = [x.cpu().numpy() for x in b.unpack_state(r.y)]
+        mixes = [x.cpu().numpy() for x in b.unpack_state(r.ymix)]
+        cs = r.conv_state()
+        for c, col in enumerate(cols):
+            if st_before[c] & ST_DONE:
+                continue
+            o, ign = tr[c]
+            accepted = acc[c] > acc_before[c]
+            o.update(ys[c], mixes[c], t[c], acc[c], accepted, col.atm.M, ign)
+            if not accepted:
+                continue
+            assert cs[c, 0] == pytest.approx(o.longdy, rel=1e-15, abs=0.0), (it, c)
+            assert cs[c, 1] == pytest.approx(o.longdydt, rel=1e-15, abs=0.0), (it, c)
+            assert cs[c, 2] == pytest.approx(o.seen_min, rel=1e-15, abs=0.0), (it, c)
+            assert int(cs[c, 3]) == o.since, (it, c)
+            conv_ref, cap_ref = o.terminate(t[c], acc[c], col.atm.Kzz, conv[c].Hp)
+            assert bool(st[c] & ST_CONVERGED) == conv_ref, (it, c)
+            assert bool(st[c] & ST_MAXSTEPS) == cap_ref, (it, c)
+            decided[c] |= conv_ref or cap_ref
+            checked += 1
+        if (st & ST_DONE).all():
+            break
+    for c, col in enumerate(cols):
+        yh, th = r.history(c)
+        oy, ot = tr[c][0].history(int(r.accept[c].item()))
+        assert np.array_equal(th, ot) and np.array_equal(yh, oy), c
+    res = r.results()
+    print(f"accepted attempts checked: {checked}; status {res['status']}, accept "
+          f"{res['accept']}, longdy {r.conv_state()[:, 0]}")
+    assert checked >= 200
+    assert decided.any() and (res["status"] & ST_CONVERGED).any()
+
+
+def test_count_max_run_matches_the_reference_property():
+    import torch
+    from vulcan_amd.outer_loop import ST_CONVERGED, ST_MAXSTEPS, RunConfig, SteadyStateRunner
+    cols = _cols(1)
+    cfg = RunConfig(conv_mode="longdy", count_max=50, count_min=1, conv_step=500,
+                    yconv_cri=-1.0, yconv_min=-1.0)
+    r = SteadyStateRunner(cols, cfg, conv=_conv(cols))
+    r.run(max_attempts=2000, check_every=25)
+    torch.cuda.synchronize()
+    res = r.results()
+    assert int(res["accept"][0]) == 51
+    assert res["status"][0] & ST_MAXSTEPS and not res["status"][0] & ST_CONVERGED
+    yh, th = r.history(0)
+    assert len(th) == 51 and yh.shape == (51, cols[0].nz, cols[0].ni)
+    assert np.all(np.diff(th) > 0)
+    assert th[-1] == res["t"][0] and np.array_equal(yh[-1], res["y"][0])
+    ld, ldt = r.conv_state()[0, :2]
+    assert np.isfinite(ld) and ld > 0 and np.isfinite(ldt) and ldt > 0
+
+
+def test_graph_replay_keeps_the_ring():
+    import torch
+    from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
+    cols = _cols(3)
+    cfg = RunConfig(conv_mode="longdy", conv_step=6, count_min=10, trun_min=1e-9)
+    r = SteadyStateRunner(cols, cfg, conv=_conv(cols))
+    for _ in range(40):
+        r.attempt()
+    torch.cuda.synchronize()
+    eager = [x.clone() for x in (r.ring_y, r.ring_t, r.conv, r.y, r.status)]
+    r.reset_to_initial()
+    r.capture(20)
+    for _ in range(2):
+        r.graph[0].replay()
+    torch.cuda.synchronize()
+    for a, e in zip((r.ring_y, r.ring_t, r.conv, r.y, r.status), eager):
+        assert torch.equal(a, e)
+
+
+def test_refreshed_hp_feeds_slope_min():
+    """With the refresh on, the rule's Hp is the refresh geometry's: it
+    starts at the given Hp and moves with the refreshes."""
+    import torch
+    from test_gpu_refresh import _columns
+    from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
+    cols, statics = _columns(n=2)
+    conv = _conv(cols)
+    r = SteadyStateRunner(cols, RunConfig(conv_mode="longdy", conv_step=6, update_frq=2),
+                          refresh=statics, conv=conv)
+    assert r.hp is r.refresh.geo
+    for c in range(2):
+        assert np.array_equal(r.refresh.geometry(c)["Hp"].cpu().numpy(), conv[c].Hp)
+    for _ in range(10):
+        r.attempt()
+    torch.cuda.synchronize()
+    assert int(r.accept.min().item()) >= 2
+    for c in range(2):
+        assert not np.array_equal(r.refresh.geometry(c)["Hp"].cpu().numpy(), conv[c].Hp)
+    r.reset_to_initial()
+    assert np.array_equal(r.refresh.geometry(0)["Hp"].cpu().numpy(), conv[0].Hp)
