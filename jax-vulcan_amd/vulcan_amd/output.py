@@ -10,9 +10,11 @@ the plotting palette, start_time), so downstream readers of ``.vul`` files
 find the same keys.  ``save_vul`` writes them with pickle protocol 4 as
 Output.save_out does (legacy_io.py:704-724).
 
-longdy is the device stopping rule's measure, the largest relative change
-of the last accepted step (the quantity of _f_dy, outer_loop.py:2163-2184),
-and longdydt = longdy / dt.
+longdy and longdydt are the reference's (against the ring's lookback
+state, outer_loop.py:636-670) when the runner uses conv_mode "longdy";
+under conv_mode "relmax" longdy is the largest relative change of the last
+accepted step (the quantity of _f_dy, outer_loop.py:2163-2184) and
+longdydt = longdy / dt.
 """
 from __future__ import annotations
 
@@ -49,11 +51,15 @@ def vul_dicts(runner, c: int, species: Sequence[str], atom_order: Sequence[str] 
     al = runner.atom_loss.view(-1, 8)[c, :na].cpu().numpy()
     compo = runner.compo[int(b.off_s[c]):int(b.off_s[c]) + ni * na].view(ni, na).cpu().numpy()
     asum = (y @ compo).sum(axis=0)
+    if getattr(runner, "conv", None) is not None:
+        longdy, longdydt = (float(v) for v in runner.conv_state()[c, :2])
+    else:
+        longdy = float(res["rel"][c])
+        longdydt = longdy / dt if dt > 0 else 0.0
     var = {"species": list(species), "nr": S - 1,
            "k": {i: k[i].copy() for i in range(1, S)},
            "y": y, "ymix": np.asarray(res["ymix"][c], np.float64),
-           "t": float(res["t"][c]), "dt": dt, "longdy": float(res["rel"][c]),
-           "longdydt": float(res["rel"][c]) / dt if dt > 0 else 0.0,
+           "t": float(res["t"][c]), "dt": dt, "longdy": longdy, "longdydt": longdydt,
            "y_ini": np.asarray(runner.columns[c].y if y_ini is None else y_ini, np.float64),
            "atom_ini": {a: float(ai[j]) for j, a in enumerate(atom_order)},
            "atom_sum": {a: float(asum[j]) for j, a in enumerate(atom_order)},

@@ -13,7 +13,8 @@
   increasing time with the last equal to the final state, and longdy and
   longdydt are finite and positive.
 * Replaying the attempts from a HIP graph gives the eager ring and
-  longdy state bit for bit.
+  longdy state bit for bit, and a checkpoint restored into a fresh runner
+  continues it bit for bit; the .vul writer reports the ring's longdy.
 """
 from __future__ import annotations
 
@@ -124,6 +125,9 @@ def test_count_max_run_matches_the_reference_property():
     assert th[-1] == res["t"][0] and np.array_equal(yh[-1], res["y"][0])
     ld, ldt = r.conv_state()[0, :2]
     assert np.isfinite(ld) and ld > 0 and np.isfinite(ldt) and ldt > 0
+    from vulcan_amd.output import vul_dicts
+    var, _, para = vul_dicts(r, 0, [f"S{i}" for i in range(cols[0].ni)])
+    assert var["longdy"] == ld and var["longdydt"] == ldt and para["count"] == 51
 
 
 def test_graph_replay_keeps_the_ring():
@@ -143,6 +147,17 @@ def test_graph_replay_keeps_the_ring():
     torch.cuda.synchronize()
     for a, e in zip((r.ring_y, r.ring_t, r.conv, r.y, r.status), eager):
         assert torch.equal(a, e)
+    # a checkpoint carries the ring: resume in a fresh runner, same 20 attempts
+    snap = r.snapshot()
+    for _ in range(20):
+        r.attempt()
+    fresh = SteadyStateRunner(cols, cfg, conv=_conv(cols))
+    fresh.restore(snap)
+    for _ in range(20):
+        fresh.attempt()
+    torch.cuda.synchronize()
+    for f in ("ring_y", "ring_t", "conv", "y", "status", "accept"):
+        assert torch.equal(getattr(fresh, f), getattr(r, f)), f
 
 
 def test_refreshed_hp_feeds_slope_min():
