@@ -120,3 +120,35 @@ def test_malformed_equation_raises(tmp_path):
     p.write_text("1 [ A + B = C ] 1.0 0.0 0.0\n")
     with pytest.raises(ValueError):
         N.parse_network(p, extra_species=())
+
+
+def test_oracle_chem_rhs_conserves_elements_on_the_real_network(net):
+    """The oracle's chem_rhs on the parsed network with the oracle's rates:
+    every layer's H, O, C, N and He production sums to zero against the
+    composition table, to roundoff of the summed rate magnitudes.  Pins the
+    oracle's use of the real tables (a wrong stoichiometric factor, a
+    dropped slot or a swapped reverse row breaks it)."""
+    from oracle import rates_np as R, ros2_np as O
+    from vulcan_amd.gibbs import load_nasa9
+    atoms, elements, _ = N.load_compositions(GOLD / "all_compose.txt", net.species)
+    keep = [e for e in range(atoms.shape[1]) if atoms[:, e].any()]
+    c9, _ = load_nasa9(net.species, GOLD)
+    nz = 12
+    T = np.linspace(1800.0, 700.0, nz)
+    M = np.logspace(21.0, 15.0, nz)
+    with np.errstate(all="ignore"):
+        k = R.build_rate_array(net, T, M, c9)
+    k = np.where(np.isfinite(k), k, 0.0)
+    rng = np.random.default_rng(11)
+    mix = rng.dirichlet(np.ones(net.ni), nz)
+    y = M[:, None] * mix
+    arr = N.to_arrays(net)
+    f = O.chem_rhs(y, M, k, arr)
+    # roundoff scale: every rate enters each element sum with |stoich * atoms|
+    ys = np.concatenate([y, np.ones((nz, 1))], axis=1)
+    rate = k.T * np.prod(ys[:, arr.reactant_idx] ** arr.reactant_stoich[None], axis=2)
+    rate = np.where(arr.is_three_body[None, :], rate * M[:, None], rate)
+    scale = np.abs(rate).sum(axis=1) * atoms.max()
+    bal = f @ atoms[:, keep]
+    assert np.all(np.abs(bal) / scale[:, None] < 1e-13), np.max(np.abs(bal) / scale[:, None])
+    assert np.all(np.abs(f).max(axis=1) > 1e-6 * scale / atoms.max())      # not trivially 0
