@@ -19,6 +19,8 @@ import sys
 import time
 from pathlib import Path
 
+FACTOR_CR = 6    # vk_tune_set(VK_TUNE_FACTOR, sl, VK_FACTOR_CR): cyclic-reduction factor
+
 ROOT = Path(__file__).resolve().parent
 sys.path[:0] = [str(ROOT), str(ROOT / "jax-vulcan_amd")]
 
@@ -120,9 +122,13 @@ def main() -> int:
                     help="serving leg: this many steady-state solve requests (sizes of the "
                          "uneven batch, Poisson arrivals at --serve-rate per s, 200 GB budget)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-factor", choices=("cr", "auto"), default="cr",
+                    help="block factor of the serving pools: cyclic reduction (latency mode) "
+                         "or the throughput kernels (--factor)")
     ap.add_argument("--serve-alone", type=int, default=0,
                     help="requests also solved one at a time, for the comparison")
-    ap.add_argument("--factor", choices=("auto", "tile", "rows", "blocked", "tile2", "wave", "tile8"), default="auto",
+    ap.add_argument("--factor", choices=("auto", "tile", "rows", "blocked", "tile2", "wave", "tile8", "cr"),
+                    default="auto",
                     help="block-factor kernel for the nz=150 / ni=93 bucket (auto: the "
                          "library default)")
     ap.add_argument("--no-rates", action="store_true",
@@ -157,7 +163,8 @@ def main() -> int:
     lib = _lib.load()
     if args.factor != "auto":
         _lib.check(lib.vk_tune_set(0, (NI + 15) // 16,
-                                   {"tile": 0, "rows": 1, "blocked": 2, "tile2": 3, "wave": 4, "tile8": 5}[args.factor]),
+                                   {"tile": 0, "rows": 1, "blocked": 2, "tile2": 3, "wave": 4, "tile8": 5,
+                                    "cr": FACTOR_CR}[args.factor]),
                    "vk_tune_set")
     B = args.batch
     t_gen = time.time()
@@ -236,6 +243,23 @@ def main() -> int:
     torch.cuda.synchronize()
     single = args.single_steps / (time.perf_counter() - t0)
     print(f"[bench] single column: {single:.1f} column-steps/s", file=sys.stderr, flush=True)
+    # the same column with the cyclic-reduction factor (latency mode)
+    sl_main = (NI + 15) // 16
+    prev = lib.vk_tune_get(0, sl_main)
+    _lib.check(lib.vk_tune_set(0, sl_main, FACTOR_CR), "vk_tune_set")
+    try:
+        run1c, _, _ = stepper(1)
+        run1c()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.single_steps):
+            run1c()
+        torch.cuda.synchronize()
+        single_cr = args.single_steps / (time.perf_counter() - t0)
+    finally:
+        lib.vk_tune_set(0, sl_main, prev)
+    print(f"[bench] single column, cyclic-reduction factor: {single_cr:.1f} column-steps/s",
+          file=sys.stderr, flush=True)
 
     # ---- uneven batch: adaptive steady-state attempts, one graph replay --------
     uneven = None
@@ -282,7 +306,11 @@ def main() -> int:
                 for i, (a, b_, r) in enumerate(sizes)]
         arrivals = synth.poisson_arrivals(1005, args.serve, args.serve_rate)
         arrivals = arrivals - arrivals[0]
-        if args.factor == "auto":
+        serve_sls = sorted({(int(r.ni) + 15) // 16 for r in reqs})
+        if args.serve_factor == "cr":
+            for sl in serve_sls:
+                _lib.check(lib.vk_tune_set(0, sl, FACTOR_CR), "vk_tune_set")
+        elif args.factor == "auto":
             autotune.tune_columns(reqs, verbose=True)
         eng = ServingEngine(hbm_budget_bytes=200e9, cfg=RunConfig(), chunk=25)
         t0 = time.perf_counter()
@@ -295,7 +323,10 @@ def main() -> int:
             t1 = time.perf_counter()
             r1.run(max_attempts=100000, check_every=25, use_graph=True, report_every=30.0)
             alone.append(time.perf_counter() - t1)
-        serving = {"requests": len(reqs), "arrival_rate_per_s": args.serve_rate,
+        if args.serve_factor == "cr":
+            for sl in serve_sls:
+                lib.vk_tune_set(0, sl, 0)
+        serving = {"factor_kernel": args.serve_factor,"requests": len(reqs), "arrival_rate_per_s": args.serve_rate,
                    "hbm_budget_gb": 200, "slots": {str(k): v for k, v in eng.slot_counts.items()},
                    "wall_s": round(wall, 3),
                    "p50_latency_s": round(summ["p50_s"], 3), "p99_latency_s": round(summ["p99_s"], 3),
@@ -364,6 +395,7 @@ def main() -> int:
                    "batch_columns": B, "nz": NZ, "ni": NI, "rate_rows": S},
         "factor_kernel": main_factor,
         "single_column_steps_per_s": round(single, 3),
+        "single_column_cr_steps_per_s": round(single_cr, 3),
         "batched_over_single": round(value / single, 2),
         "kernel_ms_per_launch": {"chem_lhs": round(kern_ms[0], 4), "bt_factor": round(kern_ms[1], 4),
                                  "bt_back1": round(kern_ms[2], 4), "rhs2": round(kern_ms[3], 4),

@@ -168,7 +168,14 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
  * two rows per lane per tile row; bitwise equal to TILE) or VK_FACTOR_WAVE
  * (the same with one wave per column, four rows per lane per tile row; no
  * barrier inside a column) or VK_FACTOR_TILE8 (eight waves per column, half
- * the rows per lane; even SL).  Process-wide; set it between launches. */
+ * the rows per lane; even SL) or VK_FACTOR_CR (block cyclic reduction for
+ * batches of a few columns: about log2(nz) levels of one workgroup per block
+ * row instead of one workgroup walking nz rows; each block inverted with the
+ * same partial pivoting, the rows combined in another order, so results
+ * agree with TILE to the system's conditioning, not bitwise.  Only
+ * vk_ros2_step_f64 and the steady-state attempt use it; it needs the larger
+ * workspace vk_ros2_workspace_bytes() reports while it is selected).
+ * Process-wide; set it between launches. */
 #define VK_TUNE_FACTOR 0
 #define VK_FACTOR_TILE 0
 #define VK_FACTOR_ROWS 1
@@ -176,6 +183,7 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
 #define VK_FACTOR_TILE2 3
 #define VK_FACTOR_WAVE 4
 #define VK_FACTOR_TILE8 5
+#define VK_FACTOR_CR 6
 /* Knob VK_TUNE_MOLPRE (sl ignored): 1 (default) has vk_ros2_step_f64 compute
  * the molecular-diffusion factors that do not depend on y once per step;
  * 0 forms them inside each chemistry launch.  Results are bitwise equal. */

@@ -77,6 +77,13 @@ int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag
                         const double* rhs, double* rmod, int* status, const int* active,
                         hipStream_t stream);
 bool vk_factor_fuses_sweep(int sl);
+bool vk_factor_is_cr(int sl);
+int vk_launch_cr_factor(int sl, const VkCol* cols, int ncols, int max_nz, const double* blk,
+                        const double* sup_d, const double* sub_d, double* cr, int* status,
+                        const int* active, hipStream_t stream);
+int vk_launch_cr_solve(int sl, const VkCol* cols, int ncols, int max_nz, const double* rhs,
+                       const double* sup_d, const double* sub_d, double* cr, double* x,
+                       SolveEpi epi, const int* active, hipStream_t stream);
 int vk_launch_bt_solve(int sl, const VkCol* cols, int ncols, const double* inv,
                        const double* sup_d, const double* sub_d, const double* rhs,
                        double* rscratch, double* x, int mode, SolveEpi epi, const int* active,
@@ -779,10 +786,23 @@ int vk_timing_read(double* ms, long long* count, int n) {
 // launch.  The coefficients are the same bit for bit either way.
 static int g_mol_precompute = 1;
 
-size_t vk_ros2_workspace_bytes(vk_batch b) {
-  if (!b) return 0;
+// Bytes before the cyclic-reduction region of the step workspace.
+static size_t ros2_ws_base_bytes(vk_batch b) {
   return align_up<double>(b->block_doubles) + 16 * align_up<double>(b->state_doubles) +
          align_up<double>(b->k_doubles);
+}
+
+// Whether a bucket of b uses the cyclic-reduction factor (it then needs 7
+// block stacks of extra workspace).
+static bool batch_uses_cr(vk_batch b) {
+  for (int sl : b->slots)
+    if (vk_factor_is_cr(sl)) return true;
+  return false;
+}
+
+size_t vk_ros2_workspace_bytes(vk_batch b) {
+  if (!b) return 0;
+  return ros2_ws_base_bytes(b) + (batch_uses_cr(b) ? align_up<double>(7 * b->block_doubles) : 0);
 }
 
 // Helper streams and events for running the SL buckets of a batch
@@ -827,6 +847,7 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
   w += align_up<double>(b->k_doubles);
   double* mst = (double*)w;  // 8 components of the molecular-diffusion static factors
   const int64_t mstride = (int64_t)(sv / sizeof(double));
+  double* crw = (double*)((char*)ws + ros2_ws_base_bytes(b));  // cyclic-reduction levels
   const double r = ros2_gamma();
   {
     KernelTimer t(6, st);
@@ -852,21 +873,31 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
         return rc;
     }
     // The factor fuses the stage-1 forward sweep unless the blocked variant
-    // is selected; then the stage-1 solve runs both sweeps.
+    // is selected; then the stage-1 solve runs both sweeps.  The
+    // cyclic-reduction variant has its own factor and solves.
+    const bool cr = vk_factor_is_cr(sl);
     const bool fused = vk_factor_fuses_sweep(sl);
     {
       KernelTimer t(1, s);
-      if ((rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, blk, 1, 0, sup, sub, blk,
-                                    fused ? rhs_y : nullptr, fused ? rmod : nullptr,
-                                    (int*)status, active, s)))
-        return rc;
+      if (cr)
+        rc = vk_launch_cr_factor(sl, b->d_cols, b->ncols, b->max_nz, blk, sup, sub, crw,
+                                 (int*)status, active, s);
+      else
+        rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, blk, 1, 0, sup, sub, blk,
+                                 fused ? rhs_y : nullptr, fused ? rmod : nullptr, (int*)status,
+                                 active, s);
+      if (rc) return rc;
     }
     SolveEpi e1{1, r, 0.0, y, nullptr, nullptr, yk2, nullptr};
     {
       KernelTimer t(2, s);
-      if ((rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, fused ? rmod : rhs_y,
-                                   rmod, k1, fused ? 1 : 0, e1, active, s)))
-        return rc;
+      if (cr)
+        rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rhs_y, sup, sub, crw, k1,
+                                e1, active, s);
+      else
+        rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, fused ? rmod : rhs_y,
+                                rmod, k1, fused ? 1 : 0, e1, active, s);
+      if (rc) return rc;
     }
     {
       KernelTimer t(3, s);
@@ -878,9 +909,13 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
     SolveEpi e2{2, 3.0 / (2.0 * r), 1.0 / (2.0 * r), y, k1, yk2, sol, delta};
     {
       KernelTimer t(4, s);
-      if ((rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, rhs2, rmod, k2, 0, e2,
-                                   active, s)))
-        return rc;
+      if (cr)
+        rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rhs2, sup, sub, crw, k2, e2,
+                                active, s);
+      else
+        rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, rhs2, rmod, k2, 0, e2,
+                                active, s);
+      if (rc) return rc;
     }
     return VK_OK;
   };
@@ -1045,7 +1080,7 @@ int vk_tune_set(int knob, int sl, int value) {
   }
   if (value != VK_FACTOR_TILE && !(value == VK_FACTOR_ROWS && sl <= 6) &&
       value != VK_FACTOR_BLOCKED && value != VK_FACTOR_TILE2 && value != VK_FACTOR_WAVE &&
-      !(value == VK_FACTOR_TILE8 && sl % 2 == 0)) {
+      !(value == VK_FACTOR_TILE8 && sl % 2 == 0) && value != VK_FACTOR_CR) {
     vk_set_error("vk_tune_set: factor variant %d not available for bucket %d", value, sl);
     return VK_ERR_UNSUPPORTED;
   }

@@ -471,6 +471,507 @@ __global__ __launch_bounds__(VK_NT) void k_tile_to_nat(const VkCol* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Block cyclic reduction: the same block-tridiagonal system, for batches of a
+// few columns, where one workgroup per column leaves the GPU idle.  Level l
+// holds n_l block rows (n_0 = nz, n_{l+1} = ceil(n_l / 2)).  Each odd row j
+// is eliminated (its diagonal block inverted with the register Gauss-Jordan
+// and its partial pivoting), giving P_j = U_{j-1} inv(D_j) and
+// Q_j = L_{j+1} inv(D_j); each even row i becomes a row of level l+1:
+//   D' = (D_i - Q_{i-1} U_{i-1}) - P_{i+1} L_{i+1},
+//   L' = -Q_{i-1} L_{i-1},  U' = -P_{i+1} U_{i+1}.
+// The solve reduces r the same way, solves the single top row and
+// substitutes back level by level:  x_j = inv(D_j) (r_j - L_j x_{j-1} -
+// U_j x_{j+1}) for the odd rows.  Every level is one launch whose
+// workgroups each own one block row, so the critical path is about
+// log2(nz) block inversions instead of nz.  Level 0 reads the diagonal
+// blocks from the tile-layout stack the chemistry writes and keeps L, U as
+// the diagonal vectors sub_d / sup_d; later levels store dense blocks, all
+// in the register-tile layout, in a region of 7 nz N^2 doubles per column
+// at 7 * off_b (N = 16 SL; the levels need at most 6.43 nz N^2).
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct CrLvl {
+  int n, h;                           // block rows, eliminated (odd) rows
+  int64_t D, L, U, Dinv, P, Q, r;     // offsets (doubles) in the region
+};
+
+// Offsets of level `level` of a column with nz rows; past the top level the
+// top level is returned (n == 1).
+__device__ inline CrLvl cr_level(int nz, int64_t blk, int N, int64_t base, int level) {
+  CrLvl v{};
+  int n = nz;
+  int64_t off = base;
+  for (int l = 0;; ++l) {
+    v.n = n;
+    v.h = n / 2;
+    v.D = v.L = v.U = v.P = v.Q = -1;
+    if (l >= 1) {
+      v.D = off;
+      off += (int64_t)n * blk;
+      if (n > 1) {
+        v.L = off;
+        off += (int64_t)n * blk;
+        v.U = off;
+        off += (int64_t)n * blk;
+      }
+    }
+    v.Dinv = off;
+    if (n > 1) {
+      off += (int64_t)v.h * blk;
+      v.P = off;
+      off += (int64_t)v.h * blk;
+      v.Q = off;
+      off += (int64_t)v.h * blk;
+    } else {
+      off += blk;
+    }
+    v.r = off;
+    off += (int64_t)n * N;
+    if (l == level || n == 1) return v;
+    n = (n + 1) / 2;
+  }
+}
+
+__device__ inline int cr_top_level(int nz) {
+  int l = 0;
+  for (int n = nz; n > 1; n = (n + 1) / 2) ++l;
+  return l;
+}
+
+// Diagonal block of row j at `level` into the register tile, zero outside ni.
+template <int SL>
+__device__ inline void cr_load_diag(double (&a)[SL][SL], const VkCol& col, const double* blk0,
+                                    const double* cr, const CrLvl& v, int level, int j) {
+  const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  const double* src = level == 0 ? blk0 + col.off_b + (int64_t)j * vk_blk<SL>()
+                                 : cr + v.D + (int64_t)j * vk_blk<SL>();
+  load_tile<SL>(a, src);
+#pragma unroll
+  for (int s = 0; s < SL; ++s)
+#pragma unroll
+    for (int t = 0; t < SL; ++t)
+      if (s * VK_G + tr >= col.ni || t * VK_G + tc >= col.ni) a[s][t] = 0.0;
+}
+
+// acc += A B for tile-layout blocks in global memory, through 16-wide LDS
+// panels (ga: the 16 columns of A, gb: the 16 rows of B).
+template <int SL>
+__device__ inline void cr_gemm(const double* __restrict__ A, const double* __restrict__ B,
+                               double (&acc)[SL][SL], double* ga, double* gb) {
+  const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+#pragma unroll 1
+  for (int tp = 0; tp < SL; ++tp) {
+#pragma unroll
+    for (int s = 0; s < SL; ++s) ga[(s * VK_G + tr) * VK_G + tc] = A[(s * SL + tp) * VK_NT + tid];
+#pragma unroll
+    for (int u = 0; u < SL; ++u) gb[(u * VK_G + tr) * VK_G + tc] = B[(tp * SL + u) * VK_NT + tid];
+    __syncthreads();
+#pragma unroll 4
+    for (int pp = 0; pp < VK_G; ++pp) {
+      double av[SL], bv[SL];
+#pragma unroll
+      for (int s = 0; s < SL; ++s) av[s] = ga[(s * VK_G + tr) * VK_G + pp];
+#pragma unroll
+      for (int u = 0; u < SL; ++u) bv[u] = gb[(u * VK_G + pp) * VK_G + tc];
+#pragma unroll
+      for (int s = 0; s < SL; ++s)
+#pragma unroll
+        for (int u = 0; u < SL; ++u) acc[s][u] = fma(av[s], bv[u], acc[s][u]);
+    }
+    __syncthreads();
+  }
+}
+
+template <int SL>
+__device__ inline void cr_zero(double (&a)[SL][SL]) {
+#pragma unroll
+  for (int s = 0; s < SL; ++s)
+#pragma unroll
+    for (int t = 0; t < SL; ++t) a[s][t] = 0.0;
+}
+
+__device__ inline void cr_epilogue(const SolveEpi& epi, int64_t e, double xv, double* x) {
+  x[e] = xv;
+  if (epi.kind == 1) {
+    epi.out1[e] = epi.y[e] + xv / epi.e0;
+  } else if (epi.kind == 2) {
+#pragma clang fp contract(off)
+    const double sol = epi.y[e] + epi.e0 * epi.k1[e] + epi.e1 * xv;
+    epi.out1[e] = sol;
+    epi.out2[e] = fabs(sol - epi.yk2[e]);
+  }
+}
+
+}  // namespace
+
+// Eliminate the odd rows of one level: inv(D_j), P_j, Q_j.  Grid (h, ncols).
+template <int SL>
+__global__ __launch_bounds__(VK_NT) void k_cr_elim(const VkCol* __restrict__ cols, int ncols,
+                                                    const double* __restrict__ blk0,
+                                                    const double* __restrict__ sup_d,
+                                                    const double* __restrict__ sub_d,
+                                                    double* __restrict__ cr, int level,
+                                                    int* __restrict__ status,
+                                                    const int* __restrict__ active) {
+  __shared__ BtShared<SL> sh;
+  __shared__ double ga[VK_G * VK_G * SL], gb[VK_G * VK_G * SL];
+  const int c = blockIdx.y, w = blockIdx.x;
+  if (c >= ncols) return;
+  if (active && (active[c] & VK_ST_DONE)) return;
+  const VkCol col = cols[c];
+  if (vk_slots(col.ni) != SL) return;
+  constexpr int BLK = vk_blk<SL>(), N = VK_G * SL;
+  const CrLvl v = cr_level(col.nz, BLK, N, 7 * col.off_b, level);
+  if (v.n <= 1 || w >= v.h) return;
+  const int ni = col.ni, tid = threadIdx.x, tr = tid >> 4;
+  const int j = 2 * w + 1;
+  double a[SL][SL];
+  cr_load_diag<SL>(a, col, blk0, cr, v, level, j);
+  const int bad = gj_invert<SL>(a, ni, sh);
+  if (bad && tid == 0 && status) atomicOr(&status[c], VK_ST_SINGULAR);
+  store_tile<SL>(a, cr + v.Dinv + (int64_t)w * BLK);
+  const bool has_q = j + 1 < v.n;
+  if (level == 0) {
+    // U_{j-1} = diag(sup_d[j-1]), L_{j+1} = diag(sub_d[j])
+    const double* su = sup_d + col.off_s + (int64_t)(j - 1) * ni;
+    const double* sl = sub_d + col.off_s + (int64_t)j * ni;
+    double p[SL][SL];
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int i = s * VK_G + tr;
+      const double u = i < ni ? su[i] : 0.0;
+#pragma unroll
+      for (int t = 0; t < SL; ++t) p[s][t] = u * a[s][t];
+    }
+    store_tile<SL>(p, cr + v.P + (int64_t)w * BLK);
+    if (has_q) {
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        const int i = s * VK_G + tr;
+        const double lv = i < ni ? sl[i] : 0.0;
+#pragma unroll
+        for (int t = 0; t < SL; ++t) p[s][t] = lv * a[s][t];
+      }
+      store_tile<SL>(p, cr + v.Q + (int64_t)w * BLK);
+    }
+  } else {
+    __syncthreads();  // inv(D_j) is read back through the panels
+    const double* Dinv = cr + v.Dinv + (int64_t)w * BLK;
+    double acc[SL][SL];
+    cr_zero<SL>(acc);
+    cr_gemm<SL>(cr + v.U + (int64_t)(j - 1) * BLK, Dinv, acc, ga, gb);
+    store_tile<SL>(acc, cr + v.P + (int64_t)w * BLK);
+    if (has_q) {
+      cr_zero<SL>(acc);
+      cr_gemm<SL>(cr + v.L + (int64_t)(j + 1) * BLK, Dinv, acc, ga, gb);
+      store_tile<SL>(acc, cr + v.Q + (int64_t)w * BLK);
+    }
+  }
+}
+
+// Even rows of one level become the rows of the next.  Grid (ceil(n/2), ncols).
+template <int SL>
+__global__ __launch_bounds__(VK_NT) void k_cr_update(const VkCol* __restrict__ cols, int ncols,
+                                                      const double* __restrict__ blk0,
+                                                      const double* __restrict__ sup_d,
+                                                      const double* __restrict__ sub_d,
+                                                      double* __restrict__ cr, int level,
+                                                      const int* __restrict__ active) {
+  __shared__ double ga[VK_G * VK_G * SL], gb[VK_G * VK_G * SL];
+  const int c = blockIdx.y, w = blockIdx.x;
+  if (c >= ncols) return;
+  if (active && (active[c] & VK_ST_DONE)) return;
+  const VkCol col = cols[c];
+  if (vk_slots(col.ni) != SL) return;
+  constexpr int BLK = vk_blk<SL>(), N = VK_G * SL;
+  const CrLvl v = cr_level(col.nz, BLK, N, 7 * col.off_b, level);
+  const int i = 2 * w;
+  if (v.n <= 1 || i >= v.n) return;
+  const CrLvl vn = cr_level(col.nz, BLK, N, 7 * col.off_b, level + 1);
+  const int ni = col.ni, tid = threadIdx.x, tc = tid & 15;
+  double d[SL][SL], m[SL][SL];
+  cr_load_diag<SL>(d, col, blk0, cr, v, level, i);
+  if (i >= 1) {  // odd row i-1: Q_{i-1} = L_i inv(D_{i-1})
+    const double* Qp = cr + v.Q + (int64_t)(w - 1) * BLK;
+    if (level == 0) {
+      // U_{i-1} = diag(sup_d[i-1]), L_{i-1} = diag(sub_d[i-2]); m becomes L'
+      load_tile<SL>(m, Qp);
+      const double* bu = sup_d + col.off_s + (int64_t)(i - 1) * ni;
+      const double* bl = sub_d + col.off_s + (int64_t)(i >= 2 ? i - 2 : 0) * ni;
+#pragma unroll
+      for (int t = 0; t < SL; ++t) {
+        const int q = t * VK_G + tc;
+        const double b = q < ni ? bu[q] : 0.0;
+        const double b2 = (q < ni && i >= 2) ? bl[q] : 0.0;
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          d[s][t] = d[s][t] - m[s][t] * b;
+          m[s][t] = -(m[s][t] * b2);
+        }
+      }
+      if (i >= 2) store_tile<SL>(m, cr + vn.L + (int64_t)w * BLK);
+    } else {
+      cr_zero<SL>(m);
+      cr_gemm<SL>(Qp, cr + v.U + (int64_t)(i - 1) * BLK, m, ga, gb);
+#pragma unroll
+      for (int s = 0; s < SL; ++s)
+#pragma unroll
+        for (int t = 0; t < SL; ++t) d[s][t] = d[s][t] - m[s][t];
+      if (i >= 2) {
+        cr_zero<SL>(m);
+        cr_gemm<SL>(Qp, cr + v.L + (int64_t)(i - 1) * BLK, m, ga, gb);
+#pragma unroll
+        for (int s = 0; s < SL; ++s)
+#pragma unroll
+          for (int t = 0; t < SL; ++t) m[s][t] = -m[s][t];
+        store_tile<SL>(m, cr + vn.L + (int64_t)w * BLK);
+      }
+    }
+  }
+  if (i + 1 < v.n) {  // odd row i+1: P_{i+1} = U_i inv(D_{i+1})
+    const double* Pp = cr + v.P + (int64_t)w * BLK;
+    const bool has_u = i + 2 < v.n;
+    if (level == 0) {
+      // L_{i+1} = diag(sub_d[i]), U_{i+1} = diag(sup_d[i+1]); m becomes U'
+      load_tile<SL>(m, Pp);
+      const double* bl = sub_d + col.off_s + (int64_t)i * ni;
+      const double* bu = sup_d + col.off_s + (int64_t)(has_u ? i + 1 : i) * ni;
+#pragma unroll
+      for (int t = 0; t < SL; ++t) {
+        const int q = t * VK_G + tc;
+        const double b = q < ni ? bl[q] : 0.0;
+        const double b2 = (q < ni && has_u) ? bu[q] : 0.0;
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          d[s][t] = d[s][t] - m[s][t] * b;
+          m[s][t] = -(m[s][t] * b2);
+        }
+      }
+      if (has_u) store_tile<SL>(m, cr + vn.U + (int64_t)w * BLK);
+    } else {
+      cr_zero<SL>(m);
+      cr_gemm<SL>(Pp, cr + v.L + (int64_t)(i + 1) * BLK, m, ga, gb);
+#pragma unroll
+      for (int s = 0; s < SL; ++s)
+#pragma unroll
+        for (int t = 0; t < SL; ++t) d[s][t] = d[s][t] - m[s][t];
+      if (has_u) {
+        cr_zero<SL>(m);
+        cr_gemm<SL>(Pp, cr + v.U + (int64_t)(i + 1) * BLK, m, ga, gb);
+#pragma unroll
+        for (int s = 0; s < SL; ++s)
+#pragma unroll
+          for (int t = 0; t < SL; ++t) m[s][t] = -m[s][t];
+        store_tile<SL>(m, cr + vn.U + (int64_t)w * BLK);
+      }
+    }
+  }
+  store_tile<SL>(d, cr + vn.D + (int64_t)w * BLK);
+}
+
+// Invert the single row of the top level.  Grid (ncols).
+template <int SL>
+__global__ __launch_bounds__(VK_NT) void k_cr_top(const VkCol* __restrict__ cols, int ncols,
+                                                   const double* __restrict__ blk0,
+                                                   double* __restrict__ cr,
+                                                   int* __restrict__ status,
+                                                   const int* __restrict__ active) {
+  __shared__ BtShared<SL> sh;
+  const int c = blockIdx.x;
+  if (c >= ncols) return;
+  if (active && (active[c] & VK_ST_DONE)) return;
+  const VkCol col = cols[c];
+  if (vk_slots(col.ni) != SL) return;
+  constexpr int BLK = vk_blk<SL>(), N = VK_G * SL;
+  const int top = cr_top_level(col.nz);
+  const CrLvl v = cr_level(col.nz, BLK, N, 7 * col.off_b, top);
+  double a[SL][SL];
+  cr_load_diag<SL>(a, col, blk0, cr, v, top, 0);
+  const int bad = gj_invert<SL>(a, col.ni, sh);
+  if (bad && threadIdx.x == 0 && status) atomicOr(&status[c], VK_ST_SINGULAR);
+  store_tile<SL>(a, cr + v.Dinv);
+}
+
+// Reduce r: r'_i = (r_i - Q_{i-1} r_{i-1}) - P_{i+1} r_{i+1}.  Grid (ceil(n/2), ncols).
+template <int SL>
+__global__ __launch_bounds__(VK_NT) void k_cr_fwd(const VkCol* __restrict__ cols, int ncols,
+                                                   const double* __restrict__ rhs,
+                                                   double* __restrict__ cr, int level,
+                                                   const int* __restrict__ active) {
+  constexpr int BLK = vk_blk<SL>(), N = VK_G * SL;
+  __shared__ double rv[3][N];
+  const int c = blockIdx.y, w = blockIdx.x;
+  if (c >= ncols) return;
+  if (active && (active[c] & VK_ST_DONE)) return;
+  const VkCol col = cols[c];
+  if (vk_slots(col.ni) != SL) return;
+  const CrLvl v = cr_level(col.nz, BLK, N, 7 * col.off_b, level);
+  const int i = 2 * w;
+  if (v.n <= 1 || i >= v.n) return;
+  const CrLvl vn = cr_level(col.nz, BLK, N, 7 * col.off_b, level + 1);
+  const int ni = col.ni, tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  for (int e = tid; e < 3 * N; e += VK_NT) {
+    const int row = i - 1 + e / N, q = e % N;
+    double x = 0.0;
+    if (row >= 0 && row < v.n && q < ni)
+      x = level == 0 ? rhs[col.off_s + (int64_t)row * ni + q] : cr[v.r + (int64_t)row * N + q];
+    rv[e / N][q] = x;
+  }
+  __syncthreads();
+  double out[SL], y[SL], a[SL][SL];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) out[s] = rv[1][s * VK_G + tr];
+  if (i >= 1) {
+    load_tile<SL>(a, cr + v.Q + (int64_t)(w - 1) * BLK);
+    tile_matvec<SL>(a, rv[0], y);
+#pragma unroll
+    for (int s = 0; s < SL; ++s) out[s] = out[s] - y[s];
+  }
+  if (i + 1 < v.n) {
+    load_tile<SL>(a, cr + v.P + (int64_t)w * BLK);
+    tile_matvec<SL>(a, rv[2], y);
+#pragma unroll
+    for (int s = 0; s < SL; ++s) out[s] = out[s] - y[s];
+  }
+  if (tc == 0) {
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int q = s * VK_G + tr;
+      cr[vn.r + (int64_t)w * N + q] = q < ni ? out[s] : 0.0;
+    }
+  }
+}
+
+// x of the top row; for nz == 1 the whole solution.  Grid (ncols).
+template <int SL>
+__global__ __launch_bounds__(VK_NT) void k_cr_top_solve(const VkCol* __restrict__ cols, int ncols,
+                                                         const double* __restrict__ rhs,
+                                                         double* __restrict__ cr,
+                                                         double* __restrict__ x, SolveEpi epi,
+                                                         const int* __restrict__ active) {
+  constexpr int BLK = vk_blk<SL>(), N = VK_G * SL;
+  __shared__ double rv[N];
+  const int c = blockIdx.x;
+  if (c >= ncols) return;
+  if (active && (active[c] & VK_ST_DONE)) return;
+  const VkCol col = cols[c];
+  if (vk_slots(col.ni) != SL) return;
+  const int top = cr_top_level(col.nz);
+  const CrLvl v = cr_level(col.nz, BLK, N, 7 * col.off_b, top);
+  const int ni = col.ni, tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  for (int q = tid; q < N; q += VK_NT)
+    rv[q] = q >= ni ? 0.0 : (top == 0 ? rhs[col.off_s + q] : cr[v.r + q]);
+  __syncthreads();
+  double a[SL][SL], y[SL];
+  load_tile<SL>(a, cr + v.Dinv);
+  tile_matvec<SL>(a, rv, y);
+  if (tc == 0) {
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int q = s * VK_G + tr;
+      if (top == 0) {
+        if (q < ni) cr_epilogue(epi, col.off_s + q, y[s], x);
+      } else {
+        cr[v.r + q] = q < ni ? y[s] : 0.0;
+      }
+    }
+  }
+}
+
+// Substitute back at one level: even rows copy x from the level above, odd
+// rows solve x_j = inv(D_j) ((r_j - L_j x_{j-1}) - U_j x_{j+1}).  Level 0
+// writes x and the epilogue.  Grid (n, ncols).
+template <int SL>
+__global__ __launch_bounds__(VK_NT) void k_cr_bwd(const VkCol* __restrict__ cols, int ncols,
+                                                   const double* __restrict__ rhs,
+                                                   const double* __restrict__ sup_d,
+                                                   const double* __restrict__ sub_d,
+                                                   double* __restrict__ cr, int level,
+                                                   double* __restrict__ x, SolveEpi epi,
+                                                   const int* __restrict__ active) {
+  constexpr int BLK = vk_blk<SL>(), N = VK_G * SL;
+  __shared__ double xv[3][N];
+  const int c = blockIdx.y, j = blockIdx.x;
+  if (c >= ncols) return;
+  if (active && (active[c] & VK_ST_DONE)) return;
+  const VkCol col = cols[c];
+  if (vk_slots(col.ni) != SL) return;
+  const CrLvl v = cr_level(col.nz, BLK, N, 7 * col.off_b, level);
+  if (v.n <= 1 || j >= v.n) return;
+  const CrLvl vn = cr_level(col.nz, BLK, N, 7 * col.off_b, level + 1);
+  const int ni = col.ni, tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  double res[SL];
+  if ((j & 1) == 0) {
+#pragma unroll
+    for (int s = 0; s < SL; ++s) res[s] = cr[vn.r + (int64_t)(j / 2) * N + s * VK_G + tr];
+  } else {
+    const bool has_u = j + 1 < v.n;
+    // xv[0] = x_{j-1}, xv[1] = r_j, xv[2] = x_{j+1} (level-l indices)
+    for (int q = tid; q < N; q += VK_NT) {
+      const bool in = q < ni;
+      xv[0][q] = in ? cr[vn.r + (int64_t)((j - 1) / 2) * N + q] : 0.0;
+      xv[2][q] = (in && has_u) ? cr[vn.r + (int64_t)((j + 1) / 2) * N + q] : 0.0;
+      xv[1][q] = !in ? 0.0
+                     : (level == 0 ? rhs[col.off_s + (int64_t)j * ni + q]
+                                   : cr[v.r + (int64_t)j * N + q]);
+    }
+    __syncthreads();
+    double t[SL], y[SL], a[SL][SL];
+#pragma unroll
+    for (int s = 0; s < SL; ++s) t[s] = 0.0;
+    if (level == 0) {
+      // L_j = diag(sub_d[j-1]), U_j = diag(sup_d[j])
+      if (tc == 0) {
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          const int q = s * VK_G + tr;
+          double r = 0.0;
+          if (q < ni) {
+            r = xv[1][q] - sub_d[col.off_s + (int64_t)(j - 1) * ni + q] * xv[0][q];
+            if (has_u) r = r - sup_d[col.off_s + (int64_t)j * ni + q] * xv[2][q];
+          }
+          t[s] = r;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < SL; ++s) t[s] = xv[1][s * VK_G + tr];
+      load_tile<SL>(a, cr + v.L + (int64_t)j * BLK);
+      tile_matvec<SL>(a, xv[0], y);
+#pragma unroll
+      for (int s = 0; s < SL; ++s) t[s] = t[s] - y[s];
+      if (has_u) {
+        load_tile<SL>(a, cr + v.U + (int64_t)j * BLK);
+        tile_matvec<SL>(a, xv[2], y);
+#pragma unroll
+        for (int s = 0; s < SL; ++s) t[s] = t[s] - y[s];
+      }
+    }
+    __syncthreads();
+    if (tc == 0) {
+#pragma unroll
+      for (int s = 0; s < SL; ++s) xv[1][s * VK_G + tr] = t[s];
+    }
+    __syncthreads();
+    load_tile<SL>(a, cr + v.Dinv + (int64_t)((j - 1) / 2) * BLK);
+    tile_matvec<SL>(a, xv[1], res);
+  }
+  if (tc == 0) {
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int q = s * VK_G + tr;
+      if (level == 0) {
+        if (q < ni) cr_epilogue(epi, col.off_s + (int64_t)j * ni + q, res[s], x);
+      } else {
+        cr[v.r + (int64_t)j * N + q] = q < ni ? res[s] : 0.0;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host launchers (dispatch on the slot count of the batch bucket)
 // ---------------------------------------------------------------------------
 
@@ -508,7 +1009,8 @@ int g_factor_variant[8] = {VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FA
                            VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE};
 
 bool vk_factor_fuses_sweep(int sl) {
-  return !(sl >= 1 && sl <= 7 && g_factor_variant[sl] == VK_FACTOR_BLOCKED);
+  return !(sl >= 1 && sl <= 7 && (g_factor_variant[sl] == VK_FACTOR_BLOCKED ||
+                                  g_factor_variant[sl] == VK_FACTOR_CR));
 }
 
 int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag, int diag_tile,
@@ -563,5 +1065,56 @@ int vk_launch_tile_to_nat(int sl, const VkCol* cols, int ncols, int max_nz, cons
   VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_tile_to_nat<SL_>, dim3(max_nz, ncols), dim3(VK_NT), 0,
                                         stream, cols, ncols, tile, nat));
   VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}
+
+bool vk_factor_is_cr(int sl) {
+  return sl >= 1 && sl <= 7 && g_factor_variant[sl] == VK_FACTOR_CR;
+}
+
+// Cyclic-reduction factor of the tile-layout diagonal stack blk (level 0)
+// into the region cr (7 * block_doubles); one launch pair per level.
+int vk_launch_cr_factor(int sl, const VkCol* cols, int ncols, int max_nz, const double* blk,
+                        const double* sup_d, const double* sub_d, double* cr, int* status,
+                        const int* active, hipStream_t stream) {
+  int level = 0;
+  for (int n = max_nz; n > 1; n = (n + 1) / 2, ++level) {
+    VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_cr_elim<SL_>, dim3(n / 2, ncols), dim3(VK_NT), 0,
+                                          stream, cols, ncols, blk, sup_d, sub_d, cr, level,
+                                          status, active));
+    VK_CHECK_HIP(hipGetLastError());
+    VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_cr_update<SL_>, dim3((n + 1) / 2, ncols),
+                                          dim3(VK_NT), 0, stream, cols, ncols, blk, sup_d, sub_d,
+                                          cr, level, active));
+    VK_CHECK_HIP(hipGetLastError());
+  }
+  VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_cr_top<SL_>, dim3(ncols), dim3(VK_NT), 0, stream, cols,
+                                        ncols, blk, cr, status, active));
+  VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}
+
+// Solve with the cyclic-reduction factors: x and the epilogue from rhs.
+int vk_launch_cr_solve(int sl, const VkCol* cols, int ncols, int max_nz, const double* rhs,
+                       const double* sup_d, const double* sub_d, double* cr, double* x,
+                       SolveEpi epi, const int* active, hipStream_t stream) {
+  int ns[40];
+  int levels = 0;
+  for (int n = max_nz; n > 1 && levels < 40; n = (n + 1) / 2) ns[levels++] = n;
+  for (int l = 0; l < levels; ++l) {
+    VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_cr_fwd<SL_>, dim3((ns[l] + 1) / 2, ncols),
+                                          dim3(VK_NT), 0, stream, cols, ncols, rhs, cr, l,
+                                          active));
+    VK_CHECK_HIP(hipGetLastError());
+  }
+  VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_cr_top_solve<SL_>, dim3(ncols), dim3(VK_NT), 0, stream,
+                                        cols, ncols, rhs, cr, x, epi, active));
+  VK_CHECK_HIP(hipGetLastError());
+  for (int l = levels - 1; l >= 0; --l) {
+    VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_cr_bwd<SL_>, dim3(ns[l], ncols), dim3(VK_NT), 0,
+                                          stream, cols, ncols, rhs, sup_d, sub_d, cr, l, x, epi,
+                                          active));
+    VK_CHECK_HIP(hipGetLastError());
+  }
   return VK_OK;
 }

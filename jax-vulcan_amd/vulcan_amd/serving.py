@@ -39,6 +39,8 @@ from . import _lib
 from .core import F64, DeviceNetwork, atm_record_layout, block_doubles, device, pack_atm_record
 from .outer_loop import (ST_CONVERGED, ST_DONE, ST_MAXSTEPS, VK_MAX_ATOMS, RunConfig, SsState)
 
+FACTOR_CR = 6      # vk_tune_set(VK_TUNE_FACTOR, sl, VK_FACTOR_CR)
+
 class RateSource(NamedTuple):
     """What a request needs to have its k built on the device."""
     net: object                      # rate network (RateNetwork or network.Network)
@@ -132,6 +134,9 @@ class SlotPool:
         cs, ck = cap_nz * cap_ni, cap_S * cap_nz
         ca = atm_record_layout(cap_nz, cap_ni)[0]
         ws = cap_nz * block_doubles(cap_ni) + 16 * cs + ck  # Ros2 workspace per slot
+        sl = (cap_ni + 15) // 16
+        if _lib.load().vk_tune_get(0, sl) == FACTOR_CR:   # the cyclic-reduction levels
+            ws += 7 * cap_nz * block_doubles(cap_ni)
         return 8 * (6 * cs + ck + ca + ws) + 8 * (3 * VK_MAX_ATOMS + 4) + 4 * 8
 
     # ------------------------------------------------------------------

@@ -1,0 +1,152 @@
+"""GPU: the cyclic-reduction block factor (VK_FACTOR_CR) in the Ros2 step.
+
+The same block-tridiagonal system as the tile factor (solver.py:33-126),
+eliminated in log2(nz) levels instead of one sweep; every diagonal block is
+still inverted with getrf's partial pivoting, but rows are combined in
+another order, so the bar is agreement to the system's conditioning:
+
+* workload 1 (nz = 100, ni = 50) at a well-conditioned dt: 1e-12 per layer
+  against the tile factor and 1e-10 against the oracle;
+* workload 1 at its own dt: 1e-10 on every layer where the two oracles
+  agree to 1e-12, within 10x their spread elsewhere (the tile factor's
+  own bar in test_gpu_parity.py);
+* an uneven batch (nz 2..61, four SL buckets, odd and even nz, powers of
+  two and one above): each column equals the same column stepped alone bit
+  for bit, and the tile factor to 1e-12 at a small dt;
+* steady-state attempts replayed from a HIP graph equal the eager ones bit
+  for bit (the factor is some 30 launches per attempt).
+"""
+from __future__ import annotations
+
+from contextlib import contextmanager
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+VK_FACTOR_TILE, VK_FACTOR_CR = 0, 6
+
+
+@contextmanager
+def factor(variant):
+    from vulcan_amd import _lib
+    lib = _lib.load()
+    try:
+        for sl in range(1, 8):
+            _lib.check(lib.vk_tune_set(0, sl, variant), "vk_tune_set")
+        yield
+    finally:
+        for sl in range(1, 8):
+            lib.vk_tune_set(0, sl, VK_FACTOR_TILE)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _layer_err(sol, ref):
+    return np.max(np.abs(sol - ref), axis=1) / np.max(np.abs(ref), axis=1)
+
+
+def test_cr_step_well_conditioned(cfg1):
+    from oracle import ros2_np as O
+    from vulcan_amd import jax_step
+    c = cfg1
+    dt = 1e-18
+    with factor(VK_FACTOR_CR):
+        cr, _ = jax_step.jax_ros2_step(c.y, c.k, dt, c.atm, c.net)
+    tile, _ = jax_step.jax_ros2_step(c.y, c.k, dt, c.atm, c.net)
+    so, _ = O.ros2_step(c.y, c.k, dt, c.atm, c.net)
+    e_tile = _layer_err(_np(cr), _np(tile)).max()
+    e_or = _layer_err(_np(cr), so).max()
+    print(f"cr vs tile {e_tile:.2e}, vs oracle {e_or:.2e}")
+    assert e_tile < 1e-12 and e_or < 1e-10
+
+
+def test_cr_step_cfg1_matches_oracle(cfg1):
+    from oracle import ros2_c, ros2_np as O
+    from vulcan_amd import core, jax_step
+    c = cfg1
+    with factor(VK_FACTOR_CR):
+        sol, delta = jax_step.jax_ros2_step(c.y, c.k, c.dt, c.atm, c.net)
+    so, do = O.ros2_step(c.y, c.k, c.dt, c.atm, c.net)
+    rec = core.pack_atm_record(c.atm).numpy()[None]
+    sc, _ = ros2_c.ros2_steps(c.y[None], c.k[None], rec, c.dt, c.net)
+    e_gpu = _layer_err(_np(sol), so)
+    e_floor = _layer_err(sc[0], so)
+    well = e_floor < 1e-12
+    print(f"cr cfg1: well-determined layers {well.sum()}/{c.nz}, gpu err there "
+          f"{e_gpu[well].max():.3e}; worst {e_gpu.max():.3e} vs oracle spread "
+          f"{e_floor.max():.3e}")
+    assert np.all(np.isfinite(_np(sol)))
+    assert e_gpu[well].max() < 1e-10
+    assert e_gpu.max() <= 10.0 * e_floor.max() + 1e-10
+
+
+def _uneven():
+    from vulcan_amd import synth
+    shapes = ((2, 8), (3, 14), (5, 20), (9, 33), (16, 20), (17, 50), (33, 14), (61, 40))
+    return [synth.make_column(900 + i, nz=nz, ni=ni, n_rev=3 * ni, dt=1e-16, p_bot=1e6,
+                              p_top=1e0) for i, (nz, ni) in enumerate(shapes)]
+
+
+def _batch_step(cols):
+    """One Ros2 step of the columns as one uneven batch (vk_ros2_step_f64)."""
+    import torch
+    from vulcan_amd import _lib
+    from vulcan_amd.core import F64, Batch, device, device_network
+    dev = device()
+    nets = [device_network(c.net) for c in cols]
+    b = Batch([c.nz for c in cols], nets, list(range(len(cols))))
+    y = b.pack_state([c.y for c in cols], dev)
+    k = b.pack_k([c.k for c in cols], dev)
+    atm = b.pack_atm([c.atm for c in cols], dev)
+    dt = torch.tensor([float(c.dt) for c in cols], dtype=F64, device=dev)
+    sol, delta = torch.empty_like(y), torch.empty_like(y)
+    status = torch.zeros(len(cols), dtype=torch.int32, device=dev)
+    ws = b.workspace()
+    _lib.check(_lib.load().vk_ros2_step_f64(b.handle, y.data_ptr(), k.data_ptr(),
+                                            atm.data_ptr(), dt.data_ptr(), None, sol.data_ptr(),
+                                            delta.data_ptr(), ws.data_ptr(), ws.numel(),
+                                            status.data_ptr(), _lib.stream_ptr()),
+               "vk_ros2_step_f64")
+    torch.cuda.synchronize()
+    return [_np(t) for t in b.unpack_state(sol)], _np(status)
+
+
+def test_cr_uneven_batch_is_bitwise_alone_and_matches_tile():
+    cols = _uneven()
+    with factor(VK_FACTOR_CR):
+        batch, st = _batch_step(cols)
+        alone = [_batch_step([c])[0][0] for c in cols]
+    tile, _ = _batch_step(cols)
+    assert not st.any()
+    for c, col in enumerate(cols):
+        assert np.array_equal(batch[c], alone[c]), (c, col.nz, col.ni)
+        assert np.all(np.isfinite(batch[c]))
+        e = _layer_err(batch[c], tile[c]).max()
+        assert e < 1e-12, (c, col.nz, col.ni, e)
+
+
+def test_cr_attempt_graph_replay_equals_eager():
+    import torch
+    from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
+    from vulcan_amd import synth
+    cols = [synth.make_column(500 + c, nz=nz, ni=ni, n_rev=n_rev, dt=1e-10, p_bot=1e5,
+                              p_top=1e-1, consistent=True)
+            for c, (nz, ni, n_rev) in enumerate(((18, 14, 40), (24, 20, 55), (16, 14, 40)))]
+    with factor(VK_FACTOR_CR):
+        r = SteadyStateRunner(cols, RunConfig(count_max=400))
+        for _ in range(30):
+            r.attempt()
+        torch.cuda.synchronize()
+        eager = [x.clone() for x in (r.y, r.dt, r.accept, r.status)]
+        r.reset_to_initial()
+        r.capture(15)
+        for _ in range(2):
+            r.graph[0].replay()
+        torch.cuda.synchronize()
+        for a, e in zip((r.y, r.dt, r.accept, r.status), eager):
+            assert torch.equal(a, e)
+        assert int(r.accept.min().item()) >= 10
