@@ -6,15 +6,19 @@ one 256-thread workgroup per column with 16x16 register tiles;
 ``VK_FACTOR_ROWS``: half a row per lane, 64-192 lanes per column;
 ``VK_FACTOR_BLOCKED``: 16-column panels with FP64 MFMA trailing updates;
 ``VK_FACTOR_TILE2`` / ``VK_FACTOR_WAVE`` / ``VK_FACTOR_TILE8``: the tile
-kernel on two / one / eight waves per column) on
+kernel on two / one / eight waves per column; ``VK_FACTOR_CR``: block cyclic
+reduction, log2(nz) levels of one workgroup per block row, the latency mode
+for a few columns) on
 the batch's own columns of that bucket, and keep the fastest
 (``vk_tune_set``).  All take the same pivots; TILE, ROWS, TILE2, WAVE and
 TILE8 give bitwise-equal inverses, BLOCKED differs in the rounding of its rank-16
-products.  Decisions are cached per
+products, CR in the order its rows are combined.  Decisions are cached per
 (SL, nz rounded up to 32, column count rounded up to a power of two).
 
 The timed call is one plain Ros2 step of the bucket's columns (y -> sol), so
-a runner's state is not touched.
+a runner's state is not touched.  Tune before building runners or serving
+pools: the cyclic-reduction factor needs a larger step workspace
+(vk_ros2_workspace_bytes), sized when they are built.
 """
 from __future__ import annotations
 
@@ -30,7 +34,7 @@ from . import _lib
 from .core import F64, Batch, device, device_network
 
 VK_TUNE_FACTOR = 0
-VARIANTS = {0: "tile", 1: "rows", 2: "blocked", 3: "tile2", 4: "wave", 5: "tile8"}
+VARIANTS = {0: "tile", 1: "rows", 2: "blocked", 3: "tile2", 4: "wave", 5: "tile8", 6: "cr"}
 _CACHE: Dict[Tuple[int, int, int], int] = {}
 
 
@@ -88,7 +92,7 @@ def tune_columns(columns: Sequence, reps: int = 2, verbose: bool = False) -> Dic
         atm = b.pack_atm([c.atm for c in cols], dev)
         dt = torch.tensor([float(c.dt) for c in cols], dtype=F64, device=dev)
         times = {}
-        for v in (0, 1, 2, 3, 4, 5):
+        for v in VARIANTS:
             if lib.vk_tune_set(VK_TUNE_FACTOR, sl, v) != 0:
                 continue  # variant not offered for this bucket
             times[v] = _time_step(b, y, k, atm, dt, reps)
