@@ -118,7 +118,7 @@ def main() -> int:
                          "0 skips it)")
     ap.add_argument("--uneven-attempts", type=int, default=10,
                     help="steady-state attempts per column timed in the uneven leg")
-    ap.add_argument("--serve", type=int, default=8,
+    ap.add_argument("--serve", type=int, default=64,
                     help="serving leg: this many steady-state solve requests (sizes of the "
                          "uneven batch, Poisson arrivals at --serve-rate per s, 200 GB budget)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
