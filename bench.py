@@ -233,19 +233,23 @@ def main() -> int:
           f"prep {kern_ms[6]:.3f}",
           file=sys.stderr, flush=True)
 
-    # ---- plain single-column kernel ------------------------------------------
-    run1, _, _ = stepper(1)
-    run1()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.single_steps):
-        run1()
-    torch.cuda.synchronize()
-    single = args.single_steps / (time.perf_counter() - t0)
-    print(f"[bench] single column: {single:.1f} column-steps/s", file=sys.stderr, flush=True)
-    # the same column with the cyclic-reduction factor (latency mode)
+    # ---- plain single-column kernel (the tile factor) --------------------------
     sl_main = (NI + 15) // 16
     prev = lib.vk_tune_get(0, sl_main)
+    _lib.check(lib.vk_tune_set(0, sl_main, 0), "vk_tune_set")
+    try:
+        run1, _, _ = stepper(1)
+        run1()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.single_steps):
+            run1()
+        torch.cuda.synchronize()
+        single = args.single_steps / (time.perf_counter() - t0)
+    finally:
+        lib.vk_tune_set(0, sl_main, prev)
+    print(f"[bench] single column: {single:.1f} column-steps/s", file=sys.stderr, flush=True)
+    # the same column with the cyclic-reduction factor (latency mode)
     _lib.check(lib.vk_tune_set(0, sl_main, FACTOR_CR), "vk_tune_set")
     try:
         run1c, _, _ = stepper(1)

@@ -555,41 +555,104 @@ __device__ inline void cr_load_diag(double (&a)[SL][SL], const VkCol& col, const
       if (s * VK_G + tr >= col.ni || t * VK_G + tc >= col.ni) a[s][t] = 0.0;
 }
 
-// acc += A B for tile-layout blocks in global memory, through 16-wide LDS
-// panels (ga: the 16 columns of A, gb: the 16 rows of B).
+// Dense products on FP64 MFMA (v_mfma_f64_16x16x4f64).  Output tile t =
+// (a, b) of the N x N result belongs to wave t mod 4, as T[t / 4] in the
+// MFMA result layout: lane l, item v holds element (16a + l/16 + 4v,
+// 16b + l%16).  Operands are tile-layout blocks in global memory, staged in
+// 16-wide panels: ga[row][17] (padded against bank conflicts), gb[k][N].
+typedef double cr_d4 __attribute__((ext_vector_type(4)));
 template <int SL>
-__device__ inline void cr_gemm(const double* __restrict__ A, const double* __restrict__ B,
-                               double (&acc)[SL][SL], double* ga, double* gb) {
-  const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
-#pragma unroll 1
-  for (int tp = 0; tp < SL; ++tp) {
+constexpr int cr_tpw() { return (SL * SL + 3) / 4; }
+
+template <int SL>
+__device__ inline void cr_mm_zero(cr_d4 (&T)[cr_tpw<SL>()]) {
 #pragma unroll
-    for (int s = 0; s < SL; ++s) ga[(s * VK_G + tr) * VK_G + tc] = A[(s * SL + tp) * VK_NT + tid];
+  for (int s = 0; s < cr_tpw<SL>(); ++s) T[s] = cr_d4{0.0, 0.0, 0.0, 0.0};
+}
+
+template <int SL>
+__device__ inline void cr_mm_load(cr_d4 (&T)[cr_tpw<SL>()], const double* __restrict__ src) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane >> 4, lc = lane & 15;
 #pragma unroll
-    for (int u = 0; u < SL; ++u) gb[(u * VK_G + tr) * VK_G + tc] = B[(tp * SL + u) * VK_NT + tid];
-    __syncthreads();
-#pragma unroll 4
-    for (int pp = 0; pp < VK_G; ++pp) {
-      double av[SL], bv[SL];
+  for (int s = 0; s < cr_tpw<SL>(); ++s) {
+    const int t = wave + 4 * s;
+    if (t < SL * SL) {
 #pragma unroll
-      for (int s = 0; s < SL; ++s) av[s] = ga[(s * VK_G + tr) * VK_G + pp];
-#pragma unroll
-      for (int u = 0; u < SL; ++u) bv[u] = gb[(u * VK_G + pp) * VK_G + tc];
-#pragma unroll
-      for (int s = 0; s < SL; ++s)
-#pragma unroll
-        for (int u = 0; u < SL; ++u) acc[s][u] = fma(av[s], bv[u], acc[s][u]);
+      for (int v = 0; v < 4; ++v) T[s][v] = src[t * VK_NT + (lr + 4 * v) * VK_G + lc];
     }
-    __syncthreads();
   }
 }
 
 template <int SL>
-__device__ inline void cr_zero(double (&a)[SL][SL]) {
+__device__ inline void cr_mm_store(const cr_d4 (&T)[cr_tpw<SL>()], double* __restrict__ dst) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane >> 4, lc = lane & 15;
 #pragma unroll
-  for (int s = 0; s < SL; ++s)
+  for (int s = 0; s < cr_tpw<SL>(); ++s) {
+    const int t = wave + 4 * s;
+    if (t < SL * SL) {
 #pragma unroll
-    for (int t = 0; t < SL; ++t) a[s][t] = 0.0;
+      for (int v = 0; v < 4; ++v) dst[t * VK_NT + (lr + 4 * v) * VK_G + lc] = T[s][v];
+    }
+  }
+}
+
+// One 16-wide panel of T += sgn * A B from ga / gb.
+template <int SL>
+__device__ inline void cr_mm_panel(double sgn, cr_d4 (&T)[cr_tpw<SL>()], const double* ga,
+                                   const double* gb) {
+  constexpr int N = VK_G * SL, GA = VK_G + 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane >> 4, lc = lane & 15;
+#pragma unroll
+  for (int s = 0; s < cr_tpw<SL>(); ++s) {
+    const int t = wave + 4 * s;
+    if (t < SL * SL) {
+      const int a = t / SL, b = t - a * SL;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double av = sgn * ga[(a * VK_G + lc) * GA + 4 * q + lr];
+        const double bw = gb[(4 * q + lr) * N + b * VK_G + lc];
+        T[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bw, T[s], 0, 0, 0);
+      }
+    }
+  }
+}
+
+// T += sgn * A B (sgn = +1 or -1, exact).  PRE: both operands are loaded
+// into registers first, so their latencies overlap (a workgroup alone on its
+// CU has nothing else to hide them behind); it costs 144 registers, which
+// the update kernel has to spare and the elimination kernel (two waves per
+// SIMD) does not, so there the panels are loaded one at a time.
+template <int SL, bool PRE>
+__device__ inline void cr_mm(const double* __restrict__ A, const double* __restrict__ B,
+                             double sgn, cr_d4 (&T)[cr_tpw<SL>()], double* ga, double* gb) {
+  constexpr int N = VK_G * SL, GA = VK_G + 1;
+  const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  if constexpr (PRE) {
+    double ra[SL][SL], rb[SL][SL];
+    load_tile<SL>(ra, A);
+    load_tile<SL>(rb, B);
+#pragma unroll
+    for (int p = 0; p < SL; ++p) {
+#pragma unroll
+      for (int s = 0; s < SL; ++s) ga[(s * VK_G + tr) * GA + tc] = ra[s][p];
+#pragma unroll
+      for (int u = 0; u < SL; ++u) gb[tr * N + u * VK_G + tc] = rb[p][u];
+      __syncthreads();
+      cr_mm_panel<SL>(sgn, T, ga, gb);
+      __syncthreads();
+    }
+  } else {
+#pragma unroll 1
+    for (int p = 0; p < SL; ++p) {
+#pragma unroll
+      for (int s = 0; s < SL; ++s) ga[(s * VK_G + tr) * GA + tc] = A[(s * SL + p) * VK_NT + tid];
+#pragma unroll
+      for (int u = 0; u < SL; ++u) gb[tr * N + u * VK_G + tc] = B[(p * SL + u) * VK_NT + tid];
+      __syncthreads();
+      cr_mm_panel<SL>(sgn, T, ga, gb);
+      __syncthreads();
+    }
+  }
 }
 
 __device__ inline void cr_epilogue(const SolveEpi& epi, int64_t e, double xv, double* x) {
@@ -616,7 +679,7 @@ __global__ __launch_bounds__(VK_NT) void k_cr_elim(const VkCol* __restrict__ col
                                                     int* __restrict__ status,
                                                     const int* __restrict__ active) {
   __shared__ BtShared<SL> sh;
-  __shared__ double ga[VK_G * VK_G * SL], gb[VK_G * VK_G * SL];
+  __shared__ double ga[(VK_G + 1) * VK_G * SL], gb[VK_G * VK_G * SL];
   const int c = blockIdx.y, w = blockIdx.x;
   if (c >= ncols) return;
   if (active && (active[c] & VK_ST_DONE)) return;
@@ -659,19 +722,21 @@ __global__ __launch_bounds__(VK_NT) void k_cr_elim(const VkCol* __restrict__ col
   } else {
     __syncthreads();  // inv(D_j) is read back through the panels
     const double* Dinv = cr + v.Dinv + (int64_t)w * BLK;
-    double acc[SL][SL];
-    cr_zero<SL>(acc);
-    cr_gemm<SL>(cr + v.U + (int64_t)(j - 1) * BLK, Dinv, acc, ga, gb);
-    store_tile<SL>(acc, cr + v.P + (int64_t)w * BLK);
+    cr_d4 T[cr_tpw<SL>()];
+    cr_mm_zero<SL>(T);
+    cr_mm<SL, false>(cr + v.U + (int64_t)(j - 1) * BLK, Dinv, 1.0, T, ga, gb);
+    cr_mm_store<SL>(T, cr + v.P + (int64_t)w * BLK);
     if (has_q) {
-      cr_zero<SL>(acc);
-      cr_gemm<SL>(cr + v.L + (int64_t)(j + 1) * BLK, Dinv, acc, ga, gb);
-      store_tile<SL>(acc, cr + v.Q + (int64_t)w * BLK);
+      cr_mm_zero<SL>(T);
+      cr_mm<SL, false>(cr + v.L + (int64_t)(j + 1) * BLK, Dinv, 1.0, T, ga, gb);
+      cr_mm_store<SL>(T, cr + v.Q + (int64_t)w * BLK);
     }
   }
 }
 
 // Even rows of one level become the rows of the next.  Grid (ceil(n/2), ncols).
+// Level 0 scales by the diffusion diagonals; later levels multiply dense
+// blocks on MFMA.
 template <int SL>
 __global__ __launch_bounds__(VK_NT) void k_cr_update(const VkCol* __restrict__ cols, int ncols,
                                                       const double* __restrict__ blk0,
@@ -679,7 +744,7 @@ __global__ __launch_bounds__(VK_NT) void k_cr_update(const VkCol* __restrict__ c
                                                       const double* __restrict__ sub_d,
                                                       double* __restrict__ cr, int level,
                                                       const int* __restrict__ active) {
-  __shared__ double ga[VK_G * VK_G * SL], gb[VK_G * VK_G * SL];
+  __shared__ double ga[(VK_G + 1) * VK_G * SL], gb[VK_G * VK_G * SL];
   const int c = blockIdx.y, w = blockIdx.x;
   if (c >= ncols) return;
   if (active && (active[c] & VK_ST_DONE)) return;
@@ -690,83 +755,73 @@ __global__ __launch_bounds__(VK_NT) void k_cr_update(const VkCol* __restrict__ c
   const int i = 2 * w;
   if (v.n <= 1 || i >= v.n) return;
   const CrLvl vn = cr_level(col.nz, BLK, N, 7 * col.off_b, level + 1);
+  const bool has_p = i + 1 < v.n, has_u = i + 2 < v.n;
+  if (level >= 1) {
+    // D' = (D_i - Q_{i-1} U_{i-1}) - P_{i+1} L_{i+1}; L' = -Q_{i-1} L_{i-1};
+    // U' = -P_{i+1} U_{i+1}
+    cr_d4 T[cr_tpw<SL>()], X[cr_tpw<SL>()];
+    cr_mm_load<SL>(T, cr + v.D + (int64_t)i * BLK);
+    if (i >= 1) {
+      const double* Qp = cr + v.Q + (int64_t)(w - 1) * BLK;
+      cr_mm<SL, true>(Qp, cr + v.U + (int64_t)(i - 1) * BLK, -1.0, T, ga, gb);
+      if (i >= 2) {
+        cr_mm_zero<SL>(X);
+        cr_mm<SL, true>(Qp, cr + v.L + (int64_t)(i - 1) * BLK, -1.0, X, ga, gb);
+        cr_mm_store<SL>(X, cr + vn.L + (int64_t)w * BLK);
+      }
+    }
+    if (has_p) {
+      const double* Pp = cr + v.P + (int64_t)w * BLK;
+      cr_mm<SL, true>(Pp, cr + v.L + (int64_t)(i + 1) * BLK, -1.0, T, ga, gb);
+      if (has_u) {
+        cr_mm_zero<SL>(X);
+        cr_mm<SL, true>(Pp, cr + v.U + (int64_t)(i + 1) * BLK, -1.0, X, ga, gb);
+        cr_mm_store<SL>(X, cr + vn.U + (int64_t)w * BLK);
+      }
+    }
+    cr_mm_store<SL>(T, cr + vn.D + (int64_t)w * BLK);
+    return;
+  }
   const int ni = col.ni, tid = threadIdx.x, tc = tid & 15;
   double d[SL][SL], m[SL][SL];
   cr_load_diag<SL>(d, col, blk0, cr, v, level, i);
-  if (i >= 1) {  // odd row i-1: Q_{i-1} = L_i inv(D_{i-1})
-    const double* Qp = cr + v.Q + (int64_t)(w - 1) * BLK;
-    if (level == 0) {
-      // U_{i-1} = diag(sup_d[i-1]), L_{i-1} = diag(sub_d[i-2]); m becomes L'
-      load_tile<SL>(m, Qp);
-      const double* bu = sup_d + col.off_s + (int64_t)(i - 1) * ni;
-      const double* bl = sub_d + col.off_s + (int64_t)(i >= 2 ? i - 2 : 0) * ni;
+  if (i >= 1) {
+    // odd row i-1: Q_{i-1} = L_i inv(D_{i-1}); U_{i-1} = diag(sup_d[i-1]),
+    // L_{i-1} = diag(sub_d[i-2]); m becomes L'
+    load_tile<SL>(m, cr + v.Q + (int64_t)(w - 1) * BLK);
+    const double* bu = sup_d + col.off_s + (int64_t)(i - 1) * ni;
+    const double* bl = sub_d + col.off_s + (int64_t)(i >= 2 ? i - 2 : 0) * ni;
 #pragma unroll
-      for (int t = 0; t < SL; ++t) {
-        const int q = t * VK_G + tc;
-        const double b = q < ni ? bu[q] : 0.0;
-        const double b2 = (q < ni && i >= 2) ? bl[q] : 0.0;
+    for (int t = 0; t < SL; ++t) {
+      const int q = t * VK_G + tc;
+      const double b = q < ni ? bu[q] : 0.0;
+      const double b2 = (q < ni && i >= 2) ? bl[q] : 0.0;
 #pragma unroll
-        for (int s = 0; s < SL; ++s) {
-          d[s][t] = d[s][t] - m[s][t] * b;
-          m[s][t] = -(m[s][t] * b2);
-        }
-      }
-      if (i >= 2) store_tile<SL>(m, cr + vn.L + (int64_t)w * BLK);
-    } else {
-      cr_zero<SL>(m);
-      cr_gemm<SL>(Qp, cr + v.U + (int64_t)(i - 1) * BLK, m, ga, gb);
-#pragma unroll
-      for (int s = 0; s < SL; ++s)
-#pragma unroll
-        for (int t = 0; t < SL; ++t) d[s][t] = d[s][t] - m[s][t];
-      if (i >= 2) {
-        cr_zero<SL>(m);
-        cr_gemm<SL>(Qp, cr + v.L + (int64_t)(i - 1) * BLK, m, ga, gb);
-#pragma unroll
-        for (int s = 0; s < SL; ++s)
-#pragma unroll
-          for (int t = 0; t < SL; ++t) m[s][t] = -m[s][t];
-        store_tile<SL>(m, cr + vn.L + (int64_t)w * BLK);
+      for (int s = 0; s < SL; ++s) {
+        d[s][t] = d[s][t] - m[s][t] * b;
+        m[s][t] = -(m[s][t] * b2);
       }
     }
+    if (i >= 2) store_tile<SL>(m, cr + vn.L + (int64_t)w * BLK);
   }
-  if (i + 1 < v.n) {  // odd row i+1: P_{i+1} = U_i inv(D_{i+1})
-    const double* Pp = cr + v.P + (int64_t)w * BLK;
-    const bool has_u = i + 2 < v.n;
-    if (level == 0) {
-      // L_{i+1} = diag(sub_d[i]), U_{i+1} = diag(sup_d[i+1]); m becomes U'
-      load_tile<SL>(m, Pp);
-      const double* bl = sub_d + col.off_s + (int64_t)i * ni;
-      const double* bu = sup_d + col.off_s + (int64_t)(has_u ? i + 1 : i) * ni;
+  if (has_p) {
+    // odd row i+1: P_{i+1} = U_i inv(D_{i+1}); L_{i+1} = diag(sub_d[i]),
+    // U_{i+1} = diag(sup_d[i+1]); m becomes U'
+    load_tile<SL>(m, cr + v.P + (int64_t)w * BLK);
+    const double* bl = sub_d + col.off_s + (int64_t)i * ni;
+    const double* bu = sup_d + col.off_s + (int64_t)(has_u ? i + 1 : i) * ni;
 #pragma unroll
-      for (int t = 0; t < SL; ++t) {
-        const int q = t * VK_G + tc;
-        const double b = q < ni ? bl[q] : 0.0;
-        const double b2 = (q < ni && has_u) ? bu[q] : 0.0;
+    for (int t = 0; t < SL; ++t) {
+      const int q = t * VK_G + tc;
+      const double b = q < ni ? bl[q] : 0.0;
+      const double b2 = (q < ni && has_u) ? bu[q] : 0.0;
 #pragma unroll
-        for (int s = 0; s < SL; ++s) {
-          d[s][t] = d[s][t] - m[s][t] * b;
-          m[s][t] = -(m[s][t] * b2);
-        }
-      }
-      if (has_u) store_tile<SL>(m, cr + vn.U + (int64_t)w * BLK);
-    } else {
-      cr_zero<SL>(m);
-      cr_gemm<SL>(Pp, cr + v.L + (int64_t)(i + 1) * BLK, m, ga, gb);
-#pragma unroll
-      for (int s = 0; s < SL; ++s)
-#pragma unroll
-        for (int t = 0; t < SL; ++t) d[s][t] = d[s][t] - m[s][t];
-      if (has_u) {
-        cr_zero<SL>(m);
-        cr_gemm<SL>(Pp, cr + v.U + (int64_t)(i + 1) * BLK, m, ga, gb);
-#pragma unroll
-        for (int s = 0; s < SL; ++s)
-#pragma unroll
-          for (int t = 0; t < SL; ++t) m[s][t] = -m[s][t];
-        store_tile<SL>(m, cr + vn.U + (int64_t)w * BLK);
+      for (int s = 0; s < SL; ++s) {
+        d[s][t] = d[s][t] - m[s][t] * b;
+        m[s][t] = -(m[s][t] * b2);
       }
     }
+    if (has_u) store_tile<SL>(m, cr + vn.U + (int64_t)w * BLK);
   }
   store_tile<SL>(d, cr + vn.D + (int64_t)w * BLK);
 }
