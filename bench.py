@@ -125,7 +125,7 @@ def main() -> int:
     ap.add_argument("--serve-factor", choices=("cr", "auto"), default="cr",
                     help="block factor of the serving pools: cyclic reduction (latency mode) "
                          "or the throughput kernels (--factor)")
-    ap.add_argument("--serve-alone", type=int, default=0,
+    ap.add_argument("--serve-alone", type=int, default=2,
                     help="requests also solved one at a time, for the comparison")
     ap.add_argument("--factor", choices=("auto", "tile", "rows", "blocked", "tile2", "wave", "tile8", "cr"),
                     default="auto",

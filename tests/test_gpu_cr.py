@@ -14,7 +14,8 @@ another order, so the bar is agreement to the system's conditioning:
   two and one above): each column equals the same column stepped alone bit
   for bit, and the tile factor to 1e-12 at a small dt;
 * steady-state attempts replayed from a HIP graph equal the eager ones bit
-  for bit (the factor is some 30 launches per attempt).
+  for bit (the factor and the two solves are about 50 launches per attempt);
+* a workspace sized for the tile factor is refused, not overrun.
 """
 from __future__ import annotations
 
@@ -150,3 +151,21 @@ def test_cr_attempt_graph_replay_equals_eager():
         for a, e in zip((r.y, r.dt, r.accept, r.status), eager):
             assert torch.equal(a, e)
         assert int(r.accept.min().item()) >= 10
+
+
+def test_cr_refuses_a_workspace_sized_for_the_tile_factor():
+    """A runner built under the tile factor holds the smaller workspace; once
+    its bucket is tuned to CR the attempt is refused with an error instead of
+    writing past it, and it runs again after the tune is restored."""
+    import torch
+    from vulcan_amd import _lib, synth
+    from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
+    col = synth.make_column(501, nz=24, ni=20, n_rev=55, dt=1e-10, p_bot=1e5, p_top=1e-1,
+                            consistent=True)
+    r = SteadyStateRunner([col], RunConfig())
+    with factor(VK_FACTOR_CR):
+        with pytest.raises(_lib.VulcanError, match="workspace"):
+            r.attempt()
+    r.attempt()
+    torch.cuda.synchronize()
+    assert int(r.attempts.item()) == 1
