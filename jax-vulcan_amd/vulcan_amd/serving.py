@@ -231,11 +231,16 @@ class ServingEngine:
     cap_T); by default it is derived from the requests given to ``run``.
     Each bucket gets ``min(max_slots, budget share / bytes per slot)``
     slots, the budget shared in proportion to the bucket's request count.
+    Requests stop by the relative-change rule (RunConfig.conv_mode
+    "relmax"); the ring of the longdy rule is not kept per slot.
     """
 
     def __init__(self, hbm_budget_bytes: float = 200e9, max_slots: int = 512,
                  cfg: RunConfig = RunConfig(), chunk: int = 20, use_graph: bool = True,
                  capacity: Optional[Dict[int, tuple]] = None):
+        if cfg.conv_mode != "relmax":
+            raise ValueError("serving pools stop requests by the relative-change rule "
+                             "(conv_mode 'relmax')")
         self.budget = float(hbm_budget_bytes)
         self.max_slots = int(max_slots)
         self.cfg = cfg

@@ -169,3 +169,16 @@ def test_cr_refuses_a_workspace_sized_for_the_tile_factor():
     r.attempt()
     torch.cuda.synchronize()
     assert int(r.attempts.item()) == 1
+
+
+def test_cr_workspace_is_seven_block_stacks_larger():
+    from vulcan_amd import _lib
+    from vulcan_amd.core import Batch, device_network
+    cols = _uneven()[:4]
+    b = Batch([c.nz for c in cols], [device_network(c.net) for c in cols], [0, 1, 2, 3])
+    lib = _lib.load()
+    plain = int(lib.vk_ros2_workspace_bytes(b.handle))
+    with factor(VK_FACTOR_CR):
+        cr = int(lib.vk_ros2_workspace_bytes(b.handle))
+    blk = int(lib.vk_batch_block_doubles(b.handle))
+    assert cr - plain >= 7 * blk * 8 and cr - plain < 7 * blk * 8 + 4096
