@@ -131,6 +131,10 @@ def main() -> int:
                     default="auto",
                     help="block-factor kernel for the nz=150 / ni=93 bucket (auto: the "
                          "library default)")
+    ap.add_argument("--conv-cost", type=int, default=0,
+                    help="columns (nz=150, ni=93) for timing one attempt under the "
+                         "relative-change rule and under the reference's longdy rule "
+                         "(ring of 500); 0 skips it")
     ap.add_argument("--no-rates", action="store_true",
                     help="skip the rate-construction measurement")
     ap.add_argument("--rates-only", action="store_true",
@@ -341,6 +345,27 @@ def main() -> int:
                             "change < 1e-4 or 5000 accepted steps"}
         print(f"[bench] serving: {serving}", file=sys.stderr, flush=True)
 
+    conv_cost = None
+    if args.conv_cost > 0:
+        from vulcan_amd.outer_loop import ColumnConv, RunConfig, SteadyStateRunner
+        ccols = synth.columns_same_network(1002, args.conv_cost, NZ, NI, 1200)
+        cconv = [ColumnConv(Hp=synth.compute_geometry(c.P, c.atm.Tco)["Hp"]) for c in ccols]
+        conv_cost = {"columns": args.conv_cost, "conv_step": 500}
+        for mode in ("relmax", "longdy"):
+            rr = SteadyStateRunner(ccols, RunConfig(conv_mode=mode, conv_step=500),
+                                   conv=cconv if mode == "longdy" else None)
+            rr.capture(10)
+            rr.graph[0].replay()
+            torch.cuda.synchronize()
+            rr.reset_to_initial()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rr.graph[0].replay()
+            torch.cuda.synchronize()
+            conv_cost[f"ms_per_attempt_{mode}"] = round((time.perf_counter() - t0) / 10 * 1e3, 3)
+            del rr
+        print(f"[bench] stopping-rule cost: {conv_cost}", file=sys.stderr, flush=True)
+
     rate_build = None
     if not args.no_rates:
         rate_build = bench_rates(B, net, atm, no_cpu=args.no_cpu)
@@ -410,6 +435,7 @@ def main() -> int:
         "singular_columns": n_singular,
         "uneven_batch": uneven,
         "serving": serving,
+        "stopping_rule_cost": conv_cost,
         "rate_build": rate_build,
         "roofline": roofline,
         "cpu_baseline": cpu,
