@@ -157,3 +157,71 @@ def test_low_temperature_caps_on_the_hd189_network():
     cap = 6.0e-29 / (1.0 + 6.0e-29 * M / kinf)
     m = T <= 277.5
     assert np.max(np.abs(k[665][m] - cap[m]) / cap[m]) < 1e-13
+
+
+def _to_steady_state(kzz, cfg, conv=False):
+    from vulcan_amd import _lib, network as N, synth
+    from vulcan_amd.outer_loop import ColumnConv, SteadyStateRunner
+    net, c9, _, y, atm = _column(nz=40, kzz=kzz, seed=3)
+    k = _k(net, c9, atm)
+    arr = N.to_arrays(net)
+    col = SimpleNamespace(nz=40, ni=net.ni, y=y, k=k, atm=atm, net=arr, dt=1e-10, photo=None)
+    lib = _lib.load()
+    try:
+        for sl in range(1, 8):
+            _lib.check(lib.vk_tune_set(0, sl, 6), "vk_tune_set")   # cyclic reduction
+        P = np.logspace(np.log10(1e9), np.log10(1e3), 40)
+        hp = [ColumnConv(Hp=synth.compute_geometry(P, atm.Tco)["Hp"])] if conv else None
+        r = SteadyStateRunner([col], cfg, conv=hp)
+        r.run(max_attempts=2 * cfg.count_max, check_every=100, use_graph=True)
+    finally:
+        for sl in range(1, 8):
+            lib.vk_tune_set(0, sl, 0)
+    return r, net, arr, k, atm
+
+
+def test_real_column_reaches_thermochemical_equilibrium():
+    """Kzz = 0, no photolysis: every layer of the HD 189733b-like column runs
+    to a steady state on the device (device-built rates, cyclic-reduction
+    factor), which must be thermochemical equilibrium, since the reverse
+    rates come from the NASA-9 Gibbs energies: each reversible pair with a
+    rate above 1e-6 of its layer's largest runs forward and backward at the
+    same rate.  The median pair balances to 1e-8 of its gross rate; the
+    hydrostatic rebalance (every accepted state rescaled to n0) keeps the
+    fixed point a little off exact equilibrium, so the worst pair is held
+    to 1e-2 and chem_rhs to 1e-3 of each layer's gross rate."""
+    from oracle import ros2_np as O
+    from vulcan_amd.outer_loop import ST_CONVERGED, RunConfig
+    r, net, arr, k, atm = _to_steady_state(0.0, RunConfig(count_max=20000))
+    res = r.results()
+    assert res["status"][0] & ST_CONVERGED, res["status"]
+    ys = res["y"][0]
+    ypad = np.concatenate([ys, np.ones((40, 1))], axis=1)
+    rate = k.T * np.prod(ypad[:, arr.reactant_idx] ** arr.reactant_stoich[None], axis=2)
+    rate = np.where(arr.is_three_body[None, :], rate * atm.M[:, None], rate)
+    fw = np.arange(1, net.stop_rev_indx, 2)
+    Rf, Rr = rate[:, fw], rate[:, fw + 1]
+    both = (k[fw] > 0) & (k[fw + 1] > 0)
+    big = both.T & (np.maximum(Rf, Rr) > 1e-6 * rate.max(axis=1, keepdims=True))
+    rel = np.abs(Rf - Rr)[big] / np.maximum(Rf, Rr)[big]
+    f = O.chem_rhs(ys, atm.M, k, arr)
+    resid = np.max(np.abs(f).max(axis=1) / np.abs(rate).sum(axis=1))
+    print(f"accepted {res['accept'][0]}, t {res['t'][0]:.3e}; {big.sum()} pairs: net/gross "
+          f"median {np.median(rel):.2e}, max {rel.max():.2e}; |f|/gross {resid:.2e}")
+    assert big.sum() > 500
+    assert np.median(rel) < 1e-8 and rel.max() < 1e-2 and resid < 1e-3
+
+
+def test_real_column_stops_by_the_reference_rule():
+    """Kzz = 1e9: the same column under the reference's stopping rule
+    (longdy against a 500-state ring) stops converged, by the strict or the
+    loose criterion, well inside count_max."""
+    from vulcan_amd.outer_loop import ST_CONVERGED, RunConfig
+    cfg = RunConfig(conv_mode="longdy", count_max=20000)
+    r, *_ = _to_steady_state(1e9, cfg, conv=True)
+    res = r.results()
+    ld, ldt = r.conv_state()[0, :2]
+    print(f"accepted {res['accept'][0]}, longdy {ld:.3e}, longdydt {ldt:.3e}")
+    assert res["status"][0] & ST_CONVERGED
+    assert res["accept"][0] < 10000
+    assert (ld < cfg.yconv_cri and ldt < cfg.slope_cri) or (ld < cfg.yconv_min and ldt < 1e-8)
