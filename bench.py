@@ -314,26 +314,21 @@ def main() -> int:
                 for i, (a, b_, r) in enumerate(sizes)]
         arrivals = synth.poisson_arrivals(1005, args.serve, args.serve_rate)
         arrivals = arrivals - arrivals[0]
-        serve_sls = sorted({(int(r.ni) + 15) // 16 for r in reqs})
-        if args.serve_factor == "cr":
-            for sl in serve_sls:
-                _lib.check(lib.vk_tune_set(0, sl, FACTOR_CR), "vk_tune_set")
-        elif args.factor == "auto":
+        latency = args.serve_factor == "cr"
+        if not latency and args.factor == "auto":
             autotune.tune_columns(reqs, verbose=True)
-        eng = ServingEngine(hbm_budget_bytes=200e9, cfg=RunConfig(), chunk=25)
+        eng = ServingEngine(hbm_budget_bytes=200e9, cfg=RunConfig(), chunk=25,
+                            latency_mode=latency)
         t0 = time.perf_counter()
         res = eng.run(reqs, arrivals, report_every=30.0)
         wall = time.perf_counter() - t0
         summ = latency_summary(res)
         alone = []
         for i in range(min(args.serve_alone, len(reqs))):
-            r1 = SteadyStateRunner([reqs[i]], RunConfig())
+            r1 = SteadyStateRunner([reqs[i]], RunConfig(), latency_mode=latency)
             t1 = time.perf_counter()
             r1.run(max_attempts=100000, check_every=25, use_graph=True, report_every=30.0)
             alone.append(time.perf_counter() - t1)
-        if args.serve_factor == "cr":
-            for sl in serve_sls:
-                lib.vk_tune_set(0, sl, 0)
         serving = {"factor_kernel": args.serve_factor,"requests": len(reqs), "arrival_rate_per_s": args.serve_rate,
                    "hbm_budget_gb": 200, "slots": {str(k): v for k, v in eng.slot_counts.items()},
                    "wall_s": round(wall, 3),

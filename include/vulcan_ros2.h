@@ -93,6 +93,14 @@ int vk_batch_slots(int nslots, int cap_nz, int cap_ni, int cap_S, int cap_T, vk_
  * rewritten by a one-thread kernel on the stream, so it is ordered with the
  * launches around it and legal between graph replays. */
 int vk_batch_set_column(vk_batch b, int slot, int nz, vk_network net, void* stream);
+/* Latency mode (no counterpart in the reference): on = 1 makes every bucket
+ * of b use the cyclic-reduction block factor (VK_FACTOR_CR) in
+ * vk_ros2_step_f64 and vk_ss_attempt_f64, whatever the process-wide
+ * VK_TUNE_FACTOR says; on = 0 follows it again.  For batches of a few
+ * columns per bucket, such as serving pools.  Set it before sizing the
+ * workspace: vk_ros2_workspace_bytes(b) then includes the 7 block stacks
+ * of the reduction levels. */
+int vk_batch_set_latency_mode(vk_batch b, int on);
 /* Doubles needed by a block stack of one layer with ni species. */
 int64_t vk_block_doubles(int ni);
 /* Total doubles of the packed block stacks of a batch. */

@@ -148,6 +148,9 @@ struct vk_batch_s {
   std::vector<std::pair<int, int>> net_dims;
   VkRatesDev* d_rates = nullptr;
   int n_rates = 0;
+  // latency mode: every bucket of this batch uses the cyclic-reduction
+  // factor, whatever the process-wide VK_TUNE_FACTOR says
+  bool latency_mode = false;
 };
 
 struct vk_rates_s {
@@ -792,12 +795,26 @@ static size_t ros2_ws_base_bytes(vk_batch b) {
          align_up<double>(b->k_doubles);
 }
 
+// Whether bucket sl of b uses the cyclic-reduction factor.
+static bool bucket_is_cr(vk_batch b, int sl) { return b->latency_mode || vk_factor_is_cr(sl); }
+
 // Whether a bucket of b uses the cyclic-reduction factor (it then needs 7
 // block stacks of extra workspace).
 static bool batch_uses_cr(vk_batch b) {
   for (int sl : b->slots)
-    if (vk_factor_is_cr(sl)) return true;
+    if (bucket_is_cr(b, sl)) return true;
   return false;
+}
+
+int vk_batch_set_latency_mode(vk_batch b, int on) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  if (on != 0 && on != 1) {
+    vk_set_error("vk_batch_set_latency_mode: on must be 0 or 1, not %d", on);
+    return VK_ERR_ARG;
+  }
+  b->latency_mode = on != 0;
+  return VK_OK;
 }
 
 size_t vk_ros2_workspace_bytes(vk_batch b) {
@@ -875,8 +892,8 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
     // The factor fuses the stage-1 forward sweep unless the blocked variant
     // is selected; then the stage-1 solve runs both sweeps.  The
     // cyclic-reduction variant has its own factor and solves.
-    const bool cr = vk_factor_is_cr(sl);
-    const bool fused = vk_factor_fuses_sweep(sl);
+    const bool cr = bucket_is_cr(b, sl);
+    const bool fused = !cr && vk_factor_fuses_sweep(sl);
     {
       KernelTimer t(1, s);
       if (cr)

@@ -30,6 +30,7 @@ _SIGS = {
     "vk_batch_destroy": (i32, [p]),
     "vk_block_doubles": (i64, [i32]),
     "vk_batch_block_doubles": (i64, [p]),
+    "vk_batch_set_latency_mode": (i32, [p, i32]),
     "vk_atm_record_layout": (i64, [i32, i32, p]),
     "vk_chem_rhs_f64": (i32, [p, p, p, p, p, p, p]),
     "vk_chem_jac_f64": (i32, [p, p, p, p, p, p, p]),

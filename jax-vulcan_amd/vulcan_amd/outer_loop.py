@@ -173,14 +173,17 @@ class SteadyStateRunner:
 
     def __init__(self, columns: Sequence, cfg: RunConfig = RunConfig(), compo=None,
                  photo: Optional[object] = None, refresh: Optional[Sequence] = None,
-                 constraints: Optional[Sequence] = None, conv: Optional[Sequence] = None):
+                 constraints: Optional[Sequence] = None, conv: Optional[Sequence] = None,
+                 latency_mode: bool = False):
         """refresh: one ``atm_refresh.AtmRefreshStatic`` per column; with
         cfg.update_frq > 0 each column's geometry (g, dzi, Hpi) and escape
         fluxes are refreshed in the loop after every update_frq-th accepted
         step (outer_loop.py:809-843).  constraints: one ``ColumnConstraints``
         (or None) per column.  conv: one ``ColumnConv`` per column, needed
         by cfg.conv_mode "longdy" (with the refresh on, slope_min follows
-        the refreshed Hp)."""
+        the refreshed Hp).  latency_mode: the cyclic-reduction block factor
+        for this batch (vk_batch_set_latency_mode), for a few columns per
+        bucket."""
         self.dev = device()
         self.cfg = cfg
         cols = list(columns)
@@ -219,6 +222,8 @@ class SteadyStateRunner:
          self.attempts, self.photo_due) = (zi() for _ in range(8))
         zero_bot = any(bool(c.atm.use_botflux) for c in cols)
         self.params = cfg.c_params(self.na, zero_bot)
+        _lib.check(_lib.load().vk_batch_set_latency_mode(b.handle, int(bool(latency_mode))),
+                   "vk_batch_set_latency_mode")
         self.ws = b.workspace()
         self.photo = None
         if photo is not None or any(getattr(c, "photo", None) is not None for c in cols):

@@ -83,7 +83,8 @@ class SlotPool:
     """Fixed-capacity slots for one 16-species bucket, on its own stream."""
 
     def __init__(self, nslots: int, cap_nz: int, cap_ni: int, cap_S: int, cap_T: int,
-                 cfg: RunConfig, use_graph: bool = True, chunk: int = 20):
+                 cfg: RunConfig, use_graph: bool = True, chunk: int = 20,
+                 latency_mode: bool = True):
         lib = _lib.load()
         self.dev = device()
         self.nslots, self.cap_nz, self.cap_ni, self.cap_S, self.cap_T = (
@@ -92,6 +93,10 @@ class SlotPool:
         _lib.check(lib.vk_batch_slots(self.nslots, self.cap_nz, self.cap_ni, self.cap_S,
                                       self.cap_T, C.byref(h)), "vk_batch_slots")
         self.handle = h
+        # the cyclic-reduction factor for this pool only (before the
+        # workspace is sized)
+        _lib.check(lib.vk_batch_set_latency_mode(h, int(bool(latency_mode))),
+                   "vk_batch_set_latency_mode")
         self.stream = torch.cuda.Stream(device=self.dev)  # default priority
         n, dev = self.nslots, self.dev
         self.cs = self.cap_nz * self.cap_ni
@@ -130,13 +135,13 @@ class SlotPool:
         self._fin = weakref.finalize(self, lib.vk_batch_destroy, h)
 
     @staticmethod
-    def bytes_per_slot(cap_nz: int, cap_ni: int, cap_S: int) -> int:
+    def bytes_per_slot(cap_nz: int, cap_ni: int, cap_S: int, latency_mode: bool = True) -> int:
         cs, ck = cap_nz * cap_ni, cap_S * cap_nz
         ca = atm_record_layout(cap_nz, cap_ni)[0]
         ws = cap_nz * block_doubles(cap_ni) + 16 * cs + ck  # Ros2 workspace per slot
         sl = (cap_ni + 15) // 16
-        if _lib.load().vk_tune_get(0, sl) == FACTOR_CR:   # the cyclic-reduction levels
-            ws += 7 * cap_nz * block_doubles(cap_ni)
+        if latency_mode or _lib.load().vk_tune_get(0, sl) == FACTOR_CR:
+            ws += 7 * cap_nz * block_doubles(cap_ni)      # the cyclic-reduction levels
         return 8 * (6 * cs + ck + ca + ws) + 8 * (3 * VK_MAX_ATOMS + 4) + 4 * 8
 
     # ------------------------------------------------------------------
@@ -233,14 +238,18 @@ class ServingEngine:
     slots, the budget shared in proportion to the bucket's request count.
     Requests stop by the relative-change rule (RunConfig.conv_mode
     "relmax"); the ring of the longdy rule is not kept per slot.
+    ``latency_mode`` (default on) gives every pool the cyclic-reduction
+    block factor (vk_batch_set_latency_mode): a pool holds a few columns per
+    bucket, where one workgroup per column would leave the GPU idle.
     """
 
     def __init__(self, hbm_budget_bytes: float = 200e9, max_slots: int = 512,
                  cfg: RunConfig = RunConfig(), chunk: int = 20, use_graph: bool = True,
-                 capacity: Optional[Dict[int, tuple]] = None):
+                 capacity: Optional[Dict[int, tuple]] = None, latency_mode: bool = True):
         if cfg.conv_mode != "relmax":
             raise ValueError("serving pools stop requests by the relative-change rule "
                              "(conv_mode 'relmax')")
+        self.latency_mode = bool(latency_mode)
         self.budget = float(hbm_budget_bytes)
         self.max_slots = int(max_slots)
         self.cfg = cfg
@@ -262,9 +271,10 @@ class ServingEngine:
         total = sum(counts.values()) or 1
         for sl, (cnz, cni, cS, cT) in sorted(caps.items()):
             share = self.budget * counts.get(sl, 0) / total if counts else self.budget / len(caps)
-            per = SlotPool.bytes_per_slot(cnz, cni, cS)
+            per = SlotPool.bytes_per_slot(cnz, cni, cS, self.latency_mode)
             n = int(min(self.max_slots, max(1, share // per), max(1, counts.get(sl, 1))))
-            self.pools[sl] = SlotPool(n, cnz, cni, cS, cT, self.cfg, self.use_graph, self.chunk)
+            self.pools[sl] = SlotPool(n, cnz, cni, cS, cT, self.cfg, self.use_graph, self.chunk,
+                                      self.latency_mode)
             self.slot_counts[sl] = n
 
     def run(self, requests: Sequence, arrivals: Optional[Sequence[float]] = None,

@@ -182,3 +182,42 @@ def test_cr_workspace_is_seven_block_stacks_larger():
         cr = int(lib.vk_ros2_workspace_bytes(b.handle))
     blk = int(lib.vk_batch_block_doubles(b.handle))
     assert cr - plain >= 7 * blk * 8 and cr - plain < 7 * blk * 8 + 4096
+
+
+def test_latency_mode_is_per_batch(cfg1):
+    """vk_batch_set_latency_mode gives one runner the cyclic-reduction factor
+    (bit for bit what the process-wide tune gives) and leaves the tune and
+    other runners on the tile factor.  At dt = 1e-10 both factors round
+    alike; workload 1 at its own dt (stiff blocks) tells them apart."""
+    import torch
+    from vulcan_amd import _lib
+    from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
+    from vulcan_amd import synth
+    cols = [synth.make_column(500 + c, nz=nz, ni=ni, n_rev=n_rev, dt=1e-10, p_bot=1e5,
+                              p_top=1e-1, consistent=True)
+            for c, (nz, ni, n_rev) in enumerate(((18, 14, 40), (24, 20, 55)))]
+    lib = _lib.load()
+    fast = SteadyStateRunner(cols, RunConfig(), latency_mode=True)
+    plain = SteadyStateRunner(cols, RunConfig())
+    assert all(lib.vk_tune_get(0, sl) == VK_FACTOR_TILE for sl in range(1, 8))
+    assert fast.ws.numel() > plain.ws.numel()
+    with factor(VK_FACTOR_CR):
+        tuned = SteadyStateRunner(cols, RunConfig())
+        for _ in range(25):
+            tuned.attempt()
+    with factor(VK_FACTOR_TILE):
+        tile = SteadyStateRunner(cols, RunConfig())
+        for _ in range(25):
+            tile.attempt()
+    for _ in range(25):
+        fast.attempt()
+        plain.attempt()
+    torch.cuda.synchronize()
+    assert torch.equal(fast.y, tuned.y) and torch.equal(fast.accept, tuned.accept)
+    assert torch.equal(plain.y, tile.y) and torch.equal(plain.accept, tile.accept)
+    stiff = [SteadyStateRunner([cfg1], RunConfig(), latency_mode=m) for m in (True, False)]
+    for r in stiff:
+        r.attempt()
+    torch.cuda.synchronize()
+    assert not torch.equal(stiff[0].sol, stiff[1].sol)
+

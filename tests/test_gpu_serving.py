@@ -20,23 +20,25 @@ def _requests():
             for i, (a, b, r) in enumerate(sizes)]
 
 
-def _alone(col):
+def _alone(col, latency=True):
+    """The request solved alone, on the block factor the engine's pools use
+    (the cyclic-reduction latency mode by default)."""
     from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
-    r = SteadyStateRunner([col], RunConfig(count_max=1500))
+    r = SteadyStateRunner([col], RunConfig(count_max=1500), latency_mode=latency)
     r.run(max_attempts=3000, check_every=100)
     res = r.results()
     return res["y"][0], int(res["attempts"][0]), int(res["status"][0])
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_served_results_equal_alone(use_graph):
+@pytest.mark.parametrize("use_graph,latency", [(True, True), (False, True), (True, False)])
+def test_served_results_equal_alone(use_graph, latency):
     from vulcan_amd.outer_loop import RunConfig
     from vulcan_amd.serving import ServingEngine, latency_summary
     reqs = _requests()
-    ref = [_alone(c) for c in reqs]
+    ref = [_alone(c, latency) for c in reqs]
     # two slots per bucket: later requests wait and refill slots freed by earlier ones
     eng = ServingEngine(hbm_budget_bytes=1e9, max_slots=2, cfg=RunConfig(count_max=1500),
-                        chunk=25, use_graph=use_graph)
+                        chunk=25, use_graph=use_graph, latency_mode=latency)
     arrivals = np.array([0.0, 0.01, 0.02, 0.2, 0.25, 0.3])
     out = eng.run(reqs, arrivals)
     assert len(out) == len(reqs)
