@@ -15,7 +15,10 @@ another order, so the bar is agreement to the system's conditioning:
   for bit, and the tile factor to 1e-12 at a small dt;
 * steady-state attempts replayed from a HIP graph equal the eager ones bit
   for bit (the factor and the two solves are about 50 launches per attempt);
-* a workspace sized for the tile factor is refused, not overrun.
+* a workspace sized for the tile factor is refused, not overrun;
+* the per-batch latency mode gives one runner the factor without touching
+  the process-wide tune, and runs with the in-loop atmosphere refresh on
+  the tile factor's trajectory at well-conditioned dt.
 """
 from __future__ import annotations
 
@@ -221,3 +224,29 @@ def test_latency_mode_is_per_batch(cfg1):
     torch.cuda.synchronize()
     assert not torch.equal(stiff[0].sol, stiff[1].sol)
 
+
+
+def test_latency_mode_with_the_in_loop_refresh():
+    """The cyclic-reduction factor inside the full attempt: molecular
+    diffusion, top fluxes and the in-loop atmosphere refresh every third
+    accepted step.  At these columns' dt the blocks are well conditioned, so
+    the trajectory follows the tile factor's: the same accept / reject
+    sequence and states within 1e-10 per layer after 30 attempts."""
+    import torch
+    from test_gpu_refresh import _columns
+    from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
+    cols, statics = _columns()
+    cfg = RunConfig(update_frq=3, count_max=400)
+    runs = [SteadyStateRunner(cols, cfg, refresh=statics, latency_mode=m) for m in (True, False)]
+    for _ in range(30):
+        for r in runs:
+            r.attempt()
+    torch.cuda.synchronize()
+    assert torch.equal(runs[0].accept, runs[1].accept)
+    assert torch.equal(runs[0].attempts, runs[1].attempts)
+    assert int(runs[0].accept.min().item()) >= 10
+    b = runs[0].batch
+    for c, (a, t) in enumerate(zip(b.unpack_state(runs[0].y), b.unpack_state(runs[1].y))):
+        a, t = a.cpu().numpy(), t.cpu().numpy()
+        err = np.max(np.abs(a - t), axis=1) / np.max(np.abs(t), axis=1)
+        assert np.all(np.isfinite(a)) and err.max() < 1e-10, (c, err.max())
