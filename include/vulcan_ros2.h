@@ -130,6 +130,19 @@ int vk_bt_factor_f64(vk_batch b, const double* diag, const double* sup_d, const 
 /* x = solve with a factored stack; rhs and x at off_s; scratch: state-sized. */
 int vk_bt_solve_f64(vk_batch b, const double* inv, const double* sup_d, const double* sub_d,
                     const double* rhs, double* x, double* scratch, void* stream);
+/* Latency mode of the two above (no counterpart in the reference; same
+ * system as solver.factor_block_thomas_diag_offdiag /
+ * solve_block_thomas_diag_offdiag, solver.py:33-112): block cyclic
+ * reduction, about log2(nz) levels of one workgroup per block row, for a
+ * few columns (the Newton polish and the adjoint solve of one column).
+ * Same arguments; the factored levels live in ws (vk_bt_cr_workspace_bytes:
+ * 8 block stacks), which the solve reads.  Results agree with the tile
+ * factor to the system's conditioning, not bit for bit. */
+size_t vk_bt_cr_workspace_bytes(vk_batch b);
+int vk_bt_cr_factor_f64(vk_batch b, const double* diag, const double* sup_d, const double* sub_d,
+                        int transpose, void* ws, size_t ws_bytes, int32_t* status, void* stream);
+int vk_bt_cr_solve_f64(vk_batch b, const double* sup_d, const double* sub_d, const double* rhs,
+                       double* x, void* ws, size_t ws_bytes, void* stream);
 /* Copy a tile-layout inverse stack into natural (nz, ni, ni) blocks. */
 int vk_bt_inverse_natural_f64(vk_batch b, const double* inv, double* nat, void* stream);
 

@@ -78,6 +78,8 @@ int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag
                         hipStream_t stream);
 bool vk_factor_fuses_sweep(int sl);
 bool vk_factor_is_cr(int sl);
+int vk_launch_nat_to_tile(int sl, const VkCol* cols, int ncols, int max_nz, const double* nat,
+                          double* tile, int transpose, hipStream_t stream);
 int vk_launch_cr_factor(int sl, const VkCol* cols, int ncols, int max_nz, const double* blk,
                         const double* sup_d, const double* sub_d, double* cr, int* status,
                         const int* active, hipStream_t stream);
@@ -720,6 +722,51 @@ int vk_bt_factor_f64(vk_batch b, const double* diag, const double* sup_d, const 
                                  nullptr, nullptr, (int*)status, nullptr, (hipStream_t)stream);
     if (rc) return rc;
   }
+  return VK_OK;
+}
+
+size_t vk_bt_cr_workspace_bytes(vk_batch b) {
+  return b ? (size_t)(8 * b->block_doubles) * sizeof(double) : 0;
+}
+
+int vk_bt_cr_factor_f64(vk_batch b, const double* diag, const double* sup_d, const double* sub_d,
+                        int transpose, void* ws, size_t ws_bytes, int32_t* status, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_plain(b, "vk_bt_cr_factor_f64")) return rc;
+  if (!diag || !sup_d || !sub_d || !ws || ws_bytes < vk_bt_cr_workspace_bytes(b)) {
+    vk_set_error("vk_bt_cr_factor_f64: null pointer or workspace %zu < %zu bytes", ws_bytes,
+                 vk_bt_cr_workspace_bytes(b));
+    return VK_ERR_ARG;
+  }
+  double* tile = (double*)ws;
+  double* cr = tile + b->block_doubles;
+  hipStream_t st = (hipStream_t)stream;
+  for (int sl : b->slots) {
+    if (int rc = vk_launch_nat_to_tile(sl, b->d_cols, b->ncols, b->max_nz, diag, tile,
+                                       transpose, st))
+      return rc;
+    if (int rc = vk_launch_cr_factor(sl, b->d_cols, b->ncols, b->max_nz, tile, sup_d, sub_d, cr,
+                                     (int*)status, nullptr, st))
+      return rc;
+  }
+  return VK_OK;
+}
+
+int vk_bt_cr_solve_f64(vk_batch b, const double* sup_d, const double* sub_d, const double* rhs,
+                       double* x, void* ws, size_t ws_bytes, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_plain(b, "vk_bt_cr_solve_f64")) return rc;
+  if (!sup_d || !sub_d || !rhs || !x || !ws || ws_bytes < vk_bt_cr_workspace_bytes(b)) {
+    vk_set_error("vk_bt_cr_solve_f64: null pointer or workspace %zu < %zu bytes", ws_bytes,
+                 vk_bt_cr_workspace_bytes(b));
+    return VK_ERR_ARG;
+  }
+  double* cr = (double*)ws + b->block_doubles;
+  SolveEpi epi{};
+  for (int sl : b->slots)
+    if (int rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rhs, sup_d, sub_d, cr, x,
+                                    epi, nullptr, (hipStream_t)stream))
+      return rc;
   return VK_OK;
 }
 

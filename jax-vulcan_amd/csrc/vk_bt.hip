@@ -1026,6 +1026,25 @@ __global__ __launch_bounds__(VK_NT) void k_cr_bwd(const VkCol* __restrict__ cols
   }
 }
 
+// Natural (nz, ni, ni) blocks, or their transposes, into the tile layout the
+// cyclic-reduction factor reads; zero in the padding.  Grid (nz, ncols).
+template <int SL>
+__global__ __launch_bounds__(VK_NT) void k_nat_to_tile(const VkCol* __restrict__ cols, int ncols,
+                                                       const double* __restrict__ nat,
+                                                       double* __restrict__ tile, int transpose) {
+  const int c = blockIdx.y;
+  const int j = blockIdx.x;
+  if (c >= ncols) return;
+  const VkCol col = cols[c];
+  if (vk_slots(col.ni) != SL || j >= col.nz) return;
+  const int ni = col.ni, tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  const double* src = nat + col.off_s * ni + (int64_t)j * ni * ni;
+  double* dst = tile + col.off_b + (int64_t)j * vk_blk<SL>();
+  for (int s = 0; s < SL; ++s)
+    for (int t = 0; t < SL; ++t)
+      dst[(s * SL + t) * VK_NT + tid] = nat_elem(src, ni, s * VK_G + tr, t * VK_G + tc, transpose);
+}
+
 // ---------------------------------------------------------------------------
 // host launchers (dispatch on the slot count of the batch bucket)
 // ---------------------------------------------------------------------------
@@ -1119,6 +1138,14 @@ int vk_launch_tile_to_nat(int sl, const VkCol* cols, int ncols, int max_nz, cons
                           double* nat, hipStream_t stream) {
   VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_tile_to_nat<SL_>, dim3(max_nz, ncols), dim3(VK_NT), 0,
                                         stream, cols, ncols, tile, nat));
+  VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}
+
+int vk_launch_nat_to_tile(int sl, const VkCol* cols, int ncols, int max_nz, const double* nat,
+                          double* tile, int transpose, hipStream_t stream) {
+  VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_nat_to_tile<SL_>, dim3(max_nz, ncols), dim3(VK_NT), 0,
+                                        stream, cols, ncols, nat, tile, transpose));
   VK_CHECK_HIP(hipGetLastError());
   return VK_OK;
 }

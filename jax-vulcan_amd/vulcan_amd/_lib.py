@@ -31,6 +31,9 @@ _SIGS = {
     "vk_block_doubles": (i64, [i32]),
     "vk_batch_block_doubles": (i64, [p]),
     "vk_batch_set_latency_mode": (i32, [p, i32]),
+    "vk_bt_cr_workspace_bytes": (sz, [p]),
+    "vk_bt_cr_factor_f64": (i32, [p, p, p, p, i32, p, sz, p, p]),
+    "vk_bt_cr_solve_f64": (i32, [p, p, p, p, p, p, sz, p]),
     "vk_atm_record_layout": (i64, [i32, i32, p]),
     "vk_chem_rhs_f64": (i32, [p, p, p, p, p, p, p]),
     "vk_chem_jac_f64": (i32, [p, p, p, p, p, p, p]),

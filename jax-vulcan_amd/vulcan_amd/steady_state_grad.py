@@ -170,10 +170,36 @@ def _build_jacobian_blocks(y, k_arr, atm: AtmStatic, net: NetworkArrays):
     return diag, sup[:-1].clone(), sub[:-1].clone()
 
 
+_LATENCY = {"on": False}
+
+
+def set_latency_mode(on: bool) -> None:
+    """Factor and solve the steady-state systems (Newton polish, adjoint) by
+    block cyclic reduction (vk_bt_cr_factor_f64): about log2(nz) block
+    inversions deep instead of nz, for one or a few columns.  Results agree
+    with the default block Thomas to the system's conditioning."""
+    _LATENCY["on"] = bool(on)
+
+
 def _factor_solve(diag, sup, sub, rhs, net, nz, transpose: bool):
     """Factor (optionally the transposed system) and solve one right-hand side."""
     b = uniform_batch(1, nz, net)
     lib = _lib.load()
+    if _LATENCY["on"]:
+        status = torch.zeros(1, dtype=torch.int32, device=diag.device)
+        a, c = (sub, sup) if transpose else (sup, sub)
+        nbytes = int(lib.vk_bt_cr_workspace_bytes(b.handle))
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=diag.device)
+        _lib.check(lib.vk_bt_cr_factor_f64(b.handle, diag.data_ptr(), a.data_ptr(), c.data_ptr(),
+                                           int(transpose), ws.data_ptr(), nbytes,
+                                           status.data_ptr(), _lib.stream_ptr()),
+                   "vk_bt_cr_factor_f64")
+        x = torch.empty_like(rhs)
+        _lib.check(lib.vk_bt_cr_solve_f64(b.handle, a.data_ptr(), c.data_ptr(),
+                                          rhs.contiguous().data_ptr(), x.data_ptr(),
+                                          ws.data_ptr(), nbytes, _lib.stream_ptr()),
+                   "vk_bt_cr_solve_f64")
+        return x, status
     inv = torch.empty(b.blk_len, dtype=F64, device=diag.device)
     status = torch.zeros(1, dtype=torch.int32, device=diag.device)
     a, c = (sub, sup) if transpose else (sup, sub)

@@ -250,3 +250,66 @@ def test_latency_mode_with_the_in_loop_refresh():
         a, t = a.cpu().numpy(), t.cpu().numpy()
         err = np.max(np.abs(a - t), axis=1) / np.max(np.abs(t), axis=1)
         assert np.all(np.isfinite(a)) and err.max() < 1e-10, (c, err.max())
+
+
+@pytest.mark.parametrize("transpose", [False, True])
+def test_standalone_cr_solver_matches_tile_and_dense(cfg1, transpose):
+    """vk_bt_cr_factor_f64 / vk_bt_cr_solve_f64 on a random diagonally
+    weighted block-tridiagonal system of workload 1's shape (nz 100, ni 50),
+    plain and transposed: against the tile solver to 1e-12 and a dense
+    solve to 1e-10."""
+    import torch
+    from vulcan_amd import steady_state_grad as G
+    from vulcan_amd.core import F64, device
+    c = cfg1
+    nz, ni = c.nz, c.ni
+    rng = np.random.default_rng(5)
+    diag = rng.normal(size=(nz, ni, ni)) + 4.0 * np.sqrt(ni) * np.eye(ni)[None]
+    sup = rng.normal(size=(nz, ni))
+    sub = rng.normal(size=(nz, ni))
+    rhs = rng.normal(size=(nz, ni))
+    dev = device()
+    t = lambda a: torch.tensor(a, dtype=F64, device=dev)
+    out = {}
+    try:
+        for mode in (False, True):
+            G.set_latency_mode(mode)
+            x, st = G._factor_solve(t(diag), t(sup), t(sub), t(rhs), c.net, nz, transpose)
+            torch.cuda.synchronize()
+            assert int(st.item()) == 0
+            out[mode] = _np(x)
+    finally:
+        G.set_latency_mode(False)
+    A = np.zeros((nz * ni, nz * ni))
+    for j in range(nz):
+        A[j * ni:(j + 1) * ni, j * ni:(j + 1) * ni] = diag[j]
+        if j >= 1:
+            A[j * ni:(j + 1) * ni, (j - 1) * ni:j * ni] = np.diag(sub[j - 1])
+            A[(j - 1) * ni:j * ni, j * ni:(j + 1) * ni] = np.diag(sup[j - 1])
+    xd = np.linalg.solve(A.T if transpose else A, rhs.reshape(-1)).reshape(nz, ni)
+    scale = np.abs(xd).max()
+    e_tile = np.abs(out[True] - out[False]).max() / scale
+    e_dense = np.abs(out[True] - xd).max() / scale
+    print(f"transpose={transpose}: cr vs tile {e_tile:.2e}, vs dense {e_dense:.2e}")
+    assert e_tile < 1e-12 and e_dense < 1e-10
+
+
+def test_steady_state_gradient_in_latency_mode():
+    """The whole gradient chain of the sensitivity test (Newton polish on the
+    device, transposed adjoint solve, parameter VJP) with the solves in
+    latency mode equals the block-Thomas one to 1e-9 of the largest
+    gradient."""
+    from test_gpu_sensitivity import Problem, _gpu_gradients
+    from vulcan_amd import steady_state_grad as G
+    pb = Problem(kzz=0.0)
+    base = _gpu_gradients(pb)
+    try:
+        G.set_latency_mode(True)
+        cr = _gpu_gradients(pb)
+    finally:
+        G.set_latency_mode(False)
+    for key in ("m", "T", "bf"):
+        a, b = cr[4][key], base[4][key]
+        scale = max(np.abs(b).max(), 1e-300)
+        assert np.abs(a - b).max() <= 1e-9 * scale, (key, np.abs(a - b).max() / scale)
+    assert np.abs(cr[0] - base[0]).max() <= 1e-12 * np.abs(base[0]).max()
