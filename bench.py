@@ -135,6 +135,10 @@ def main() -> int:
                     help="columns (nz=150, ni=93) for timing one attempt under the "
                          "relative-change rule and under the reference's longdy rule "
                          "(ring of 500); 0 skips it")
+    ap.add_argument("--solve-latency", action="store_true",
+                    help="time one transposed factor + solve of a random nz=150, ni=93 "
+                         "block system (the adjoint solve of one column), block Thomas "
+                         "against cyclic reduction")
     ap.add_argument("--no-rates", action="store_true",
                     help="skip the rate-construction measurement")
     ap.add_argument("--rates-only", action="store_true",
@@ -361,6 +365,29 @@ def main() -> int:
             del rr
         print(f"[bench] stopping-rule cost: {conv_cost}", file=sys.stderr, flush=True)
 
+    solve_latency = None
+    if args.solve_latency:
+        from vulcan_amd import steady_state_grad as G
+        rng = np.random.default_rng(1)
+        tt = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)
+        sdiag = tt(rng.normal(size=(NZ, NI, NI)) + 4.0 * np.sqrt(NI) * np.eye(NI)[None])
+        ssup, ssub, srhs = (tt(rng.normal(size=(NZ, NI))) for _ in range(3))
+        solve_latency = {"system": f"random block tridiagonal, nz={NZ}, ni={NI}, transposed"}
+        try:
+            for mode in (False, True):
+                G.set_latency_mode(mode)
+                G._factor_solve(sdiag, ssup, ssub, srhs, net, NZ, True)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(10):
+                    G._factor_solve(sdiag, ssup, ssub, srhs, net, NZ, True)
+                torch.cuda.synchronize()
+                key = "ms_cyclic_reduction" if mode else "ms_block_thomas"
+                solve_latency[key] = round((time.perf_counter() - t0) / 10 * 1e3, 3)
+        finally:
+            G.set_latency_mode(False)
+        print(f"[bench] solve latency: {solve_latency}", file=sys.stderr, flush=True)
+
     rate_build = None
     if not args.no_rates:
         rate_build = bench_rates(B, net, atm, no_cpu=args.no_cpu)
@@ -431,6 +458,7 @@ def main() -> int:
         "uneven_batch": uneven,
         "serving": serving,
         "stopping_rule_cost": conv_cost,
+        "solve_latency": solve_latency,
         "rate_build": rate_build,
         "roofline": roofline,
         "cpu_baseline": cpu,
