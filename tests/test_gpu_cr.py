@@ -12,7 +12,9 @@ another order, so the bar is agreement to the system's conditioning:
   own bar in test_gpu_parity.py);
 * an uneven batch (nz 2..61, four SL buckets, odd and even nz, powers of
   two and one above): each column equals the same column stepped alone bit
-  for bit, and the tile factor to 1e-12 at a small dt;
+  for bit, and the tile factor to 1e-12 at a small dt; the wide buckets
+  (ni 69, 93, 112: SL 5-7) likewise, and the standalone solver there
+  against a dense solve;
 * steady-state attempts replayed from a HIP graph equal the eager ones bit
   for bit (the factor and the two solves are about 50 launches per attempt);
 * a workspace sized for the tile factor is refused, not overrun;
@@ -313,3 +315,43 @@ def test_steady_state_gradient_in_latency_mode():
         scale = max(np.abs(b).max(), 1e-300)
         assert np.abs(a - b).max() <= 1e-9 * scale, (key, np.abs(a - b).max() / scale)
     assert np.abs(cr[0] - base[0]).max() <= 1e-12 * np.abs(base[0]).max()
+
+
+@pytest.mark.parametrize("ni", [69, 93, 112])
+def test_cr_wide_buckets_match_tile(ni):
+    """SL 5, 6 and 7 (the widest, whose elimination kernel spills): a batch
+    step on the cyclic-reduction factor equals the tile factor to 1e-12 at a
+    well-conditioned dt, and the standalone solver matches a dense solve."""
+    import torch
+    from vulcan_amd import steady_state_grad as G, synth
+    from vulcan_amd.core import F64, device
+    cols = [synth.make_column(1200 + ni + i, nz=nz, ni=ni, n_rev=3 * ni, dt=1e-16, p_bot=1e6,
+                              p_top=1e0) for i, nz in enumerate((7, 20))]
+    with factor(VK_FACTOR_CR):
+        cr, st = _batch_step(cols)
+    tile, _ = _batch_step(cols)
+    assert not st.any()
+    for c in range(len(cols)):
+        assert np.all(np.isfinite(cr[c]))
+        assert _layer_err(cr[c], tile[c]).max() < 1e-12, (ni, c)
+    nz = 20
+    rng = np.random.default_rng(ni)
+    diag = rng.normal(size=(nz, ni, ni)) + 4.0 * np.sqrt(ni) * np.eye(ni)[None]
+    sup, sub, rhs = (rng.normal(size=(nz, ni)) for _ in range(3))
+    dev = device()
+    t = lambda a: torch.tensor(a, dtype=F64, device=dev)
+    try:
+        G.set_latency_mode(True)
+        x, s = G._factor_solve(t(diag), t(sup), t(sub), t(rhs), cols[1].net, nz, False)
+        torch.cuda.synchronize()
+    finally:
+        G.set_latency_mode(False)
+    A = np.zeros((nz * ni, nz * ni))
+    for j in range(nz):
+        A[j * ni:(j + 1) * ni, j * ni:(j + 1) * ni] = diag[j]
+        if j >= 1:
+            A[j * ni:(j + 1) * ni, (j - 1) * ni:j * ni] = np.diag(sub[j - 1])
+            A[(j - 1) * ni:j * ni, j * ni:(j + 1) * ni] = np.diag(sup[j - 1])
+    xd = np.linalg.solve(A, rhs.reshape(-1)).reshape(nz, ni)
+    assert int(s.item()) == 0
+    assert np.abs(_np(x) - xd).max() < 1e-10 * np.abs(xd).max()
