@@ -82,10 +82,11 @@ int vk_launch_nat_to_tile(int sl, const VkCol* cols, int ncols, int max_nz, cons
                           double* tile, int transpose, hipStream_t stream);
 int vk_launch_cr_factor(int sl, const VkCol* cols, int ncols, int max_nz, const double* blk,
                         const double* sup_d, const double* sub_d, double* cr, int* status,
-                        const int* active, hipStream_t stream);
+                        const int* active, hipStream_t stream, const double* rhs_fused = nullptr);
 int vk_launch_cr_solve(int sl, const VkCol* cols, int ncols, int max_nz, const double* rhs,
                        const double* sup_d, const double* sub_d, double* cr, double* x,
-                       SolveEpi epi, const int* active, hipStream_t stream);
+                       SolveEpi epi, const int* active, hipStream_t stream,
+                       bool fwd_done = false);
 int vk_launch_bt_solve(int sl, const VkCol* cols, int ncols, const double* inv,
                        const double* sup_d, const double* sub_d, const double* rhs,
                        double* rscratch, double* x, int mode, SolveEpi epi, const int* active,
@@ -943,9 +944,9 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
     const bool fused = !cr && vk_factor_fuses_sweep(sl);
     {
       KernelTimer t(1, s);
-      if (cr)
+      if (cr)  // the stage-1 reduction of rhs_y rides in the update launches
         rc = vk_launch_cr_factor(sl, b->d_cols, b->ncols, b->max_nz, blk, sup, sub, crw,
-                                 (int*)status, active, s);
+                                 (int*)status, active, s, rhs_y);
       else
         rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, blk, 1, 0, sup, sub, blk,
                                  fused ? rhs_y : nullptr, fused ? rmod : nullptr, (int*)status,
@@ -957,7 +958,7 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
       KernelTimer t(2, s);
       if (cr)
         rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rhs_y, sup, sub, crw, k1,
-                                e1, active, s);
+                                e1, active, s, true);
       else
         rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, fused ? rmod : rhs_y,
                                 rmod, k1, fused ? 1 : 0, e1, active, s);

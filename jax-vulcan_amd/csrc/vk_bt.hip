@@ -743,8 +743,10 @@ __global__ __launch_bounds__(VK_NT) void k_cr_update(const VkCol* __restrict__ c
                                                       const double* __restrict__ sup_d,
                                                       const double* __restrict__ sub_d,
                                                       double* __restrict__ cr, int level,
+                                                      const double* __restrict__ rhs,
                                                       const int* __restrict__ active) {
   __shared__ double ga[(VK_G + 1) * VK_G * SL], gb[VK_G * VK_G * SL];
+  __shared__ double rv[3][VK_G * SL];
   const int c = blockIdx.y, w = blockIdx.x;
   if (c >= ncols) return;
   if (active && (active[c] & VK_ST_DONE)) return;
@@ -755,6 +757,9 @@ __global__ __launch_bounds__(VK_NT) void k_cr_update(const VkCol* __restrict__ c
   const int i = 2 * w;
   if (v.n <= 1 || i >= v.n) return;
   const CrLvl vn = cr_level(col.nz, BLK, N, 7 * col.off_b, level + 1);
+  // the stage-1 right-hand side's reduction, which needs only this level's
+  // P and Q (written by k_cr_elim before this launch)
+  if (rhs) cr_fwd_row<SL>(col, v, vn, level, w, rhs, cr, rv);
   const bool has_p = i + 1 < v.n, has_u = i + 2 < v.n;
   if (level >= 1) {
     // D' = (D_i - Q_{i-1} U_{i-1}) - P_{i+1} L_{i+1}; L' = -Q_{i-1} L_{i-1};
@@ -849,23 +854,17 @@ __global__ __launch_bounds__(VK_NT) void k_cr_top(const VkCol* __restrict__ cols
   store_tile<SL>(a, cr + v.Dinv);
 }
 
-// Reduce r: r'_i = (r_i - Q_{i-1} r_{i-1}) - P_{i+1} r_{i+1}.  Grid (ceil(n/2), ncols).
+namespace {
+
+// r_{l+1}[w] = (r_i - Q_{i-1} r_{i-1}) - P_{i+1} r_{i+1} for even row i = 2w
+// of level `level` (level 0 reads r from rhs).  Shared by k_cr_fwd and, for
+// the stage-1 right-hand side, by k_cr_update, so both form it alike.
 template <int SL>
-__global__ __launch_bounds__(VK_NT) void k_cr_fwd(const VkCol* __restrict__ cols, int ncols,
-                                                   const double* __restrict__ rhs,
-                                                   double* __restrict__ cr, int level,
-                                                   const int* __restrict__ active) {
+__device__ inline void cr_fwd_row(const VkCol& col, const CrLvl& v, const CrLvl& vn, int level,
+                                  int w, const double* __restrict__ rhs, double* __restrict__ cr,
+                                  double (&rv)[3][VK_G * SL]) {
   constexpr int BLK = vk_blk<SL>(), N = VK_G * SL;
-  __shared__ double rv[3][N];
-  const int c = blockIdx.y, w = blockIdx.x;
-  if (c >= ncols) return;
-  if (active && (active[c] & VK_ST_DONE)) return;
-  const VkCol col = cols[c];
-  if (vk_slots(col.ni) != SL) return;
-  const CrLvl v = cr_level(col.nz, BLK, N, 7 * col.off_b, level);
   const int i = 2 * w;
-  if (v.n <= 1 || i >= v.n) return;
-  const CrLvl vn = cr_level(col.nz, BLK, N, 7 * col.off_b, level + 1);
   const int ni = col.ni, tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
   for (int e = tid; e < 3 * N; e += VK_NT) {
     const int row = i - 1 + e / N, q = e % N;
@@ -897,6 +896,27 @@ __global__ __launch_bounds__(VK_NT) void k_cr_fwd(const VkCol* __restrict__ cols
       cr[vn.r + (int64_t)w * N + q] = q < ni ? out[s] : 0.0;
     }
   }
+}
+
+}  // namespace
+
+// Reduce r: r'_i = (r_i - Q_{i-1} r_{i-1}) - P_{i+1} r_{i+1}.  Grid (ceil(n/2), ncols).
+template <int SL>
+__global__ __launch_bounds__(VK_NT) void k_cr_fwd(const VkCol* __restrict__ cols, int ncols,
+                                                   const double* __restrict__ rhs,
+                                                   double* __restrict__ cr, int level,
+                                                   const int* __restrict__ active) {
+  constexpr int BLK = vk_blk<SL>(), N = VK_G * SL;
+  __shared__ double rv[3][N];
+  const int c = blockIdx.y, w = blockIdx.x;
+  if (c >= ncols) return;
+  if (active && (active[c] & VK_ST_DONE)) return;
+  const VkCol col = cols[c];
+  if (vk_slots(col.ni) != SL) return;
+  const CrLvl v = cr_level(col.nz, BLK, N, 7 * col.off_b, level);
+  if (v.n <= 1 || 2 * w >= v.n) return;
+  const CrLvl vn = cr_level(col.nz, BLK, N, 7 * col.off_b, level + 1);
+  cr_fwd_row<SL>(col, v, vn, level, w, rhs, cr, rv);
 }
 
 // x of the top row; for nz == 1 the whole solution.  Grid (ncols).
@@ -1155,10 +1175,12 @@ bool vk_factor_is_cr(int sl) {
 }
 
 // Cyclic-reduction factor of the tile-layout diagonal stack blk (level 0)
-// into the region cr (7 * block_doubles); one launch pair per level.
+// into the region cr (7 * block_doubles); one launch pair per level.  With
+// rhs_fused the update launches also reduce that right-hand side, and the
+// solve for it is called with fwd_done.
 int vk_launch_cr_factor(int sl, const VkCol* cols, int ncols, int max_nz, const double* blk,
                         const double* sup_d, const double* sub_d, double* cr, int* status,
-                        const int* active, hipStream_t stream) {
+                        const int* active, hipStream_t stream, const double* rhs_fused) {
   int level = 0;
   for (int n = max_nz; n > 1; n = (n + 1) / 2, ++level) {
     VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_cr_elim<SL_>, dim3(n / 2, ncols), dim3(VK_NT), 0,
@@ -1167,7 +1189,7 @@ int vk_launch_cr_factor(int sl, const VkCol* cols, int ncols, int max_nz, const 
     VK_CHECK_HIP(hipGetLastError());
     VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_cr_update<SL_>, dim3((n + 1) / 2, ncols),
                                           dim3(VK_NT), 0, stream, cols, ncols, blk, sup_d, sub_d,
-                                          cr, level, active));
+                                          cr, level, rhs_fused, active));
     VK_CHECK_HIP(hipGetLastError());
   }
   VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_cr_top<SL_>, dim3(ncols), dim3(VK_NT), 0, stream, cols,
@@ -1179,11 +1201,11 @@ int vk_launch_cr_factor(int sl, const VkCol* cols, int ncols, int max_nz, const 
 // Solve with the cyclic-reduction factors: x and the epilogue from rhs.
 int vk_launch_cr_solve(int sl, const VkCol* cols, int ncols, int max_nz, const double* rhs,
                        const double* sup_d, const double* sub_d, double* cr, double* x,
-                       SolveEpi epi, const int* active, hipStream_t stream) {
+                       SolveEpi epi, const int* active, hipStream_t stream, bool fwd_done) {
   int ns[40];
   int levels = 0;
   for (int n = max_nz; n > 1 && levels < 40; n = (n + 1) / 2) ns[levels++] = n;
-  for (int l = 0; l < levels; ++l) {
+  for (int l = 0; l < levels && !fwd_done; ++l) {
     VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_cr_fwd<SL_>, dim3((ns[l] + 1) / 2, ncols),
                                           dim3(VK_NT), 0, stream, cols, ncols, rhs, cr, l,
                                           active));
