@@ -209,6 +209,17 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
  * the molecular-diffusion factors that do not depend on y once per step;
  * 0 forms them inside each chemistry launch.  Results are bitwise equal. */
 #define VK_TUNE_MOLPRE 1
+/* Knob VK_TUNE_REFINE (sl ignored): one step of iterative refinement of the
+ * Ros2 step's block solves.  A solve with explicit block inverses is not
+ * backward stable; the refined one is, as the reference's lu_solve is
+ * (solver.py:91-112).  Tile-layout factors refine each block solve of the
+ * backward sweep, x_j = X_j t_j; x_j += X_j (t_j - A'_j x_j) with A'_j the
+ * Schur complement X_j inverts (one extra read of the diagonal-block stack
+ * per refined sweep); the cyclic-reduction factor takes one whole-column
+ * step (residual, correction solve).  0: none; 1: the stage-1 solve;
+ * 2 (default): both stages.  Unrefined, a step leaves up to ~1e-9 relative
+ * error on cells that the reference's LU determines to 1e-12. */
+#define VK_TUNE_REFINE 2
 int vk_tune_set(int knob, int sl, int value);
 int vk_tune_get(int knob, int sl); /* -1 for an unknown knob / bucket */
 

@@ -90,9 +90,12 @@ int vk_launch_cr_solve(int sl, const VkCol* cols, int ncols, int max_nz, const d
 int vk_launch_bt_solve(int sl, const VkCol* cols, int ncols, const double* inv,
                        const double* sup_d, const double* sub_d, const double* rhs,
                        double* rscratch, double* x, int mode, SolveEpi epi, const int* active,
-                       hipStream_t stream);
+                       hipStream_t stream, const double* refine_diag = nullptr);
 int vk_launch_tile_to_nat(int sl, const VkCol* cols, int ncols, int max_nz, const double* tile,
                           double* nat, hipStream_t stream);
+int vk_launch_bt_residual(int sl, const VkCol* cols, int ncols, int max_nz, const double* blk,
+                          const double* sup_d, const double* sub_d, const double* rhs,
+                          const double* x, double* res, const int* active, hipStream_t stream);
 int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max_nz, int max_S,
                          int max_T, const VkNetDev* nets, const double* y, const double* M,
                          const double* atm, const double* k, const double* dt, const uint8_t* fix,
@@ -837,10 +840,22 @@ int vk_timing_read(double* ms, long long* count, int n) {
 // launch.  The coefficients are the same bit for bit either way.
 static int g_mol_precompute = 1;
 
-// Bytes before the cyclic-reduction region of the step workspace.
-static size_t ros2_ws_base_bytes(vk_batch b) {
+// VK_TUNE_REFINE: 0 none; 1 the stage-1 solve is refined; 2 (default) both
+// stages' solves.  Tile-layout factors refine each block solve of the
+// backward sweep; the cyclic-reduction factor takes one whole-column
+// refinement step (residual, correction solve).
+static int g_refine = 2;
+
+// Offset of the inverse stack in the step workspace: the factor writes the
+// inverses there, so the diagonal blocks D survive for the refinement.
+static size_t ros2_ws_inv_offset(vk_batch b) {
   return align_up<double>(b->block_doubles) + 16 * align_up<double>(b->state_doubles) +
          align_up<double>(b->k_doubles);
+}
+
+// Bytes before the cyclic-reduction region of the step workspace.
+static size_t ros2_ws_base_bytes(vk_batch b) {
+  return ros2_ws_inv_offset(b) + align_up<double>(b->block_doubles);
 }
 
 // Whether bucket sl of b uses the cyclic-reduction factor.
@@ -913,6 +928,9 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
   double* mst = (double*)w;  // 8 components of the molecular-diffusion static factors
   const int64_t mstride = (int64_t)(sv / sizeof(double));
   double* crw = (double*)((char*)ws + ros2_ws_base_bytes(b));  // cyclic-reduction levels
+  double* inv = (double*)((char*)ws + ros2_ws_inv_offset(b));  // block inverses X_j
+  const double* ref1 = g_refine >= 1 ? blk : nullptr;          // D for the refinement
+  const double* ref2 = g_refine >= 2 ? blk : nullptr;
   const double r = ros2_gamma();
   {
     KernelTimer t(6, st);
@@ -948,7 +966,7 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
         rc = vk_launch_cr_factor(sl, b->d_cols, b->ncols, b->max_nz, blk, sup, sub, crw,
                                  (int*)status, active, s, rhs_y);
       else
-        rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, blk, 1, 0, sup, sub, blk,
+        rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, blk, 1, 0, sup, sub, inv,
                                  fused ? rhs_y : nullptr, fused ? rmod : nullptr, (int*)status,
                                  active, s);
       if (rc) return rc;
@@ -956,12 +974,26 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
     SolveEpi e1{1, r, 0.0, y, nullptr, nullptr, yk2, nullptr};
     {
       KernelTimer t(2, s);
-      if (cr)
+      if (cr && ref1) {
+        // x0 into k2 (free until stage 2), its residual into rmod, then the
+        // correction solve writes k1 = x0 + dx and yk2
+        SolveEpi e0{};
+        if ((rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rhs_y, sup, sub, crw,
+                                     k2, e0, active, s, true)))
+          return rc;
+        if ((rc = vk_launch_bt_residual(sl, b->d_cols, b->ncols, b->max_nz, blk, sup, sub, rhs_y,
+                                        k2, rmod, active, s)))
+          return rc;
+        SolveEpi e1r = e1;
+        e1r.base = k2;
+        rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rmod, sup, sub, crw, k1, e1r,
+                                active, s);
+      } else if (cr)
         rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rhs_y, sup, sub, crw, k1,
                                 e1, active, s, true);
       else
-        rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, fused ? rmod : rhs_y,
-                                rmod, k1, fused ? 1 : 0, e1, active, s);
+        rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, inv, sup, sub, fused ? rmod : rhs_y,
+                                rmod, k1, fused ? 1 : 0, e1, active, s, ref1);
       if (rc) return rc;
     }
     {
@@ -974,12 +1006,26 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
     SolveEpi e2{2, 3.0 / (2.0 * r), 1.0 / (2.0 * r), y, k1, yk2, sol, delta};
     {
       KernelTimer t(4, s);
-      if (cr)
+      if (cr && ref2) {
+        // x0 into rmod, its residual into rhs_y (free after stage 1), then
+        // the correction solve writes k2 = x0 + dx, sol and delta
+        SolveEpi e0{};
+        if ((rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rhs2, sup, sub, crw,
+                                     rmod, e0, active, s)))
+          return rc;
+        if ((rc = vk_launch_bt_residual(sl, b->d_cols, b->ncols, b->max_nz, blk, sup, sub, rhs2,
+                                        rmod, rhs_y, active, s)))
+          return rc;
+        SolveEpi e2r = e2;
+        e2r.base = rmod;
+        rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rhs_y, sup, sub, crw, k2, e2r,
+                                active, s);
+      } else if (cr)
         rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rhs2, sup, sub, crw, k2, e2,
                                 active, s);
       else
-        rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, blk, sup, sub, rhs2, rmod, k2, 0, e2,
-                                active, s);
+        rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, inv, sup, sub, rhs2, rmod, k2, 0, e2,
+                                active, s, ref2);
       if (rc) return rc;
     }
     return VK_OK;
@@ -1131,6 +1177,14 @@ int vk_step_error_f64(vk_batch b, const double* sol, const double* delta, double
 
 int vk_tune_set(int knob, int sl, int value) {
   g_err[0] = 0;
+  if (knob == VK_TUNE_REFINE) {
+    if (value < 0 || value > 2) {
+      vk_set_error("vk_tune_set: VK_TUNE_REFINE takes 0, 1 or 2, not %d", value);
+      return VK_ERR_ARG;
+    }
+    g_refine = value;
+    return VK_OK;
+  }
   if (knob == VK_TUNE_MOLPRE) {
     if (value != 0 && value != 1) {
       vk_set_error("vk_tune_set: VK_TUNE_MOLPRE takes 0 or 1, not %d", value);
@@ -1154,6 +1208,7 @@ int vk_tune_set(int knob, int sl, int value) {
 }
 int vk_tune_get(int knob, int sl) {
   if (knob == VK_TUNE_MOLPRE) return g_mol_precompute;
+  if (knob == VK_TUNE_REFINE) return g_refine;
   if (knob != VK_TUNE_FACTOR || sl < 1 || sl > 7) return -1;
   return g_factor_variant[sl];
 }

@@ -233,6 +233,24 @@ __device__ inline void tile_matvec(const double (&a)[SL][SL], const double* v, d
   }
 }
 
+// y_i = sum_q B[i][q] v[q] for this thread's rows, B a tile-layout block
+// read straight from memory (not kept in registers), v in shared memory.
+template <int SL>
+__device__ inline void stream_matvec(const double* __restrict__ src, const double* v,
+                                     double (&y)[SL]) {
+  const int tid = threadIdx.x, tc = tid & 15;
+  double vv[SL];
+#pragma unroll
+  for (int t = 0; t < SL; ++t) vv[t] = v[t * VK_G + tc];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) {
+    double acc = 0.0;
+#pragma unroll
+    for (int t = 0; t < SL; ++t) acc = fma(src[(s * SL + t) * VK_NT + tid], vv[t], acc);
+    y[s] = vk_rowsum16(acc);
+  }
+}
+
 template <int SL>
 __device__ inline void load_tile(double (&a)[SL][SL], const double* src) {
   const int tid = threadIdx.x;
@@ -365,14 +383,26 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(TILE_IN &
 // element of x_j (all pointers at off_s):
 //   epi 1: out1 = y + x / e0                          (Ros2 stage-1 yk2)
 //   epi 2: out1 = y + e0*k1 + e1*x, out2 = |out1 - yk2|  (Ros2 sol, delta)
+//
+// With dstk (the tile-layout diagonal blocks D_j the inverses were formed
+// from) each block solve of the backward sweep takes one step of iterative
+// refinement:
+//   x_j = X_j t_j;  x_j += X_j (t_j - A'_j x_j),
+//   A'_j x = D_j x - c_{j-1} o (X_{j-1} (b_{j-1} o x))   (b = sup, c = sub),
+// A'_j being the Schur complement whose inverse X_j is.  A product with an
+// explicit inverse is not backward stable; the refined block solve is, as
+// the reference's lu_solve on A'_j is (solver.py:91-112).  D_j and X_{j-1}
+// stream from memory, so the sweep keeps one register tile.
 
 template <int SL>
 __global__ __launch_bounds__(VK_NT) void k_bt_solve(
     const VkCol* __restrict__ cols, int ncols, const double* __restrict__ inv,
     const double* __restrict__ sup_d, const double* __restrict__ sub_d,
     const double* __restrict__ rhs, double* __restrict__ rscratch, double* __restrict__ x,
-    int mode, SolveEpi epi, const int* __restrict__ active) {
+    int mode, SolveEpi epi, const int* __restrict__ active, const double* __restrict__ dstk) {
   __shared__ double vec[2][VK_G * SL];
+  __shared__ double rv[VK_G * SL];  // refinement: x_j, then the residual
+  __shared__ double bx[VK_G * SL];  // refinement: b_{j-1} o x_j
   const int c = blockIdx.x;
   if (c >= ncols) return;
   if (active && (active[c] & VK_ST_DONE)) return;
@@ -418,9 +448,43 @@ __global__ __launch_bounds__(VK_NT) void k_bt_solve(
   for (int i = tid; i < VK_G * SL; i += VK_NT)
     vec[(nz - 1) & 1][i] = (i < ni) ? rm[col.off_s + (int64_t)(nz - 1) * ni + i] : 0.0;
   __syncthreads();
+  const double* dblk = dstk ? dstk + col.off_b : nullptr;
   for (int j = nz - 1; j >= 0; --j) {
     load_tile<SL>(a, stk + (int64_t)j * BLK);
     tile_matvec<SL>(a, vec[j & 1], w);
+    if (dblk) {
+      if (tc == 0) {
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          const int i = s * VK_G + tr;
+          rv[i] = (i < ni) ? w[s] : 0.0;
+          bx[i] = (j > 0 && i < ni) ? bsup[(int64_t)(j - 1) * ni + i] * w[s] : 0.0;
+        }
+      }
+      __syncthreads();
+      double q[SL], pz[SL];
+      stream_matvec<SL>(dblk + (int64_t)j * BLK, rv, q);
+      if (j > 0) stream_matvec<SL>(stk + (int64_t)(j - 1) * BLK, bx, pz);
+      __syncthreads();
+      if (tc == 0) {
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          const int i = s * VK_G + tr;
+          double r = 0.0;
+          if (i < ni) {
+            double ax = q[s];
+            if (j > 0) ax = ax - bsub[(int64_t)(j - 1) * ni + i] * pz[s];
+            r = vec[j & 1][i] - ax;
+          }
+          rv[i] = r;
+        }
+      }
+      __syncthreads();
+      double d[SL];
+      tile_matvec<SL>(a, rv, d);
+#pragma unroll
+      for (int s = 0; s < SL; ++s) w[s] = w[s] + d[s];
+    }
     if (tc == 0) {
 #pragma unroll
       for (int s = 0; s < SL; ++s) {
@@ -447,6 +511,42 @@ __global__ __launch_bounds__(VK_NT) void k_bt_solve(
       }
     }
     __syncthreads();
+  }
+}
+
+// Residual of the block-tridiagonal system at x, one workgroup per (layer,
+// column):  res_j = ((r_j - D_j x_j) - b_j o x_{j+1}) - c_{j-1} o x_{j-1},
+// D_j the tile-layout diagonal blocks, b = sup, c = sub.  The refinement
+// step of the cyclic-reduction solves.
+template <int SL>
+__global__ __launch_bounds__(VK_NT) void k_bt_residual(
+    const VkCol* __restrict__ cols, int ncols, const double* __restrict__ blk,
+    const double* __restrict__ sup_d, const double* __restrict__ sub_d,
+    const double* __restrict__ rhs, const double* __restrict__ x, double* __restrict__ res,
+    const int* __restrict__ active) {
+  __shared__ double xv[VK_G * SL];
+  const int c = blockIdx.y, j = blockIdx.x;
+  if (c >= ncols) return;
+  if (active && (active[c] & VK_ST_DONE)) return;
+  const VkCol col = cols[c];
+  if (vk_slots(col.ni) != SL || j >= col.nz) return;
+  const int nz = col.nz, ni = col.ni, tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  const int64_t row = col.off_s + (int64_t)j * ni;
+  for (int i = tid; i < VK_G * SL; i += VK_NT) xv[i] = (i < ni) ? x[row + i] : 0.0;
+  __syncthreads();
+  double q[SL];
+  stream_matvec<SL>(blk + col.off_b + (int64_t)j * vk_blk<SL>(), xv, q);
+  if (tc == 0) {
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int i = s * VK_G + tr;
+      if (i < ni) {
+        double r = rhs[row + i] - q[s];
+        if (j + 1 < nz) r = r - sup_d[row + i] * x[row + ni + i];
+        if (j > 0) r = r - sub_d[row - ni + i] * x[row - ni + i];
+        res[row + i] = r;
+      }
+    }
   }
 }
 
@@ -656,6 +756,7 @@ __device__ inline void cr_mm(const double* __restrict__ A, const double* __restr
 }
 
 __device__ inline void cr_epilogue(const SolveEpi& epi, int64_t e, double xv, double* x) {
+  if (epi.base) xv = epi.base[e] + xv;
   x[e] = xv;
   if (epi.kind == 1) {
     epi.out1[e] = epi.y[e] + xv / epi.e0;
@@ -1146,10 +1247,20 @@ int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag
 int vk_launch_bt_solve(int sl, const VkCol* cols, int ncols, const double* inv,
                        const double* sup_d, const double* sub_d, const double* rhs,
                        double* rscratch, double* x, int mode, SolveEpi epi, const int* active,
-                       hipStream_t stream) {
+                       hipStream_t stream, const double* refine_diag) {
   VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_bt_solve<SL_>, dim3(ncols), dim3(VK_NT), 0, stream,
                                         cols, ncols, inv, sup_d, sub_d, rhs, rscratch, x, mode,
-                                        epi, active));
+                                        epi, active, refine_diag));
+  VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}
+
+int vk_launch_bt_residual(int sl, const VkCol* cols, int ncols, int max_nz, const double* blk,
+                          const double* sup_d, const double* sub_d, const double* rhs,
+                          const double* x, double* res, const int* active, hipStream_t stream) {
+  VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_bt_residual<SL_>, dim3(max_nz, ncols), dim3(VK_NT), 0,
+                                        stream, cols, ncols, blk, sup_d, sub_d, rhs, x, res,
+                                        active));
   VK_CHECK_HIP(hipGetLastError());
   return VK_OK;
 }

@@ -204,6 +204,9 @@ struct SolveEpi {
   const double* yk2;
   double* out1;
   double* out2;
+  // when set (cyclic-reduction solves only): the solution is base + the
+  // solve's result, i.e. the solve computed a refinement correction
+  const double* base = nullptr;
 };
 
 // Thread-local error text set by the host API.

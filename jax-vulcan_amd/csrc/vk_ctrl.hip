@@ -36,6 +36,13 @@ __device__ inline T block_reduce(T v, T* red, Op op) {
 
 struct AddD { __device__ double operator()(double a, double b) const { return a + b; } };
 struct MaxD { __device__ double operator()(double a, double b) const { return a > b ? a : b; } };
+// jnp.max semantics: a NaN operand makes the maximum NaN
+__device__ inline double nan_max(double a, double b) { return (a > b || a != a) ? a : b; }
+struct NanMaxD { __device__ double operator()(double a, double b) const { return nan_max(a, b); } };
+// jnp.clip semantics (maximum then minimum, both NaN-propagating)
+__device__ inline double nan_clip(double x, double lo, double hi) {
+  return x != x ? x : fmin(fmax(x, lo), hi);
+}
 struct OrI { __device__ int operator()(int a, int b) const { return a | b; } };
 
 }  // namespace
@@ -63,7 +70,10 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
 
   // ---- pass 1: clip, ymix_new, delta, sign flags, atom sums ----------------
   double dmax = 0.0;
-  int flags = 0;  // 1: a negative value after the clip, 2: a non-finite value
+  // 1: a value after the clip that is not >= 0 (NaN included: all(sol_clip >= 0)
+  //    fails on NaN, outer_loop.py:747), 2: a non-finite value,
+  // 4: a value after the clip that is < 0 (any(sol_clip < 0), :757)
+  int flags = 0;
   double at_clip[VK_MAX_ATOMS];
   for (int a = 0; a < VK_MAX_ATOMS; ++a) at_clip[a] = 0.0;
   for (int z = tid; z < nz; z += NT) {
@@ -76,20 +86,18 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
     for (int i = 0; i < ni; ++i) {
       const int64_t e = (int64_t)z * ni + i;
       const double sc = clip_val(sol[e], ymix[e], p);
-      if (sc < 0.0) flags |= 1;
+      if (!(sc >= 0.0)) flags |= 1;
+      if (sc < 0.0) flags |= 4;
       if (!isfinite(sol[e])) flags |= 2;
       for (int a = 0; a < na; ++a) at_clip[a] += compo[i * na + a] * sc;
       // aggregated delta (outer_loop.py:238-248) with the pre-step ymix
       double m = (ymix[e] < p.mtol) ? 0.0 : dar[e];
       if (sc < p.atol) m = 0.0;
       if (p.zero_bot_row && z == 0) m = 0.0;
-      if (sc > 0.0) {
-        const double r = m / fmax(fabs(sc), 1e-300);
-        dmax = r > dmax ? r : dmax;
-      }
+      if (sc > 0.0) dmax = nan_max(dmax, m / fmax(fabs(sc), 1e-300));
     }
   }
-  dmax = block_reduce(dmax, redd, MaxD());
+  dmax = block_reduce(dmax, redd, NanMaxD());
   flags = block_reduce(flags, redi, OrI());
   for (int a = 0; a < na; ++a) at_clip[a] = block_reduce(at_clip[a], redd, AddD());
 
@@ -100,8 +108,7 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
   double loss_diff = 0.0;
   for (int a = 0; a < na; ++a) {
     const double aln = (at_clip[a] - ai[a]) / ai[a];
-    const double d = fabs(aln - alp[a]);
-    loss_diff = d > loss_diff ? d : loss_diff;
+    loss_diff = nan_max(loss_diff, fabs(aln - alp[a]));
   }
   const double dt = s.dt[c];
   const int retry = s.retry[c];
@@ -286,8 +293,8 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
     const bool delta_too_big = dmax > p.rtol;
     const bool reject = !do_accept;
     if (reject && delta_too_big) s.dcount[c] += 1;
-    if (reject && !delta_too_big && (flags & 1)) s.nega[c] += 1;
-    if (reject && !delta_too_big && !(flags & 1)) s.loss[c] += 1;
+    if (reject && !delta_too_big && (flags & 4)) s.nega[c] += 1;
+    if (reject && !delta_too_big && !(flags & 4)) s.loss[c] += 1;
     const double t_old = s.t[c];
     const double t_new = do_accept ? t_old + (force ? p.dt_min : dt) : t_old;
     s.t[c] = t_new;
@@ -301,9 +308,8 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
     double dt_new;
     if (do_accept) {
       const double de = (dmax == 0.0) ? 0.01 * p.rtol : dmax;
-      double h = 0.9 * sqrt(p.rtol / de);
-      h = fmin(fmax(h, p.dt_var_min), p.dt_var_max);
-      dt_new = fmin(fmax(base * h, p.dt_min), p.dt_max);
+      const double h = nan_clip(0.9 * sqrt(p.rtol / de), p.dt_var_min, p.dt_var_max);
+      dt_new = nan_clip(base * h, p.dt_min, p.dt_max);
     } else {
       dt_new = next_dt_rej;
     }

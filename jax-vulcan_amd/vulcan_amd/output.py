@@ -23,7 +23,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-__all__ = ["vul_dicts", "save_vul", "TABLEAU20"]
+__all__ = ["vul_dicts", "save_vul", "end_case", "TABLEAU20"]
 
 # the normalised Tableau 20 palette published in the parameter dictionary
 TABLEAU20 = [(r / 255.0, g / 255.0, b / 255.0) for r, g, b in (
@@ -31,6 +31,26 @@ TABLEAU20 = [(r / 255.0, g / 255.0, b / 255.0) for r, g, b in (
     (140, 86, 75), (227, 119, 194), (127, 127, 127), (188, 189, 34), (23, 190, 207),
     (174, 199, 232), (255, 187, 120), (152, 223, 138), (255, 152, 150), (197, 176, 213),
     (196, 156, 148), (247, 182, 210), (199, 199, 199), (219, 219, 141), (158, 218, 229))]
+
+
+def end_case(accept_count: int, t: float, status: int, cfg) -> int:
+    """The reference's end_case code of a finished run: 3 when the step cap
+    was exceeded, 2 when the runtime was, else 1 (outer_loop.py:1938-1942,
+    2326-2339).  Under conv_mode "longdy" the cap is the reference's own
+    (accept_count > count_max); under "relmax" the runner stops at
+    count_max accepted steps, so a column that stopped there without
+    converging counts as capped."""
+    from .outer_loop import ST_CONVERGED, ST_MAXSTEPS
+    if cfg.conv_mode == "longdy":
+        capped = int(accept_count) > int(cfg.count_max)
+    else:
+        capped = (int(accept_count) >= int(cfg.count_max) and bool(int(status) & ST_MAXSTEPS)
+                  and not int(status) & ST_CONVERGED)
+    if capped:
+        return 3
+    if float(t) > float(cfg.runtime):
+        return 2
+    return 1
 
 
 def vul_dicts(runner, c: int, species: Sequence[str], atom_order: Sequence[str] = ("X",),
@@ -88,7 +108,8 @@ def vul_dicts(runner, c: int, species: Sequence[str], atom_order: Sequence[str] 
     para = {"count": int(res["accept"][c]), "nega_count": int(counts["nega"][c]),
             "loss_count": int(counts["loss"][c]), "delta_count": int(counts["delta"][c]),
             "delta": float(res["delta"][c]), "small_y": 0.0, "nega_y": 0.0,
-            "end_case": int(res["status"][c]), "solver_str": "solver",
+            "end_case": end_case(res["accept"][c], res["t"][c], res["status"][c], runner.cfg),
+            "device_status": int(res["status"][c]), "solver_str": "solver",
             "switch_final_photo_frq": False, "where_varies_most": np.zeros_like(y),
             "pic_count": 0, "fix_species_start": False, "tableau20": list(TABLEAU20),
             "start_time": float(start_time)}

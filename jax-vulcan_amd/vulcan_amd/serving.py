@@ -138,7 +138,8 @@ class SlotPool:
     def bytes_per_slot(cap_nz: int, cap_ni: int, cap_S: int, latency_mode: bool = True) -> int:
         cs, ck = cap_nz * cap_ni, cap_S * cap_nz
         ca = atm_record_layout(cap_nz, cap_ni)[0]
-        ws = cap_nz * block_doubles(cap_ni) + 16 * cs + ck  # Ros2 workspace per slot
+        # Ros2 workspace per slot: the diagonal blocks, their inverses, vectors
+        ws = 2 * cap_nz * block_doubles(cap_ni) + 16 * cs + ck
         sl = (cap_ni + 15) // 16
         if latency_mode or _lib.load().vk_tune_get(0, sl) == FACTOR_CR:
             ws += 7 * cap_nz * block_doubles(cap_ni)      # the cyclic-reduction levels

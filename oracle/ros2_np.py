@@ -328,6 +328,44 @@ def block_thomas_dense(diag, sup, sub, rhs):
     return np.stack(x[::-1])
 
 
+def banded_solve(diag, sup, sub, rhs):
+    """The block-tridiagonal system (diagonal off-diagonal blocks) solved as
+    one banded matrix by LAPACK gbsv: partial pivoting over the whole column
+    rather than within a block.  A third elimination order next to the
+    per-block LU of bt_factor and the C oracle, used to tell which cells of
+    an ill-conditioned step FP64 determines (not a restatement of a
+    reference function)."""
+    nz, ni, _ = diag.shape
+    N, bw = nz * ni, 2 * ni - 1
+    ab = np.zeros((2 * bw + 1, N))          # A[i, j] at ab[bw + i - j, j]
+    cols = np.arange(ni)
+    for z in range(nz):
+        r0 = z * ni
+        for a in range(ni):
+            ab[bw + a - cols, r0 + cols] = diag[z, a]
+        if z + 1 < nz:
+            i = r0 + cols
+            ab[bw - ni, i + ni] = sup[z]       # A[i, i + ni]
+            ab[bw + ni, i] = sub[z]            # A[i + ni, i]
+    return sla.solve_banded((bw, bw), ab, rhs.reshape(-1), check_finite=False).reshape(nz, ni)
+
+
+def chem_term_scale(y, M, k, net):
+    """Per (layer, species): the sum of |terms| of chem_rhs (gross
+    production + gross loss), the natural scale of its cancellation error."""
+    nz, ni = y.shape
+    rate, _, _ = _rates(y, M, k, net)
+    S, T = net.reactant_idx.shape
+    out = np.zeros((nz, ni))
+    for z in range(nz):
+        rr = np.repeat(np.abs(rate[z]), T)
+        acc = np.zeros(ni + 1)
+        np.add.at(acc, net.reactant_idx.reshape(-1), net.reactant_stoich.reshape(-1) * rr)
+        np.add.at(acc, net.product_idx.reshape(-1), net.product_stoich.reshape(-1) * rr)
+        out[z] = acc[:ni]
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Ros2 step (jax_step.py:249-315) and fixed-dt driver (integrate.py:20-36)
 # ---------------------------------------------------------------------------
@@ -378,6 +416,21 @@ def ros2_step(y, k, dt, atm, net, fix_mask=None, return_stages=False):
     if return_stages:
         return sol, delta, dict(k1=k1, k2=k2, yk2=yk2, rhs_y=rhs_y, rhs2=rhs2)
     return sol, delta
+
+
+def ros2_step_banded(y, k, dt, atm, net):
+    """ros2_step with both stage solves done by banded_solve (global partial
+    pivoting) instead of the block Thomas factor; same formulas otherwise
+    (jax_step.py:257-314)."""
+    r = GAMMA
+    grav = compute_diff_grav(atm)
+    diag, sup_neg, sub_neg, rhs_y = ros2_lhs(y, k, dt, atm, net, grav)
+    k1 = banded_solve(diag, sup_neg, sub_neg, rhs_y)
+    yk2 = y + k1 / r
+    rhs2 = total_rhs(yk2, k, atm, net, grav) - (2.0 / (r * dt)) * k1
+    k2 = banded_solve(diag, sup_neg, sub_neg, rhs2)
+    sol = y + (3.0 / (2.0 * r)) * k1 + (1.0 / (2.0 * r)) * k2
+    return sol, np.abs(sol - yk2)
 
 
 def integrate_fixed_dt(y0, k, dt, n_steps, atm, net):

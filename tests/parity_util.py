@@ -1,0 +1,70 @@
+"""Elementwise parity of a Ros2 step against three CPU reference solves.
+
+A step of the synthetic workloads moves trace species by up to 1e30 of
+their value and its blocks reach cond ~ 1e15, so FP64 does not determine
+every cell of the result: two correct LU codes already differ by up to
+1e-4 on the stiffest cells.  Which cells it does determine is read off
+three reference solves that eliminate in different orders:
+
+* ``ros2_np.ros2_step``: per-block LAPACK LU (getrf/getrs), the reference's
+  own formulation (solver.py:33-112);
+* ``ros2_c``: the C restatement, per-block LU with its own loop order;
+* ``ros2_np.ros2_step_banded``: the whole column as one banded matrix,
+  LAPACK gbsv with partial pivoting across blocks.
+
+Cells are compared elementwise, |a - ref| / |ref|, over the cells the
+reference's step control looks at (y >= atol = 0.1 and ymix >= mtol =
+1e-22, outer_loop.py:227-250; vulcan_cfg.py:134-141).  A cell is
+"determined" when the three references agree on it to 1e-12.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+ATOL, MTOL = 0.1, 1e-22
+
+
+def elem_err(a, ref, mask):
+    """|a - ref| / |ref| on the masked cells, 0 elsewhere."""
+    return np.where(mask, np.abs(a - ref) / np.maximum(np.abs(ref), 1e-300), 0.0)
+
+
+def layer_err(a, ref):
+    """max_i |a - ref| / max_i |ref| per layer."""
+    return np.max(np.abs(a - ref), axis=1) / np.max(np.abs(ref), axis=1)
+
+
+class StepRefs:
+    """The three reference steps of one column and the cell classes."""
+
+    def __init__(self, y, k, dt, atm, net):
+        from oracle import ros2_c, ros2_np as O
+        from vulcan_amd import core
+        self.np, self.delta = O.ros2_step(y, k, dt, atm, net)
+        rec = core.pack_atm_record(atm).numpy()[None]
+        sc, _ = ros2_c.ros2_steps(y[None], k[None], rec, dt, net)
+        self.c = sc[0]
+        self.band, _ = O.ros2_step_banded(y, k, dt, atm, net)
+        ymix = y / y.sum(axis=1, keepdims=True)
+        self.mask = (np.abs(self.np) >= ATOL) & (ymix >= MTOL)
+        e_c = elem_err(self.c, self.np, self.mask)
+        e_b = elem_err(self.band, self.np, self.mask)
+        e_cb = elem_err(self.c, self.band, self.mask)
+        self.spread = np.maximum(np.maximum(e_c, e_b), e_cb)
+        self.determined = self.mask & (self.spread < 1e-12)
+        self.layer_spread = np.maximum(layer_err(self.c, self.np), layer_err(self.band, self.np))
+
+    def judge(self, sol, name=""):
+        """(worst error on determined cells, worst error on any masked cell,
+        the references' worst spread, report line)."""
+        e = elem_err(sol, self.np, self.mask)
+        e_det = float(e[self.determined].max()) if self.determined.any() else 0.0
+        z, i = np.unravel_index(int(np.argmax(e)), e.shape)
+        e_all = float(e[z, i])
+        spread = float(self.spread.max())
+        le = layer_err(sol, self.np).max()
+        line = (f"{name}: determined cells {int(self.determined.sum())}/{int(self.mask.sum())}, "
+                f"worst there {e_det:.2e}; worst masked cell {e_all:.2e} (layer {z}, species {i}; "
+                f"references' spread there {self.spread[z, i]:.2e}, worst spread "
+                f"{spread:.2e}); per-layer {le:.2e} vs references {self.layer_spread.max():.2e}")
+        return e_det, e_all, spread, line

@@ -79,3 +79,29 @@ def test_vul_schema_and_round_trip(tmp_path):
     assert np.array_equal(np.stack([var["k"][i] for i in range(1, col.net.nr + 1)]), col.k[1:])
     assert para["count"] == int(res["accept"][c]) and len(para["tableau20"]) == 20
     assert var["species"] == names and var["longdydt"] == var["longdy"] / var["dt"]
+    # still running after 20 attempts: the reference's end_case 1 (outer_loop.py:2326-2339)
+    assert para["end_case"] == 1 and para["device_status"] == int(res["status"][c])
+
+
+def test_vul_end_case_of_capped_and_converged_columns():
+    """end_case is the reference's code (1 converged, 3 step cap exceeded),
+    not the device status word: one column capped at 10 accepted steps, and a
+    small column run to convergence."""
+    from test_gpu_refresh import _columns
+    from test_gpu_steady import _small
+    from vulcan_amd import output
+    from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner, ST_CONVERGED, ST_MAXSTEPS
+    cols, _ = _columns(seed=7)
+    r = SteadyStateRunner(cols[:1], RunConfig(count_max=10))
+    r.run(max_attempts=400, check_every=20)
+    res = r.results()
+    assert res["status"][0] & ST_MAXSTEPS and not res["status"][0] & ST_CONVERGED
+    para = output.vul_dicts(r, 0, [f"S{i}" for i in range(cols[0].ni)])[2]
+    assert para["end_case"] == 3 and para["count"] == 10
+    c = _small(21)
+    r = SteadyStateRunner([c], RunConfig())
+    r.run(max_attempts=20000, check_every=100)
+    res = r.results()
+    assert res["status"][0] & ST_CONVERGED
+    para = output.vul_dicts(r, 0, [f"S{i}" for i in range(c.ni)])[2]
+    assert para["end_case"] == 1 and para["device_status"] & ST_CONVERGED

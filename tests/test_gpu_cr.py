@@ -88,6 +88,12 @@ def test_cr_step_cfg1_matches_oracle(cfg1):
     assert np.all(np.isfinite(_np(sol)))
     assert e_gpu[well].max() < 1e-10
     assert e_gpu.max() <= 10.0 * e_floor.max() + 1e-10
+    from parity_util import StepRefs
+    r = StepRefs(c.y, c.k, c.dt, c.atm, c.net)
+    e_det, e_all, spread, line = r.judge(_np(sol), "cr cfg1 elementwise")
+    print(line)
+    assert e_det < 1e-10
+    assert e_all <= 10.0 * spread + 1e-10
 
 
 def _uneven():

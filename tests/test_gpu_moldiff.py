@@ -92,3 +92,9 @@ def test_ros2_step_molecular_diffusion_matches_oracle(flags):
     assert well.sum() >= c.nz // 2
     assert err[well].max() < 1e-10
     assert err.max() <= 10.0 * floor.max() + 1e-10
+    from parity_util import StepRefs
+    r = StepRefs(c.y, c.k, dt, atm, c.net)
+    e_det, e_all, spread, line = r.judge(sol, f"flags {flags} elementwise")
+    print(line)
+    assert e_det < 1e-10
+    assert e_all <= 10.0 * spread + 1e-10

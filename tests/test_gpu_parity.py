@@ -168,6 +168,15 @@ def test_ros2_step_cfg1_matches_oracle(cfg1):
     assert well.sum() >= c.nz // 2
     assert e_gpu[well].max() < 1e-10
     assert e_gpu.max() <= 10.0 * e_floor.max() + 1e-10
+    # elementwise, over the cells the step control looks at (y >= atol,
+    # ymix >= mtol): 1e-10 where three reference solves agree to 1e-12
+    from parity_util import StepRefs
+    r = StepRefs(c.y, c.k, c.dt, c.atm, c.net)
+    e_det, e_all, spread, line = r.judge(_np(sol), "ros2 cfg1 elementwise")
+    print(line)
+    assert r.determined.sum() >= r.mask.sum() // 2
+    assert e_det < 1e-10
+    assert e_all <= 10.0 * spread + 1e-10
 
 
 def test_ros2_step_well_conditioned_1e10(cfg1):
@@ -179,6 +188,10 @@ def test_ros2_step_well_conditioned_1e10(cfg1):
     sol, _ = jax_step.jax_ros2_step(c.y, c.k, dt, c.atm, c.net)
     so, _ = O.ros2_step(c.y, c.k, dt, c.atm, c.net)
     assert _layer_err(_np(sol), so).max() < 1e-10
+    from parity_util import ATOL, elem_err
+    e = elem_err(_np(sol), so, np.abs(so) >= ATOL)
+    print(f"ros2 cfg1 at dt=1e-18: worst cell {e.max():.2e}")
+    assert e.max() < 1e-10
 
 
 def test_block_thomas_backward_error_on_ros2_system(cfg1):

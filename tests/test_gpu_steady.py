@@ -108,3 +108,42 @@ def test_uneven_batch_graph_replay_equals_eager():
     for i in range(len(cols)):
         assert np.array_equal(a["y"][i], b["y"][i]), i
     assert np.array_equal(a["attempts"], b["attempts"])
+
+
+def test_nonfinite_step_is_rejected_like_the_oracle():
+    """A step whose solution holds NaN is rejected, as the reference does:
+    all(sol_clip >= 0) is false for NaN and the NaN atom loss fails its test
+    (outer_loop.py:741-750).  A NaN rate row makes every attempt's solution
+    NaN, so dt halves on each reject, the loss counter counts them, and after
+    batch_max_retries, or once dt would fall below dt_min, the step is
+    force-accepted with max(y_prev, 0) and dt_min (outer_loop.py:753-768,
+    1031-1038): 6 forced steps in 20 attempts.  The same dt
+    sequence, accept flags, counters and state as the oracle; the state
+    stays finite and the non-finite status bit is set."""
+    import dataclasses
+    from oracle import ros2_np as O
+    from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner, ST_NONFINITE
+    c = _small(24)
+    k = c.k.copy()
+    k[5, :] = np.nan
+    c = dataclasses.replace(c, k=k)
+    tr = []
+    _, info = O.steady_state_run(c.y, k, c.atm, c.net, c.dt, trace=tr, max_attempts=20)
+    so = info["state"]
+    r = SteadyStateRunner([c], RunConfig())
+    dts = []
+    for _ in range(20):
+        dts.append(float(r.dt.item()))
+        r.attempt()
+    res = r.results()
+    dto = np.array([x["dt"] for x in tr])
+    print(f"dt gpu {dts[:10]} oracle {dto[:10]}; accepted {res['accept'][0]} vs "
+          f"{so.accept_count}; counts {res['counts']}")
+    assert np.array_equal(np.array(dts), dto)
+    assert int(res["accept"][0]) == so.accept_count == 6
+    assert int(res["counts"]["loss"][0]) == so.loss_count
+    assert int(res["counts"]["nega"][0]) == so.nega_count
+    assert int(res["counts"]["delta"][0]) == so.delta_count
+    assert float(res["t"][0]) == so.t and float(res["dt"][0]) == so.dt
+    assert np.all(np.isfinite(res["y"][0])) and np.array_equal(res["y"][0], so.y)
+    assert res["status"][0] & ST_NONFINITE

@@ -64,6 +64,54 @@ def test_block_thomas_transposed_swaps_sup_sub():
     assert np.max(np.abs(lam - ref)) < 1e-12 * np.max(np.abs(ref))
 
 
+def test_banded_solve_matches_dense():
+    """The banded (global-pivoting) solve used to judge which cells a step
+    determines: the seed-7 system and a badly scaled VULCAN-shaped one
+    (test_block_thomas_diag.py:71-77 magnitudes) against a dense solve; a
+    sup/sub swap or an off-by-one band offset fails both."""
+    for seed, nz, ni, scale in ((7, 8, 5, 10.0), (8, 12, 17, 1e10)):
+        rng = np.random.default_rng(seed)
+        diag = rng.standard_normal((nz, ni, ni)) + scale * np.eye(ni)
+        sup = rng.standard_normal((nz - 1, ni)) * (1.0 if scale < 100 else 1e-3)
+        sub = rng.standard_normal((nz - 1, ni)) * (1.0 if scale < 100 else 1e-3)
+        rhs = rng.standard_normal((nz, ni))
+        x = O.banded_solve(diag, sup, sub, rhs)
+        ref = np.linalg.solve(O.assemble_dense(diag, sup, sub), rhs.reshape(-1)).reshape(nz, ni)
+        assert np.max(np.abs(x - ref) / np.maximum(np.abs(ref), 1e-12)) < 1e-12
+    sup2 = rng.standard_normal((nz - 1, ni))
+    x_sw = O.banded_solve(diag, sup2, sup2 * 0.0, rhs)
+    ref_sw = np.linalg.solve(O.assemble_dense(diag, sup2, sup2 * 0.0), rhs.reshape(-1))
+    assert np.allclose(x_sw.reshape(-1), ref_sw, rtol=1e-12)
+
+
+def test_ros2_banded_equals_block_thomas_when_well_conditioned(cfg1):
+    """At dt = 1e-18 every block is dominated by c0 I: the banded and block
+    Thomas Ros2 steps agree to roundoff on every cell."""
+    c = cfg1
+    a, da = O.ros2_step(c.y, c.k, 1e-18, c.atm, c.net)
+    b, db = O.ros2_step_banded(c.y, c.k, 1e-18, c.atm, c.net)
+    assert np.max(np.abs(a - b) / np.maximum(np.abs(a), 1e-300)) < 1e-13
+
+
+def test_chem_term_scale_bounds_rhs(cfg1):
+    """The gross term sum is >= |chem_rhs| on every cell, and equals it for a
+    species that only ever appears on one side (one irreversible reaction)."""
+    c = cfg1
+    f = O.chem_rhs(c.y, c.atm.M, c.k, c.net)
+    g = O.chem_term_scale(c.y, c.atm.M, c.k, c.net)
+    assert np.all(g >= np.abs(f) * (1 - 1e-14))
+    ni, nr = 3, 2
+    r_idx = np.array([[3, 3], [0, 1], [3, 3]], np.int64)
+    p_idx = np.array([[3, 3], [2, 3], [3, 3]], np.int64)
+    r_st = np.array([[0, 0], [1.0, 1.0], [0, 0]])
+    p_st = np.array([[0, 0], [2.0, 0], [0, 0]])
+    net = NetworkArrays(ni, nr, r_idx, p_idx, r_st, p_st, np.zeros(nr + 1, bool))
+    y = np.array([[2.0, 3.0, 5.0]])
+    k = np.array([[0.0], [0.5], [0.0]])
+    np.testing.assert_allclose(O.chem_term_scale(y, np.ones(1), k, net), [[3.0, 3.0, 6.0]])
+    np.testing.assert_allclose(np.abs(O.chem_rhs(y, np.ones(1), k, net)), [[3.0, 3.0, 6.0]])
+
+
 # ---------------------------------------------------------------------------
 # chemistry
 # ---------------------------------------------------------------------------
