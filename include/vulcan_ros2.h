@@ -154,6 +154,22 @@ int vk_ros2_step_f64(vk_batch b, const double* y, const double* k, const double*
                      const double* dt, const uint8_t* fix_mask, double* sol, double* delta,
                      void* ws, size_t ws_bytes, int32_t* status, void* stream);
 
+/* The Ros2 step without a workspace argument and without any allocation
+ * (no hipMalloc, hipFree, stream or event creation).  No counterpart in the
+ * reference: it is for callers that cannot allocate, such as code under HIP
+ * graph capture and tight serving loops.  The workspace is a file-scope
+ * device array of VK_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends (160 MiB: one column of nz = 200, ni <= 96 in latency mode, or four
+ * such columns with the tile factor; vk_ros2_workspace_bytes says what a
+ * batch needs).  A batch needing more is refused with VK_ERR_UNSUPPORTED.
+ * Same arguments and results as vk_ros2_step_f64, bit for bit; the size
+ * buckets of an uneven batch run in order on the given stream.  Every call
+ * on a GPU shares its array: calls that overlap on one GPU race. */
+#define VK_NOALLOC_SCRATCH_BYTES (160ull << 20)
+int vk_ros2_step_noalloc_f64(vk_batch b, const double* y, const double* k, const double* atm,
+                             const double* dt, const uint8_t* fix_mask, double* sol,
+                             double* delta, int32_t* status, void* stream);
+
 /* Step error of the fixed-dt loop (integrate.py:30-33): out[c * stride] =
  * max over column c of delta / max(|sol|, 1e-30); NaN propagates. */
 int vk_step_error_f64(vk_batch b, const double* sol, const double* delta, double* out,

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <mutex>
 #include <numeric>
 #include <queue>
 #include <set>
@@ -42,6 +43,8 @@ struct Timing {
   long long count[VK_NKERN] = {0};
 };
 Timing g_timing;
+// Set while the allocation-free step runs: no timing events are taken.
+thread_local bool g_timing_suppressed = false;
 
 hipEvent_t take_event() {
   if (!g_timing.pool.empty()) {
@@ -59,7 +62,7 @@ struct KernelTimer {
   hipStream_t st;
   hipEvent_t a = nullptr, b = nullptr;
   KernelTimer(int k, hipStream_t s) : kind(k), st(s) {
-    if (!g_timing.on) return;
+    if (!g_timing.on || g_timing_suppressed) return;
     a = take_event();
     b = take_event();
     if (a && b) (void)hipEventRecord(a, st);
@@ -911,7 +914,8 @@ static bool ensure_aux(vk_batch b) {
 
 static int ros2_step_impl(vk_batch b, const double* y, const double* k, const double* atm,
                           const double* dt, const uint8_t* fix_mask, double* sol, double* delta,
-                          void* ws, int32_t* status, const int* active, hipStream_t st) {
+                          void* ws, int32_t* status, const int* active, hipStream_t st,
+                          bool fork = true) {
   char* w = (char*)ws;
   double* blk = (double*)w; w += align_up<double>(b->block_doubles);
   const size_t sv = align_up<double>(b->state_doubles);
@@ -1030,7 +1034,7 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
     }
     return VK_OK;
   };
-  if (ensure_aux(b)) {
+  if (fork && ensure_aux(b)) {
     // fork: every bucket's stream waits for the work before this point on st
     VK_CHECK_HIP(hipEventRecord(b->ev_fork, st));
     for (size_t i = 0; i < b->slots.size(); ++i) {
@@ -1063,6 +1067,78 @@ int vk_ros2_step_f64(vk_batch b, const double* y, const double* k, const double*
   }
   return ros2_step_impl(b, y, k, atm, dt, fix_mask, sol, delta, ws, status, nullptr,
                         (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The allocation-free Ros2 step.  Its workspace is a file-scope device
+// array; host code reaches it only through the address hipGetSymbolAddress
+// returns for the current device (the array's name is a host-side stand-in,
+// not a device address).  Each device holds its own copy until the process
+// ends; every call on a device shares it, so calls that overlap on one
+// device race.  Nothing in the step reads workspace memory it has not
+// written in the same call, so nothing needs clearing between calls.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[VK_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_of_current_device(void** out) {
+  int dev = -1;
+  VK_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    vk_set_error("vk_ros2_step_noalloc_f64: device %d outside 0..%d", dev, kMaxDevices - 1);
+    return VK_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      vk_set_error("vk_ros2_step_noalloc_f64: hipGetSymbolAddress of the scratch array on "
+                   "device %d failed: %s", dev, hipGetErrorString(e));
+      return VK_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return VK_OK;
+}
+
+struct TimingOff {
+  TimingOff() { g_timing_suppressed = true; }
+  ~TimingOff() { g_timing_suppressed = false; }
+};
+}  // namespace
+
+extern "C" {
+
+int vk_ros2_step_noalloc_f64(vk_batch b, const double* y, const double* k, const double* atm,
+                             const double* dt, const uint8_t* fix_mask, double* sol,
+                             double* delta, int32_t* status, void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_plain(b, "vk_ros2_step_noalloc_f64")) return rc;
+  if (!y || !k || !atm || !dt || !sol || !delta) {
+    vk_set_error("vk_ros2_step_noalloc_f64: null pointer");
+    return VK_ERR_ARG;
+  }
+  const size_t need = vk_ros2_workspace_bytes(b);
+  if (need > (size_t)VK_NOALLOC_SCRATCH_BYTES) {
+    vk_set_error("vk_ros2_step_noalloc_f64: this batch needs %zu bytes of workspace, more than "
+                 "the %llu-byte scratch array; split the batch or call vk_ros2_step_f64 with a "
+                 "workspace", need, (unsigned long long)VK_NOALLOC_SCRATCH_BYTES);
+    return VK_ERR_UNSUPPORTED;
+  }
+  void* ws = nullptr;
+  if (int rc = scratch_of_current_device(&ws)) return rc;
+  TimingOff no_events;
+  // the buckets run in order on the caller's stream (no helper streams)
+  return ros2_step_impl(b, y, k, atm, dt, fix_mask, sol, delta, ws, status, nullptr,
+                        (hipStream_t)stream, false);
 }
 
 static int check_ss(vk_batch b, const double* atm, const vk_ss_params* p, const vk_ss_state* s) {

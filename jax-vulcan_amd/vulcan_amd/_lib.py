@@ -42,6 +42,7 @@ _SIGS = {
     "vk_bt_inverse_natural_f64": (i32, [p, p, p, p]),
     "vk_ros2_workspace_bytes": (sz, [p]),
     "vk_ros2_step_f64": (i32, [p, p, p, p, p, p, p, p, p, sz, p, p]),
+    "vk_ros2_step_noalloc_f64": (i32, [p, p, p, p, p, p, p, p, p, p]),
     "vk_residual_f64": (i32, [p, p, p, p, p, p]),
     "vk_jacobian_blocks_f64": (i32, [p, p, p, p, p, p, p, p]),
     "vk_ss_init_f64": (i32, [p, p, p, p, p]),
