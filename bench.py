@@ -20,6 +20,7 @@ import time
 from pathlib import Path
 
 FACTOR_CR = 6    # vk_tune_set(VK_TUNE_FACTOR, sl, VK_FACTOR_CR): cyclic-reduction factor
+VK_TUNE_REFINE, VK_TUNE_SPLIT = 2, 3
 
 ROOT = Path(__file__).resolve().parent
 sys.path[:0] = [str(ROOT), str(ROOT / "jax-vulcan_amd")]
@@ -131,6 +132,12 @@ def main() -> int:
                     default="auto",
                     help="block-factor kernel for the nz=150 / ni=93 bucket (auto: the "
                          "library default)")
+    ap.add_argument("--split", type=int, default=0,
+                    help="column chunks per size bucket, each on its own stream "
+                         "(VK_TUNE_SPLIT; 0: the library default)")
+    ap.add_argument("--refine", type=int, default=-1,
+                    help="refined block solves (VK_TUNE_REFINE: 0 none, 1 stage 1, 2 both; "
+                         "-1: the library default)")
     ap.add_argument("--conv-cost", type=int, default=0,
                     help="columns (nz=150, ni=93) for timing one attempt under the "
                          "relative-change rule and under the reference's longdy rule "
@@ -174,6 +181,10 @@ def main() -> int:
                                    {"tile": 0, "rows": 1, "blocked": 2, "tile2": 3, "wave": 4, "tile8": 5,
                                     "cr": FACTOR_CR}[args.factor]),
                    "vk_tune_set")
+    if args.split > 0:
+        _lib.check(lib.vk_tune_set(VK_TUNE_SPLIT, 0, args.split), "vk_tune_set")
+    if args.refine >= 0:
+        _lib.check(lib.vk_tune_set(VK_TUNE_REFINE, 0, args.refine), "vk_tune_set")
     B = args.batch
     t_gen = time.time()
     cols = synth.columns_same_network(1002, B, NZ, NI, NREV, NPHOTO, dt=1e-10, name="bench")

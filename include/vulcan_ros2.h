@@ -228,14 +228,21 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
 /* Knob VK_TUNE_REFINE (sl ignored): one step of iterative refinement of the
  * Ros2 step's block solves.  A solve with explicit block inverses is not
  * backward stable; the refined one is, as the reference's lu_solve is
- * (solver.py:91-112).  Tile-layout factors refine each block solve of the
- * backward sweep, x_j = X_j t_j; x_j += X_j (t_j - A'_j x_j) with A'_j the
- * Schur complement X_j inverts (one extra read of the diagonal-block stack
- * per refined sweep); the cyclic-reduction factor takes one whole-column
- * step (residual, correction solve).  0: none; 1: the stage-1 solve;
+ * (solver.py:91-112).  The block-Thomas factors refine each block solve of
+ * the backward sweep, x_j = X_j t_j; x_j += X_j (t_j - A'_j x_j) with A'_j
+ * the Schur complement X_j inverts (one extra read of the diagonal-block
+ * stack per refined sweep); the cyclic-reduction factor takes one
+ * whole-column step (residual, correction solve).  0: none; 1: the stage-1 solve;
  * 2 (default): both stages.  Unrefined, a step leaves up to ~1e-9 relative
  * error on cells that the reference's LU determines to 1e-12. */
 #define VK_TUNE_REFINE 2
+/* Knob VK_TUNE_SPLIT (sl ignored): 1..8 column chunks per size bucket in
+ * vk_ros2_step_f64 and the steady-state attempt (default 1; a chunk holds at
+ * least 128 columns).  Each chunk runs on its own stream, so one chunk's
+ * latency-bound block factor can overlap another's bandwidth-bound
+ * chemistry and sweeps; measured on MI355X the factor then slows more than
+ * the overlap gains.  Results are bitwise independent of the split. */
+#define VK_TUNE_SPLIT 3
 int vk_tune_set(int knob, int sl, int value);
 int vk_tune_get(int knob, int sl); /* -1 for an unknown knob / bucket */
 

@@ -844,7 +844,7 @@ int vk_timing_read(double* ms, long long* count, int n) {
 static int g_mol_precompute = 1;
 
 // VK_TUNE_REFINE: 0 none; 1 the stage-1 solve is refined; 2 (default) both
-// stages' solves.  Tile-layout factors refine each block solve of the
+// stages' solves.  The block-Thomas factors refine each block solve of the
 // backward sweep; the cyclic-reduction factor takes one whole-column
 // refinement step (residual, correction solve).
 static int g_refine = 2;
@@ -890,10 +890,10 @@ size_t vk_ros2_workspace_bytes(vk_batch b) {
 
 // Helper streams and events for running the SL buckets of a batch
 // concurrently; created on first use.  false: run the buckets in order.
-static bool ensure_aux(vk_batch b) {
-  if (b->slots.size() < 2) return false;
-  if (b->aux.size() == b->slots.size()) return true;
-  for (size_t i = b->aux.size(); i < b->slots.size(); ++i) {
+static bool ensure_aux(vk_batch b, size_t n) {
+  if (n < 2) return false;
+  if (b->aux.size() >= n) return true;
+  for (size_t i = b->aux.size(); i < n; ++i) {
     hipStream_t a = nullptr;
     hipEvent_t e = nullptr;
     if (hipStreamCreateWithFlags(&a, hipStreamNonBlocking) != hipSuccess ||
@@ -911,6 +911,13 @@ static bool ensure_aux(vk_batch b) {
   }
   return true;
 }
+
+// Column chunks per size bucket in the Ros2 step (VK_TUNE_SPLIT): each
+// chunk runs the bucket's kernel sequence on its own stream, so the
+// latency-bound factor of one chunk overlaps the bandwidth-bound chemistry
+// and sweeps of another.  A chunk holds at least kMinChunk columns.
+static int g_split = 1;
+constexpr int kMinChunk = 128;
 
 static int ros2_step_impl(vk_batch b, const double* y, const double* k, const double* atm,
                           const double* dt, const uint8_t* fix_mask, double* sol, double* delta,
@@ -932,8 +939,11 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
   double* mst = (double*)w;  // 8 components of the molecular-diffusion static factors
   const int64_t mstride = (int64_t)(sv / sizeof(double));
   double* crw = (double*)((char*)ws + ros2_ws_base_bytes(b));  // cyclic-reduction levels
-  double* inv = (double*)((char*)ws + ros2_ws_inv_offset(b));  // block inverses X_j
-  const double* ref1 = g_refine >= 1 ? blk : nullptr;          // D for the refinement
+  // with refinement the factor writes the inverses X_j to the second block
+  // stack and the diagonal blocks D_j stay for the refined solves; without,
+  // the inverses overwrite D_j
+  double* inv = g_refine > 0 ? (double*)((char*)ws + ros2_ws_inv_offset(b)) : blk;
+  const double* ref1 = g_refine >= 1 ? blk : nullptr;
   const double* ref2 = g_refine >= 2 ? blk : nullptr;
   const double r = ros2_gamma();
   {
@@ -949,14 +959,21 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
       mst = nullptr;
     }
   }
-  // The kernel sequence of one SL bucket on stream s.
-  auto run_bucket = [&](int sl, hipStream_t s) -> int {
+  // The kernel sequence of one SL bucket over the columns [c0, c0 + nc) on
+  // stream s.  Every per-column array the kernels index by position (the
+  // descriptors, dt, status, active) is offset to the range; state arrays
+  // are addressed through the descriptors' offsets.
+  auto run_bucket = [&](int sl, int c0, int nc, hipStream_t s) -> int {
     int rc;
+    const VkCol* cols = b->d_cols + c0;
+    const double* dtc = dt + c0;
+    int32_t* stc = status ? status + c0 : nullptr;
+    const int* act = active ? active + c0 : nullptr;
     {
       KernelTimer t(0, s);
-      if ((rc = vk_launch_chem_layer(sl, M_LHS, b->d_cols, b->ncols, b->max_nz, b->max_S,
-                                     b->max_T, b->d_nets, y, nullptr, atm, kT, dt, fix_mask,
-                                     nullptr, rhs_y, blk, sup, sub, active, 1, mst, mstride, s)))
+      if ((rc = vk_launch_chem_layer(sl, M_LHS, cols, nc, b->max_nz, b->max_S,
+                                     b->max_T, b->d_nets, y, nullptr, atm, kT, dtc, fix_mask,
+                                     nullptr, rhs_y, blk, sup, sub, act, 1, mst, mstride, s)))
         return rc;
     }
     // The factor fuses the stage-1 forward sweep unless the blocked variant
@@ -967,12 +984,12 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
     {
       KernelTimer t(1, s);
       if (cr)  // the stage-1 reduction of rhs_y rides in the update launches
-        rc = vk_launch_cr_factor(sl, b->d_cols, b->ncols, b->max_nz, blk, sup, sub, crw,
-                                 (int*)status, active, s, rhs_y);
+        rc = vk_launch_cr_factor(sl, cols, nc, b->max_nz, blk, sup, sub, crw,
+                                 (int*)stc, act, s, rhs_y);
       else
-        rc = vk_launch_bt_factor(sl, b->d_cols, b->ncols, blk, 1, 0, sup, sub, inv,
-                                 fused ? rhs_y : nullptr, fused ? rmod : nullptr, (int*)status,
-                                 active, s);
+        rc = vk_launch_bt_factor(sl, cols, nc, blk, 1, 0, sup, sub, inv,
+                                 fused ? rhs_y : nullptr, fused ? rmod : nullptr, (int*)stc,
+                                 act, s);
       if (rc) return rc;
     }
     SolveEpi e1{1, r, 0.0, y, nullptr, nullptr, yk2, nullptr};
@@ -982,29 +999,29 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
         // x0 into k2 (free until stage 2), its residual into rmod, then the
         // correction solve writes k1 = x0 + dx and yk2
         SolveEpi e0{};
-        if ((rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rhs_y, sup, sub, crw,
-                                     k2, e0, active, s, true)))
+        if ((rc = vk_launch_cr_solve(sl, cols, nc, b->max_nz, rhs_y, sup, sub, crw,
+                                     k2, e0, act, s, true)))
           return rc;
-        if ((rc = vk_launch_bt_residual(sl, b->d_cols, b->ncols, b->max_nz, blk, sup, sub, rhs_y,
-                                        k2, rmod, active, s)))
+        if ((rc = vk_launch_bt_residual(sl, cols, nc, b->max_nz, blk, sup, sub, rhs_y,
+                                        k2, rmod, act, s)))
           return rc;
         SolveEpi e1r = e1;
         e1r.base = k2;
-        rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rmod, sup, sub, crw, k1, e1r,
-                                active, s);
+        rc = vk_launch_cr_solve(sl, cols, nc, b->max_nz, rmod, sup, sub, crw, k1, e1r,
+                                act, s);
       } else if (cr)
-        rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rhs_y, sup, sub, crw, k1,
-                                e1, active, s, true);
+        rc = vk_launch_cr_solve(sl, cols, nc, b->max_nz, rhs_y, sup, sub, crw, k1,
+                                e1, act, s, true);
       else
-        rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, inv, sup, sub, fused ? rmod : rhs_y,
-                                rmod, k1, fused ? 1 : 0, e1, active, s, ref1);
+        rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, fused ? rmod : rhs_y,
+                                rmod, k1, fused ? 1 : 0, e1, act, s, ref1);
       if (rc) return rc;
     }
     {
       KernelTimer t(3, s);
-      if ((rc = vk_launch_chem_layer(sl, M_RHS2, b->d_cols, b->ncols, b->max_nz, b->max_S,
-                                     b->max_T, b->d_nets, yk2, nullptr, atm, kT, dt, fix_mask, k1,
-                                     rhs2, nullptr, nullptr, nullptr, active, 1, mst, mstride, s)))
+      if ((rc = vk_launch_chem_layer(sl, M_RHS2, cols, nc, b->max_nz, b->max_S,
+                                     b->max_T, b->d_nets, yk2, nullptr, atm, kT, dtc, fix_mask, k1,
+                                     rhs2, nullptr, nullptr, nullptr, act, 1, mst, mstride, s)))
         return rc;
     }
     SolveEpi e2{2, 3.0 / (2.0 * r), 1.0 / (2.0 * r), y, k1, yk2, sol, delta};
@@ -1014,41 +1031,54 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
         // x0 into rmod, its residual into rhs_y (free after stage 1), then
         // the correction solve writes k2 = x0 + dx, sol and delta
         SolveEpi e0{};
-        if ((rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rhs2, sup, sub, crw,
-                                     rmod, e0, active, s)))
+        if ((rc = vk_launch_cr_solve(sl, cols, nc, b->max_nz, rhs2, sup, sub, crw,
+                                     rmod, e0, act, s)))
           return rc;
-        if ((rc = vk_launch_bt_residual(sl, b->d_cols, b->ncols, b->max_nz, blk, sup, sub, rhs2,
-                                        rmod, rhs_y, active, s)))
+        if ((rc = vk_launch_bt_residual(sl, cols, nc, b->max_nz, blk, sup, sub, rhs2,
+                                        rmod, rhs_y, act, s)))
           return rc;
         SolveEpi e2r = e2;
         e2r.base = rmod;
-        rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rhs_y, sup, sub, crw, k2, e2r,
-                                active, s);
+        rc = vk_launch_cr_solve(sl, cols, nc, b->max_nz, rhs_y, sup, sub, crw, k2, e2r,
+                                act, s);
       } else if (cr)
-        rc = vk_launch_cr_solve(sl, b->d_cols, b->ncols, b->max_nz, rhs2, sup, sub, crw, k2, e2,
-                                active, s);
+        rc = vk_launch_cr_solve(sl, cols, nc, b->max_nz, rhs2, sup, sub, crw, k2, e2,
+                                act, s);
       else
-        rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, inv, sup, sub, rhs2, rmod, k2, 0, e2,
-                                active, s, ref2);
+        rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, rhs2, rmod, k2, 0, e2,
+                                act, s, ref2);
       if (rc) return rc;
     }
     return VK_OK;
   };
-  if (fork && ensure_aux(b)) {
-    // fork: every bucket's stream waits for the work before this point on st
+  // column chunks: contiguous index ranges, each running every bucket (a
+  // kernel returns at once for a column of another bucket)
+  const int nchunk = std::max(1, std::min(g_split, b->ncols / kMinChunk));
+  const size_t nstreams = (size_t)nchunk * b->slots.size();
+  auto chunk = [&](int i, int* c0, int* nc) {
+    *c0 = (int)((int64_t)b->ncols * i / nchunk);
+    *nc = (int)((int64_t)b->ncols * (i + 1) / nchunk) - *c0;
+  };
+  if (fork && ensure_aux(b, nstreams)) {
+    // fork: every stream waits for the work before this point on st
     VK_CHECK_HIP(hipEventRecord(b->ev_fork, st));
-    for (size_t i = 0; i < b->slots.size(); ++i) {
-      VK_CHECK_HIP(hipStreamWaitEvent(b->aux[i], b->ev_fork, 0));
-      if (int rc = run_bucket(b->slots[i], b->aux[i])) return rc;
-      VK_CHECK_HIP(hipEventRecord(b->ev_join[i], b->aux[i]));
+    size_t i = 0;
+    for (int ch = 0; ch < nchunk; ++ch) {
+      int c0, nc;
+      chunk(ch, &c0, &nc);
+      for (int sl : b->slots) {
+        VK_CHECK_HIP(hipStreamWaitEvent(b->aux[i], b->ev_fork, 0));
+        if (int rc = run_bucket(sl, c0, nc, b->aux[i])) return rc;
+        VK_CHECK_HIP(hipEventRecord(b->ev_join[i], b->aux[i]));
+        ++i;
+      }
     }
-    // join: st waits for every bucket
-    for (size_t i = 0; i < b->slots.size(); ++i)
-      VK_CHECK_HIP(hipStreamWaitEvent(st, b->ev_join[i], 0));
+    // join: st waits for every stream
+    for (size_t j = 0; j < nstreams; ++j) VK_CHECK_HIP(hipStreamWaitEvent(st, b->ev_join[j], 0));
     return VK_OK;
   }
   for (int sl : b->slots)
-    if (int rc = run_bucket(sl, st)) return rc;
+    if (int rc = run_bucket(sl, 0, b->ncols, st)) return rc;
   return VK_OK;
 }
 
@@ -1253,6 +1283,14 @@ int vk_step_error_f64(vk_batch b, const double* sol, const double* delta, double
 
 int vk_tune_set(int knob, int sl, int value) {
   g_err[0] = 0;
+  if (knob == VK_TUNE_SPLIT) {
+    if (value < 1 || value > 8) {
+      vk_set_error("vk_tune_set: VK_TUNE_SPLIT takes 1..8, not %d", value);
+      return VK_ERR_ARG;
+    }
+    g_split = value;
+    return VK_OK;
+  }
   if (knob == VK_TUNE_REFINE) {
     if (value < 0 || value > 2) {
       vk_set_error("vk_tune_set: VK_TUNE_REFINE takes 0, 1 or 2, not %d", value);
@@ -1285,6 +1323,7 @@ int vk_tune_set(int knob, int sl, int value) {
 int vk_tune_get(int knob, int sl) {
   if (knob == VK_TUNE_MOLPRE) return g_mol_precompute;
   if (knob == VK_TUNE_REFINE) return g_refine;
+  if (knob == VK_TUNE_SPLIT) return g_split;
   if (knob != VK_TUNE_FACTOR || sl < 1 || sl > 7) return -1;
   return g_factor_variant[sl];
 }

@@ -384,17 +384,31 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(TILE_IN &
 //   epi 1: out1 = y + x / e0                          (Ros2 stage-1 yk2)
 //   epi 2: out1 = y + e0*k1 + e1*x, out2 = |out1 - yk2|  (Ros2 sol, delta)
 //
-// With dstk (the tile-layout diagonal blocks D_j the inverses were formed
-// from) each block solve of the backward sweep takes one step of iterative
-// refinement:
-//   x_j = X_j t_j;  x_j += X_j (t_j - A'_j x_j),
-//   A'_j x = D_j x - c_{j-1} o (X_{j-1} (b_{j-1} o x))   (b = sup, c = sub),
+// REFINE 1 (dstk holds the tile-layout diagonal blocks D_j the inverses were
+// formed from): each block solve of the backward sweep takes one step of
+// iterative refinement,
+//   x_j = X_j t_j + X_j (t_j - A'_j X_j t_j),
+//   A'_j v = D_j v - c_{j-1} o (X_{j-1} (b_{j-1} o v))   (b = sup, c = sub),
 // A'_j being the Schur complement whose inverse X_j is.  A product with an
 // explicit inverse is not backward stable; the refined block solve is, as
-// the reference's lu_solve on A'_j is (solver.py:91-112).  D_j and X_{j-1}
-// stream from memory, so the sweep keeps one register tile.
+// the reference's lu_solve on A'_j is (solver.py:91-112).  The correction of
+// x_j is made in the next step down, where X_{j-1} is loaded anyway and X_j
+// is still in registers, so a refined sweep reads each inverse once and D
+// once.
 
-template <int SL>
+__device__ inline void solve_epilogue(const SolveEpi& epi, int64_t e, double xv, double* x) {
+  x[e] = xv;
+  if (epi.kind == 1) {
+    epi.out1[e] = epi.y[e] + xv / epi.e0;
+  } else if (epi.kind == 2) {
+#pragma clang fp contract(off)
+    const double sol = epi.y[e] + epi.e0 * epi.k1[e] + epi.e1 * xv;
+    epi.out1[e] = sol;
+    epi.out2[e] = fabs(sol - epi.yk2[e]);
+  }
+}
+
+template <int SL, int REFINE>
 __global__ __launch_bounds__(VK_NT) void k_bt_solve(
     const VkCol* __restrict__ cols, int ncols, const double* __restrict__ inv,
     const double* __restrict__ sup_d, const double* __restrict__ sub_d,
@@ -448,11 +462,78 @@ __global__ __launch_bounds__(VK_NT) void k_bt_solve(
   for (int i = tid; i < VK_G * SL; i += VK_NT)
     vec[(nz - 1) & 1][i] = (i < ni) ? rm[col.off_s + (int64_t)(nz - 1) * ni + i] : 0.0;
   __syncthreads();
-  const double* dblk = dstk ? dstk + col.off_b : nullptr;
-  for (int j = nz - 1; j >= 0; --j) {
-    load_tile<SL>(a, stk + (int64_t)j * BLK);
-    tile_matvec<SL>(a, vec[j & 1], w);
-    if (dblk) {
+  if constexpr (REFINE == 0) {
+    for (int j = nz - 1; j >= 0; --j) {
+      load_tile<SL>(a, stk + (int64_t)j * BLK);
+      tile_matvec<SL>(a, vec[j & 1], w);
+      if (tc == 0) {
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          const int i = s * VK_G + tr;
+          if (i < ni) {
+            const int64_t e = col.off_s + (int64_t)j * ni + i;
+            const double xv = w[s];
+            solve_epilogue(epi, e, xv, x);
+            if (j > 0) {
+              const int64_t e2 = col.off_s + (int64_t)(j - 1) * ni + i;
+              vec[(j - 1) & 1][i] = rm[e2] - bsup[(int64_t)(j - 1) * ni + i] * xv;
+            }
+          } else if (j > 0) {
+            vec[(j - 1) & 1][i] = 0.0;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  } else {
+    // step j: load X_j; finish layer u = j + 1 (its residual needs X_j, its
+    // correction X_u, kept in xp); form x0_j = X_j t_j.  rv holds x0_u, bx
+    // holds b_j o x0_u, w (registers) x0_u.
+    const double* dblk = dstk + col.off_b;
+    double xp[SL][SL];
+    for (int j = nz - 1; j >= -1; --j) {
+      if (j >= 0) load_tile<SL>(a, stk + (int64_t)j * BLK);
+      if (j < nz - 1) {
+        const int u = j + 1;
+        double q[SL], pz[SL];
+        stream_matvec<SL>(dblk + (int64_t)u * BLK, rv, q);   // D_u x0_u
+        if (j >= 0) tile_matvec<SL>(a, bx, pz);              // X_j (b_j o x0_u)
+        __syncthreads();
+        if (tc == 0) {
+#pragma unroll
+          for (int s = 0; s < SL; ++s) {
+            const int i = s * VK_G + tr;
+            double r = 0.0;
+            if (i < ni) {
+              double ax = q[s];
+              if (j >= 0) ax = ax - bsub[(int64_t)j * ni + i] * pz[s];
+              r = vec[u & 1][i] - ax;
+            }
+            rv[i] = r;
+          }
+        }
+        __syncthreads();
+        double d[SL];
+        tile_matvec<SL>(xp, rv, d);                          // X_u (t_u - A'_u x0_u)
+        if (tc == 0) {
+#pragma unroll
+          for (int s = 0; s < SL; ++s) {
+            const int i = s * VK_G + tr;
+            if (i < ni) {
+              const double xv = w[s] + d[s];
+              solve_epilogue(epi, col.off_s + (int64_t)u * ni + i, xv, x);
+              if (j >= 0)
+                vec[j & 1][i] = rm[col.off_s + (int64_t)j * ni + i] -
+                                bsup[(int64_t)j * ni + i] * xv;
+            } else if (j >= 0) {
+              vec[j & 1][i] = 0.0;
+            }
+          }
+        }
+        __syncthreads();
+      }
+      if (j < 0) break;
+      tile_matvec<SL>(a, vec[j & 1], w);                     // x0_j
       if (tc == 0) {
 #pragma unroll
         for (int s = 0; s < SL; ++s) {
@@ -462,55 +543,11 @@ __global__ __launch_bounds__(VK_NT) void k_bt_solve(
         }
       }
       __syncthreads();
-      double q[SL], pz[SL];
-      stream_matvec<SL>(dblk + (int64_t)j * BLK, rv, q);
-      if (j > 0) stream_matvec<SL>(stk + (int64_t)(j - 1) * BLK, bx, pz);
-      __syncthreads();
-      if (tc == 0) {
 #pragma unroll
-        for (int s = 0; s < SL; ++s) {
-          const int i = s * VK_G + tr;
-          double r = 0.0;
-          if (i < ni) {
-            double ax = q[s];
-            if (j > 0) ax = ax - bsub[(int64_t)(j - 1) * ni + i] * pz[s];
-            r = vec[j & 1][i] - ax;
-          }
-          rv[i] = r;
-        }
-      }
-      __syncthreads();
-      double d[SL];
-      tile_matvec<SL>(a, rv, d);
+      for (int s = 0; s < SL; ++s)
 #pragma unroll
-      for (int s = 0; s < SL; ++s) w[s] = w[s] + d[s];
+        for (int t = 0; t < SL; ++t) xp[s][t] = a[s][t];
     }
-    if (tc == 0) {
-#pragma unroll
-      for (int s = 0; s < SL; ++s) {
-        const int i = s * VK_G + tr;
-        if (i < ni) {
-          const int64_t e = col.off_s + (int64_t)j * ni + i;
-          const double xv = w[s];
-          x[e] = xv;
-          if (epi.kind == 1) {
-            epi.out1[e] = epi.y[e] + xv / epi.e0;
-          } else if (epi.kind == 2) {
-#pragma clang fp contract(off)
-            const double sol = epi.y[e] + epi.e0 * epi.k1[e] + epi.e1 * xv;
-            epi.out1[e] = sol;
-            epi.out2[e] = fabs(sol - epi.yk2[e]);
-          }
-          if (j > 0) {
-            const int64_t e2 = col.off_s + (int64_t)(j - 1) * ni + i;
-            vec[(j - 1) & 1][i] = rm[e2] - bsup[(int64_t)(j - 1) * ni + i] * xv;
-          }
-        } else if (j > 0) {
-          vec[(j - 1) & 1][i] = 0.0;
-        }
-      }
-    }
-    __syncthreads();
   }
 }
 
@@ -1248,9 +1285,15 @@ int vk_launch_bt_solve(int sl, const VkCol* cols, int ncols, const double* inv,
                        const double* sup_d, const double* sub_d, const double* rhs,
                        double* rscratch, double* x, int mode, SolveEpi epi, const int* active,
                        hipStream_t stream, const double* refine_diag) {
-  VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_bt_solve<SL_>, dim3(ncols), dim3(VK_NT), 0, stream,
-                                        cols, ncols, inv, sup_d, sub_d, rhs, rscratch, x, mode,
-                                        epi, active, refine_diag));
+  if (refine_diag) {
+    VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_bt_solve<SL_, 1>), dim3(ncols), dim3(VK_NT), 0,
+                                          stream, cols, ncols, inv, sup_d, sub_d, rhs, rscratch,
+                                          x, mode, epi, active, refine_diag));
+  } else {
+    VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_bt_solve<SL_, 0>), dim3(ncols), dim3(VK_NT), 0,
+                                          stream, cols, ncols, inv, sup_d, sub_d, rhs, rscratch,
+                                          x, mode, epi, active, nullptr));
+  }
   VK_CHECK_HIP(hipGetLastError());
   return VK_OK;
 }
