@@ -9,6 +9,7 @@ float64 tensors on the GPU.
 """
 from __future__ import annotations
 
+import re
 from pathlib import Path
 
 import numpy as np
@@ -26,23 +27,37 @@ P0 = 1.0e6               # gibbs.py:12
 CORR = KB / P0           # gibbs.py:13
 
 
+# a decimal number with an optional exponent, Fortran's D exponent included
+_NUMBER = re.compile(r"[-+]?(?:\d+\.\d*|\.\d+|\d+)(?:[EeDd][-+]?\d+)?")
+
+
+def _nasa9_fit(path: Path) -> np.ndarray:
+    """The (2, 10) coefficients of one NASA-9 fit file: the low and the high
+    temperature range, ten numbers each (a1..a7, an unused slot, b1, b2), in
+    file order; anything after the first twenty numbers is ignored."""
+    numbers = [float(tok.translate(_D_TO_E)) for tok in _NUMBER.findall(path.read_text())]
+    if len(numbers) < 20:
+        raise ValueError(f"{path}: two ranges of ten NASA-9 coefficients expected, "
+                         f"{len(numbers)} numbers found")
+    return np.asarray(numbers[:20], dtype=np.float64).reshape(2, 10)
+
+
+_D_TO_E = str.maketrans("Dd", "Ee")
+
+
 def load_nasa9(species, thermo_dir):
-    """(coeffs (ni, 2, 10), present (ni,)) from ``<thermo_dir>/NASA9/<sp>.txt``;
-    the low-T range first, zeros for a species without a file
-    (gibbs.py:16-38)."""
-    thermo_dir = Path(thermo_dir)
-    coeffs = np.zeros((len(species), 2, 10), dtype=np.float64)
-    present = np.zeros(len(species), dtype=bool)
-    for j, sp in enumerate(species):
-        fp = thermo_dir / "NASA9" / f"{sp}.txt"
-        if not fp.exists():
-            continue
-        flat = np.loadtxt(fp).flatten()
-        if flat.size < 20:
-            raise ValueError(f"NASA-9 file {fp} has only {flat.size} entries; need at least 20")
-        coeffs[j, 0] = flat[0:10]
-        coeffs[j, 1] = flat[10:20]
-        present[j] = True
+    """(coeffs (ni, 2, 10), present (ni,)) for the species names, read from
+    the files ``<thermo_dir>/NASA9/<name>.txt``; a species without a file
+    keeps zero coefficients and present False (gibbs.py:16-38)."""
+    folder = Path(thermo_dir) / "NASA9"
+    names = list(species)
+    fits = [_nasa9_fit(folder / f"{n}.txt") if (folder / f"{n}.txt").is_file() else None
+            for n in names]
+    present = np.array([f is not None for f in fits], dtype=bool)
+    coeffs = np.zeros((len(names), 2, 10), dtype=np.float64)
+    for j, f in enumerate(fits):
+        if f is not None:
+            coeffs[j] = f
     return coeffs, present
 
 

@@ -621,13 +621,28 @@ def control_update(s: ControlState, sol, delta_arr, n0, compo, atom_ini,
 # photolysis (photo.py)
 # ---------------------------------------------------------------------------
 
-def compute_tau(y, dz, absp_idx, absp_cross, scat_idx, scat_cross):
-    """tau (nz+1, nbin), top-down cumulative; photo.py:84-123."""
+def _t_abs(v, absp_T_idx, absp_T_cross):
+    """sum over T-dependent absorbers s of v[z, idx_s] * cross_T[s, z, :]
+    (photo.py:107-111, 156-160); zeros when there are none."""
+    nz = v.shape[0]
+    if absp_T_idx is None or np.asarray(absp_T_idx).size == 0:
+        return None
+    cT = np.asarray(absp_T_cross)                       # (n_T, nz, nbin)
+    return np.sum(v[:, absp_T_idx][:, :, None] * np.transpose(cT, (1, 0, 2)), axis=1)
+
+
+def compute_tau(y, dz, absp_idx, absp_cross, scat_idx, scat_cross, absp_T_idx=None,
+                absp_T_cross=None):
+    """tau (nz+1, nbin), top-down cumulative; photo.py:84-123, with the
+    temperature-dependent absorbers (n_T, nz, nbin) of photo.py:107-111."""
     nz = y.shape[0]
     nbin = absp_cross.shape[1]
     per = np.zeros((nz, nbin))
     if absp_idx.size:
         per = per + (y[:, absp_idx] @ absp_cross) * dz[:, None]
+    tT = _t_abs(y, absp_T_idx, absp_T_cross)
+    if tT is not None:
+        per = per + tT * dz[:, None]
     if scat_idx.size:
         per = per + (y[:, scat_idx] @ scat_cross) * dz[:, None]
     tau = np.zeros((nz + 1, nbin))
@@ -636,13 +651,18 @@ def compute_tau(y, dz, absp_idx, absp_cross, scat_idx, scat_cross):
 
 
 def compute_flux(tau, sflux_top, ymix, absp_idx, absp_cross, scat_idx, scat_cross,
-                 bins, mu_zenith, edd, ag0, hc, dflux_u_prev):
-    """Two-stream Eddington flux; photo.py:126-245 (ag0 = 0 branch)."""
+                 bins, mu_zenith, edd, ag0, hc, dflux_u_prev, absp_T_idx=None,
+                 absp_T_cross=None):
+    """Two-stream Eddington flux; photo.py:126-245 (both ag0 branches; the
+    T-dependent absorbers enter w0, photo.py:156-160)."""
     mu_ang = -1.0 * mu_zenith
     nz = tau.shape[0] - 1
     nbin = tau.shape[1]
     dtau = tau[:-1] - tau[1:]
     tot_abs = ymix[:, absp_idx] @ absp_cross if absp_idx.size else np.zeros_like(dtau)
+    tT = _t_abs(ymix, absp_T_idx, absp_T_cross)
+    if tT is not None:
+        tot_abs = tot_abs + tT
     tot_scat = ymix[:, scat_idx] @ scat_cross if scat_idx.size else np.zeros_like(dtau)
     with np.errstate(invalid="ignore", divide="ignore"):
         w0 = tot_scat / np.maximum(tot_abs + tot_scat, 1e-300)
@@ -693,6 +713,19 @@ def compute_J(aflux, cross_J, din12, dbin1, dbin2):
                                       + f1[:, -1][None] * c1[:, -1][:, None]) * dbin1
     J2 = (c2 @ f2.T) * dbin2 - 0.5 * (f2[:, 0][None] * c2[:, 0][:, None]
                                       + f2[:, -1][None] * c2[:, -1][:, None]) * dbin2
+    return J1 + J2
+
+
+def compute_J_T(aflux, cross_J_T, din12, dbin1, dbin2):
+    """J of the temperature-dependent branches (n_brT, nz): per layer z the
+    same split trapezoid against cross_J_T[:, z, :]; photo.py:334-347."""
+    cT = np.asarray(cross_J_T)
+    f1, f2 = aflux[:, :din12], aflux[:, din12:]
+    c1, c2 = cT[:, :, :din12], cT[:, :, din12:]
+    J1 = np.sum(f1[None] * c1, axis=2) * dbin1 - 0.5 * (
+        f1[:, 0][None] * c1[:, :, 0] + f1[:, -1][None] * c1[:, :, -1]) * dbin1
+    J2 = np.sum(f2[None] * c2, axis=2) * dbin2 - 0.5 * (
+        f2[:, 0][None] * c2[:, :, 0] + f2[:, -1][None] * c2[:, :, -1]) * dbin2
     return J1 + J2
 
 

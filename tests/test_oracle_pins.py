@@ -442,6 +442,158 @@ def test_flux_two_stream_single_layer_energy_balance():
     assert np.isfinite(af).all() and bottom_net <= top_in * (1 + 1e-9)
 
 
+def _lagged_fixed_point(tau, w0, ag0, mu=0.7, bins=(150.0, 250.0), max_iter=5000):
+    """compute_flux fed its own dflux_u until it stops changing (the state
+    the in-loop lag converges to), one absorber and one scatterer mixed so
+    that the single-scattering albedo is w0 in every layer."""
+    nz = tau.shape[0] - 1
+    bins = np.asarray(bins)
+    nbin = bins.size
+    ymix = np.full((nz, 2), 0.5)
+    du = np.zeros((nz + 1, nbin))
+    for it in range(max_iter):
+        af, sf, dd, du2 = O.compute_flux(tau, np.ones(nbin), ymix, np.array([0]),
+                                         np.full((1, nbin), 1.0 - w0), np.array([1]),
+                                         np.full((1, nbin), w0), bins, mu, 0.5, ag0, synth.HC, du)
+        done = np.array_equal(du2, du)
+        du = du2
+        if done:
+            return af, sf, dd, du
+    raise AssertionError("the lagged two-stream did not reach its fixed point")
+
+
+def _stack(dtaus, nbin=2):
+    """tau (nz+1, nbin) of layers with the given optical thicknesses (bottom first)."""
+    d = np.asarray(dtaus, dtype=np.float64)
+    tau = np.zeros((d.size + 1, nbin))
+    tau[:-1] = np.cumsum(d[::-1])[::-1][:, None]
+    return tau
+
+
+@pytest.mark.parametrize("ag0", [0.0, 0.3])
+def test_flux_layer_subdivision_invariance(ag0):
+    """The two-stream layer solution (photo.py:167-236) is the exact solution
+    of a homogeneous layer, so splitting one layer of optical depth 0.8 into
+    2 or 5 equal sublayers leaves the emerging diffuse fluxes unchanged at
+    the lag's fixed point.  A halved source term, a swapped g_p / g_m or a
+    wrong dflux index in either sweep changes them."""
+    out = []
+    for n in (1, 2, 5):
+        af, sf, dd, du = _lagged_fixed_point(_stack([0.8 / n] * n), 0.75, ag0)
+        out.append((du[-1], dd[0]))
+    for up, down in out[1:]:
+        np.testing.assert_allclose(up, out[0][0], rtol=1e-13)
+        np.testing.assert_allclose(down, out[0][1], rtol=1e-13)
+
+
+@pytest.mark.parametrize("ag0", [0.0, 0.3])
+def test_flux_conservative_scattering_keeps_net_flux(ag0):
+    """w0 = 1 - 1e-8 (the reference's clip, photo.py:167-169) over 24
+    layers of total optical depth ~6: at the fixed point the net downward
+    flux (direct mu + diffuse down - diffuse up) is the same at every
+    level; the only loss is the 1e-8 absorption per scattering (~100
+    scatterings here), so it stays within 2e-6 of its top value."""
+    d = np.random.default_rng(1).uniform(0.01, 0.5, 24)
+    af, sf, dd, du = _lagged_fixed_point(_stack(d), 1.0 - 1e-8, ag0)
+    net = sf * 0.7 + dd - du
+    spread = (net.max(axis=0) - net.min(axis=0)) / net.max(axis=0)
+    assert np.all(spread < 2e-6), spread
+    assert np.all(du[-1] > 0.1) and np.all(dd[0] > 0.0)   # real multiple scattering
+
+
+@pytest.mark.parametrize("ag0", [0.0, 0.3])
+def test_actinic_flux_energy_balance(ag0):
+    """What a layer absorbs is (1 - w0) dtau times its actinic flux (the
+    direct beam's flux plus the diffuse fluxes over edd, photo.py:238-244):
+    the drop of the net flux across each of 200 layers of dtau = 0.005 at
+    w0 = 0.6 equals it to the trapezoid error of the layer average (1e-6
+    measured; a halved diffuse term misses by 20 %)."""
+    dtl, w0 = 0.005, 0.6
+    bins = np.array([150.0, 250.0])
+    af, sf, dd, du = _lagged_fixed_point(_stack([dtl] * 200), w0, ag0, bins=bins)
+    net = sf * 0.7 + dd - du
+    absorbed = (1.0 - w0) * dtl * af * (synth.HC / bins[None, :])
+    np.testing.assert_allclose(net[1:] - net[:-1], absorbed, rtol=1e-5)
+
+
+def test_rel_change_known_answer():
+    """_f_dy (outer_loop.py:2163-2184): max |y - y_prev| / y over cells with
+    ymix >= mtol, y >= atol and y > 0, worked by hand."""
+    y = np.array([[10.0, 0.05, 5.0], [2.0, 20.0, 0.0]])
+    y_prev = np.array([[8.0, 0.04, 5.0], [3.0, 10.0, 1.0]])
+    ymix = np.array([[0.5, 0.5, 0.5], [1e-30, 0.5, 0.5]])
+    # (0,0): 0.2; (0,1): y < atol; (0,2): 0; (1,0): ymix < mtol; (1,1): 0.5; (1,2): y = 0
+    assert O.rel_change(y, y_prev, ymix, 1e-22, 0.1) == 0.5
+    ymix2 = ymix.copy()
+    ymix2[1, 1] = 1e-25
+    assert O.rel_change(y, y_prev, ymix2, 1e-22, 0.1) == 0.2
+    assert O.rel_change(y, y_prev, ymix, 1e-22, 100.0) == 0.0
+    assert O.rel_change(-y, y_prev, ymix, 1e-22, -1e9) == 0.0   # no positive cell
+
+
+def _t_case(seed=3, nz=6, ni=5, nbin=7):
+    rng = np.random.default_rng(seed)
+    y = rng.uniform(1e9, 1e12, (nz, ni))
+    ymix = y / y.sum(1, keepdims=True)
+    dz = rng.uniform(1e4, 1e6, nz)
+    cross = rng.uniform(1e-21, 1e-18, (ni, nbin))
+    scat = rng.uniform(1e-23, 1e-20, (1, nbin))
+    cz = rng.uniform(0.2, 3.0, nz)           # layer factor of the T-dependent absorbers
+    return y, ymix, dz, cross, scat, cz
+
+
+def test_tau_T_dependent_absorbers_reduce_to_scaled_densities():
+    """photo.py:107-111: with separable T-dependent cross sections
+    sigma_s(lambda) c_z, tau equals the plain absorber path with species s's
+    density scaled by c_z in layer z; a species/layer index swap or a dropped
+    dz fails it."""
+    y, ymix, dz, cross, scat, cz = _t_case()
+    Tidx = np.array([1, 3])
+    crossT = cross[Tidx][:, None, :] * cz[None, :, None]         # (n_T, nz, nbin)
+    tau = O.compute_tau(y, dz, np.array([0]), cross[[0]], np.array([4]), scat,
+                        absp_T_idx=Tidx, absp_T_cross=crossT)
+    ys = y.copy()
+    ys[:, Tidx] *= cz[:, None]
+    ref = O.compute_tau(ys, dz, np.array([0, 1, 3]), cross[[0, 1, 3]], np.array([4]), scat)
+    np.testing.assert_allclose(tau, ref, rtol=1e-13)
+    assert not np.allclose(tau, O.compute_tau(y, dz, np.array([0, 1, 3]), cross[[0, 1, 3]],
+                                              np.array([4]), scat), rtol=1e-6)
+
+
+@pytest.mark.parametrize("ag0", [0.0, 0.3])
+def test_flux_T_dependent_absorbers_reduce_to_scaled_mixing_ratios(ag0):
+    """photo.py:156-160: the T-dependent absorbers enter w0 through the
+    mixing ratios; separable cross sections equal scaled mixing ratios on
+    the plain path (tau given, so only w0 differs)."""
+    y, ymix, dz, cross, scat, cz = _t_case(seed=4)
+    nz, nbin = y.shape[0], cross.shape[1]
+    Tidx = np.array([2])
+    crossT = cross[Tidx][:, None, :] * cz[None, :, None]
+    tau = O.compute_tau(y, dz, np.array([0, 2]), cross[[0, 2]], np.array([4]), scat)
+    bins = np.linspace(100.0, 300.0, nbin)
+    args = (np.full(nbin, 2.0), None, np.array([0]), cross[[0]], np.array([4]), scat * 1e2, bins,
+            0.6, 0.5, ag0, synth.HC, np.zeros((nz + 1, nbin)))
+    a = O.compute_flux(tau, args[0], ymix, *args[2:], absp_T_idx=Tidx, absp_T_cross=crossT)
+    ms = ymix.copy()
+    ms[:, 2] *= cz
+    b = O.compute_flux(tau, args[0], ms, np.array([0, 2]), cross[[0, 2]], *args[4:])
+    for u, v in zip(a, b):
+        np.testing.assert_allclose(u, v, rtol=1e-12, atol=0.0)
+
+
+def test_J_T_dependent_branches_reduce_per_layer():
+    """photo.py:334-347: J of a T-dependent branch at layer z is the plain
+    split trapezoid against that layer's cross section."""
+    rng = np.random.default_rng(5)
+    nz, nbin, nbr = 4, 9, 3
+    aflux = rng.uniform(0.1, 5.0, (nz, nbin))
+    cT = rng.uniform(0.0, 1.0, (nbr, nz, nbin))
+    JT = O.compute_J_T(aflux, cT, 4, 0.3, 1.1)
+    for z in range(nz):
+        np.testing.assert_allclose(JT[:, z], O.compute_J(aflux[z:z + 1], cT[:, z, :], 4, 0.3,
+                                                         1.1)[:, 0], rtol=1e-14)
+
+
 # ---------------------------------------------------------------------------
 # adjoint
 # ---------------------------------------------------------------------------
