@@ -268,6 +268,14 @@ typedef struct {
    * count_max; needs the ring and conv fields of vk_ss_state. */
   int32_t conv_mode, conv_step, conv_stall_window;
   double st_factor, yconv_cri, slope_cri, yconv_min, mtol_conv, runtime;
+  /* photolysis cadence switch (outer_loop.py:716-724, 1040-1049): with
+   * photo_frq_final > 0 and vk_ss_state.photo_cad, a column starts at
+   * photo_frq (the reference's ini_update_photo_frq) and moves to
+   * photo_frq_final once longdy < photo_sw_longdy and longdydt <
+   * photo_sw_longdydt (the carried values of conv_mode 1; under conv_mode 0
+   * the last accepted step's relative change and it over dt) */
+  int32_t photo_frq_final;
+  double photo_sw_longdy, photo_sw_longdydt;
 } vk_ss_params;
 
 /* Device state of a batch of runs.  State-shaped arrays at off_s; compo is
@@ -305,6 +313,9 @@ typedef struct {
   const int64_t* off_hp;
   const double* conv_ign;
   const int64_t* off_ign;
+  /* (ncols) current photolysis cadence of each column, or NULL (fixed
+   * photo_frq); vk_ss_init sets it to photo_frq */
+  int32_t* photo_cad;
 } vk_ss_state;
 #define VK_PIN_CHARGE 1  /* e = -sum_i charge_i y_i in every layer */
 #define VK_PIN_ALL_BOT 2 /* bottom layer = the bottom densities */
@@ -362,13 +373,21 @@ int vk_atm_refresh_f64(vk_batch b, int mode, const double* ymix, const double* y
  * off_k (S, nz).  Replaces compute_tau_jax (photo.py:84-123),
  * compute_flux_jax (photo.py:126-245; ag0 = 0 and ag0 != 0 branches),
  * compute_J_jax_flat (photo.py:309-351, 377-392) and update_k_with_J
- * (photo.py:455-478); no temperature-dependent cross sections. */
+ * (photo.py:455-478).  The J contraction (n_br x nbin) . (nbin x nz) runs on
+ * FP64 MFMA.  Temperature-dependent cross sections (photo.py:107-111,
+ * 156-160, 334-347) are per column, at off_l[c] * n * nbin: absp_T_cross
+ * (n_absT, nz, nbin) for the absorbers absp_T_idx, cross_J_T (n_brT, nz,
+ * nbin) for the branches whose J_T (n_brT, nz, at off_l[c] * n_brT in the
+ * state) fills the k rows branch_T_re_idx where branch_T_active. */
 typedef struct {
   int32_t nbin, n_abs, n_scat, n_br, din12, ag0_zero;
   double dbin1, dbin2, mu_zenith, edd, ag0, hc, f_diurnal, flux_atol;
   const double *bins, *sflux_top, *absp_cross, *scat_cross, *cross_J, *dz;
   const int32_t *absp_idx, *scat_idx, *branch_re_idx, *branch_active;
   const int64_t *off_p, *off_l;
+  int32_t n_absT, n_brT;
+  const double *absp_T_cross, *cross_J_T;
+  const int32_t *absp_T_idx, *branch_T_re_idx, *branch_T_active;
 } vk_photo_setup;
 
 /* due (may be NULL): per-column flag, columns with 0 are skipped; status
@@ -377,6 +396,7 @@ typedef struct {
 typedef struct {
   double *tau, *w0, *sflux, *dflux_d, *dflux_u, *aflux, *J, *aflux_change;
   const int32_t *due, *status;
+  double* J_T; /* (n_brT, nz) per column; may be NULL when n_brT == 0 */
 } vk_photo_state;
 
 /* parts: 1 = optics (tau, w0 from y), 2 = flux sweeps, 4 = J (+ k rows when

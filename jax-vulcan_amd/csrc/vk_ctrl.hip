@@ -358,9 +358,26 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
     if (s.refresh_due)
       s.refresh_due[c] = (do_accept && p.update_frq > 0 &&
                           ((acc - 1) % p.update_frq) == 0) ? 1 : 0;
-    // photolysis due before the next attempt (outer_loop.py:253-257)
-    if (s.photo_due) s.photo_due[c] = (s.retry[c] == 0 && p.photo_frq > 0 &&
-                                       (acc % p.photo_frq) == 0) ? 1 : 0;
+    // photolysis cadence: the ini -> final switch on the carried longdy and
+    // longdydt (outer_loop.py:1040-1049); under conv_mode 0 the last
+    // accepted step's relative change stands in for longdy
+    int frq = s.photo_cad ? s.photo_cad[c] : p.photo_frq;
+    if (s.photo_cad && p.photo_frq_final > 0 && frq != p.photo_frq_final) {
+      double ld, ldt;
+      if (p.conv_mode == 1) {
+        ld = s.conv[(int64_t)c * 4];
+        ldt = s.conv[(int64_t)c * 4 + 1];
+      } else {
+        ld = s.rel[c];
+        ldt = s.rel[c] / dt;
+      }
+      if (ld < p.photo_sw_longdy && ldt < p.photo_sw_longdydt) {
+        frq = p.photo_frq_final;
+        s.photo_cad[c] = frq;
+      }
+    }
+    // photolysis due before the next attempt (outer_loop.py:253-257, 716-724)
+    if (s.photo_due) s.photo_due[c] = (s.retry[c] == 0 && frq > 0 && (acc % frq) == 0) ? 1 : 0;
   }
 }
 
@@ -400,9 +417,10 @@ __global__ __launch_bounds__(NT) void k_ss_init(const VkCol* __restrict__ cols, 
     for (int64_t e = tid; e < n; e += NT) ry[e] = 0.0;
     for (int i = tid; i < p.conv_step; i += NT) s.ring_t[(int64_t)c * p.conv_step + i] = 0.0;
     if (tid == 0) {
+      // longdy and longdydt start at 1 (state.py:706-707)
       double* cv = s.conv + (int64_t)c * 4;
-      cv[0] = 0.0;
-      cv[1] = 0.0;
+      cv[0] = 1.0;
+      cv[1] = 1.0;
       cv[2] = INFINITY;
       cv[3] = 0.0;
     }
@@ -416,7 +434,7 @@ __global__ __launch_bounds__(NT) void k_ss_init(const VkCol* __restrict__ cols, 
     }
     s.t[c] = 0.0;
     s.delta[c] = 0.0;
-    s.rel[c] = 0.0;
+    s.rel[c] = 1.0;   // the relative change before any accepted step, as longdy (state.py:706)
     s.accept[c] = 0;
     s.retry[c] = 0;
     s.nega[c] = 0;
@@ -425,6 +443,7 @@ __global__ __launch_bounds__(NT) void k_ss_init(const VkCol* __restrict__ cols, 
     s.attempts[c] = 0;
     s.status[c] = 0;
     if (s.photo_due) s.photo_due[c] = p.photo_frq > 0 ? 1 : 0;
+    if (s.photo_cad) s.photo_cad[c] = p.photo_frq;
   }
 }
 

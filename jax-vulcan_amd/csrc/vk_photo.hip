@@ -12,8 +12,16 @@
 // k_photo_flux: one thread per bin; the down sweep uses the PREVIOUS call's
 //   dflux_u (photo.py:141-145, 210-217); the up sweep recomputes the layer
 //   coefficients from tau and w0 and forms the actinic flux.
-// k_photo_J: one workgroup per (layer, column); J per branch as the split
-//   trapezoid, written to J and, times f_diurnal, into the k rows.
+// k_photo_J: the contraction J = cross_J (n_br x nbin) . aflux^T (nbin x nz)
+//   on FP64 MFMA, one workgroup per column: each wave owns 16-layer tiles and
+//   computes every 16-branch tile of a 64-branch group over all bins.  The
+//   two trapezoid spacings ride in the flux operand (af * dbin), so one
+//   accumulator per tile holds both partial sums; the half end-point terms
+//   are subtracted after.  J goes to the state and, times f_diurnal, into
+//   the k rows.
+// k_photo_JT: the temperature-dependent branches, whose cross section
+//   differs per layer (a dot product per layer, not a contraction): one
+//   workgroup per (layer, column) as the split trapezoid.
 #include "vk_common.h"
 
 namespace {
@@ -60,6 +68,7 @@ __global__ __launch_bounds__(NT) void k_photo_optics(const VkCol* __restrict__ c
                                                      const double* y) {
   __shared__ double dens_a[MAX_SP];
   __shared__ double dens_s[MAX_SP];
+  __shared__ double dens_t[MAX_SP];
   const int c = blockIdx.y;
   if (c >= ncols) return;
   if (S.due && !S.due[c]) return;
@@ -73,6 +82,7 @@ __global__ __launch_bounds__(NT) void k_photo_optics(const VkCol* __restrict__ c
   const double* dz = P.dz + P.off_l[c];
   double* tau = S.tau + op;
   double* w0a = S.w0 + op;
+  const double* crossT = P.absp_T_cross + P.off_l[c] * P.n_absT * nbin;
   // ---- optical depth (top-down cumulative) and single-scattering albedo ----
   double acc = 0.0;
   if (act) tau[(int64_t)nz * nbin + b] = 0.0;
@@ -80,18 +90,26 @@ __global__ __launch_bounds__(NT) void k_photo_optics(const VkCol* __restrict__ c
     __syncthreads();
     for (int s = threadIdx.x; s < P.n_abs; s += NT) dens_a[s] = yc[(int64_t)z * ni + P.absp_idx[s]];
     for (int s = threadIdx.x; s < P.n_scat; s += NT) dens_s[s] = yc[(int64_t)z * ni + P.scat_idx[s]];
+    for (int s = threadIdx.x; s < P.n_absT; s += NT) dens_t[s] = yc[(int64_t)z * ni + P.absp_T_idx[s]];
     __syncthreads();
     if (act) {
-      double sa = 0.0, ss = 0.0;
+      double sa = 0.0, ss = 0.0, st = 0.0;
       for (int s = 0; s < P.n_abs; ++s) sa = fma(dens_a[s], P.absp_cross[(int64_t)s * nbin + b], sa);
+      // T-dependent absorbers: cross section of this layer (photo.py:107-111)
+      for (int s = 0; s < P.n_absT; ++s)
+        st = fma(dens_t[s], crossT[((int64_t)s * nz + z) * nbin + b], st);
       for (int s = 0; s < P.n_scat; ++s) ss = fma(dens_s[s], P.scat_cross[(int64_t)s * nbin + b], ss);
-      // photo.py:100-122: (sum_abs * dz) + (sum_scat * dz), cumulated from the top
-      const double layer = sa * dz[z] + ss * dz[z];
+      // photo.py:100-122: ((sum_abs * dz) + (sum_absT * dz)) + (sum_scat * dz),
+      // cumulated from the top
+      double layer = sa * dz[z];
+      if (P.n_absT > 0) layer = layer + st * dz[z];
+      layer = layer + ss * dz[z];
       acc = acc + layer;
       tau[(int64_t)z * nbin + b] = acc;
       // photo.py:153-169: w0 from mixing ratios; the ratio is scale-free so
       // the number densities of the layer give it directly
-      double w0 = ss / fmax(sa + ss, 1e-300);
+      const double tot_abs = P.n_absT > 0 ? sa + st : sa;
+      double w0 = ss / fmax(tot_abs + ss, 1e-300);
       if (isnan(w0)) w0 = 0.0;
       w0 = fmin(w0, 1.0 - 1e-8);
       w0a[(int64_t)z * nbin + b] = w0;
@@ -178,22 +196,119 @@ __global__ __launch_bounds__(NT) void k_photo_flux(const VkCol* __restrict__ col
     atomicMax((unsigned long long*)&S.aflux_change[c], (unsigned long long)__double_as_longlong(amax));
 }
 
-// J per branch and layer (photo.py:319-330) and the k rows (photo.py:455-478).
+// J of the non-T branches on FP64 MFMA (photo.py:309-330) and the k rows
+// (photo.py:455-478).  v_mfma_f64_16x16x4f64 operands: lane (lr = lane/16,
+// lc = lane%16) holds A[row lc][k lr] and B[k lr][col lc]; result item v is
+// element (lr + 4v, lc).  Here A = cross_J rows (branches), B = the flux
+// weighted by the bin spacing, k = bins, columns = layers.
+typedef double ph_d4 __attribute__((ext_vector_type(4)));
+constexpr int J_TILES = 2;   // 16-layer tiles per wave and pass
+
 __global__ __launch_bounds__(NT) void k_photo_J(const VkCol* __restrict__ cols, int ncols,
-                                                vk_photo_setup P, vk_photo_state S, double* k) {
+                                                vk_photo_setup P, vk_photo_state S) {
+  const int c = blockIdx.x;
+  if (c >= ncols) return;
+  if (S.due && !S.due[c]) return;
+  if (S.status && (S.status[c] & VK_ST_DONE)) return;
+  const VkCol col = cols[c];
+  const int nz = col.nz, nbin = P.nbin, d12 = P.din12, n_br = P.n_br;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane >> 4, lc = lane & 15;
+  const double* af = S.aflux + P.off_p[c];
+  const double* cr = P.cross_J;
+  const int ntile = (nz + 15) / 16;
+  const int nchunk = (nbin + 3) / 4;
+  for (int m0 = 0; m0 < n_br; m0 += 64) {
+    for (int t0 = wave; t0 < ntile; t0 += 4 * J_TILES) {
+      ph_d4 acc[J_TILES][4];
+#pragma unroll
+      for (int u = 0; u < J_TILES; ++u)
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) acc[u][mt] = ph_d4{0.0, 0.0, 0.0, 0.0};
+      // this lane's rows: branches m0 + 16 mt + lc (A) and layers of each tile (B)
+      int arow[4], zrow[J_TILES];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) arow[mt] = min(m0 + 16 * mt + lc, n_br - 1);
+#pragma unroll
+      for (int u = 0; u < J_TILES; ++u) zrow[u] = min((t0 + 4 * u) * 16 + lc, nz - 1);
+#pragma unroll 1
+      for (int ch = 0; ch < nchunk; ++ch) {
+        const int bb = 4 * ch + lr;
+        const bool inb = bb < nbin;
+        const int b = inb ? bb : nbin - 1;
+        const double wgt = inb ? (b < d12 ? P.dbin1 : P.dbin2) : 0.0;
+        double av[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+          const bool ok = m0 + 16 * mt + lc < n_br;
+          av[mt] = ok ? cr[(int64_t)arow[mt] * nbin + b] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < J_TILES; ++u) {
+          const bool okz = (t0 + 4 * u) * 16 + lc < nz;
+          const double bw = okz ? af[(int64_t)zrow[u] * nbin + b] * wgt : 0.0;
+#pragma unroll
+          for (int mt = 0; mt < 4; ++mt)
+            acc[u][mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bw, acc[u][mt], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < J_TILES; ++u) {
+        const int z = (t0 + 4 * u) * 16 + lc;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int br = m0 + 16 * mt + lr + 4 * v;
+            if (z < nz && br < n_br) S.J[P.off_l[c] * n_br + (int64_t)br * nz + z] = acc[u][mt][v];
+          }
+      }
+    }
+  }
+}
+
+// The half end-point terms of the two trapezoids (photo.py:319-330) taken
+// off the MFMA sums, and the k rows (photo.py:455-478); a thread per
+// (branch, layer) of a column.
+__global__ __launch_bounds__(NT) void k_photo_J_ends(const VkCol* __restrict__ cols, int ncols,
+                                                     vk_photo_setup P, vk_photo_state S,
+                                                     double* k) {
+  const int c = blockIdx.y;
+  if (c >= ncols) return;
+  if (S.due && !S.due[c]) return;
+  if (S.status && (S.status[c] & VK_ST_DONE)) return;
+  const VkCol col = cols[c];
+  const int nz = col.nz, nbin = P.nbin, d12 = P.din12;
+  const int e = blockIdx.x * NT + threadIdx.x;
+  if (e >= P.n_br * nz) return;
+  const int br = e / nz, z = e - br * nz;
+  const double* afz = S.aflux + P.off_p[c] + (int64_t)z * nbin;
+  const double* crb = P.cross_J + (int64_t)br * nbin;
+  const double e1 = 0.5 * (afz[0] * crb[0] + afz[d12 - 1] * crb[d12 - 1]) * P.dbin1;
+  const double e2 = 0.5 * (afz[d12] * crb[d12] + afz[nbin - 1] * crb[nbin - 1]) * P.dbin2;
+  double* Jp = S.J + P.off_l[c] * P.n_br + e;
+  const double J = (*Jp - e1) - e2;
+  *Jp = J;
+  if (k && P.branch_active[br])
+    k[col.off_k + (int64_t)P.branch_re_idx[br] * nz + z] = J * P.f_diurnal;
+}
+
+// J of the temperature-dependent branches (photo.py:334-347): per (layer,
+// column) the split trapezoid against this layer's cross section.
+__global__ __launch_bounds__(NT) void k_photo_JT(const VkCol* __restrict__ cols, int ncols,
+                                                 vk_photo_setup P, vk_photo_state S, double* k) {
   __shared__ double red[2][NT / 64];
-  __shared__ double ends[4];
   const int c = blockIdx.y;
   const int z = blockIdx.x;
   if (c >= ncols) return;
   if (S.due && !S.due[c]) return;
   if (S.status && (S.status[c] & VK_ST_DONE)) return;
   const VkCol col = cols[c];
-  if (z >= col.nz) return;
+  const int nz = col.nz;
+  if (z >= nz) return;
   const int nbin = P.nbin, d12 = P.din12, tid = threadIdx.x;
   const double* af = S.aflux + P.off_p[c] + (int64_t)z * nbin;
-  for (int br = 0; br < P.n_br; ++br) {
-    const double* cr = P.cross_J + (int64_t)br * nbin;
+  for (int br = 0; br < P.n_brT; ++br) {
+    const double* cr = P.cross_J_T + (P.off_l[c] * P.n_brT + (int64_t)br * nz + z) * nbin;
     double s1 = 0.0, s2 = 0.0;
     for (int b = tid; b < nbin; b += NT) {
       const double v = af[b] * cr[b];
@@ -212,11 +327,11 @@ __global__ __launch_bounds__(NT) void k_photo_J(const VkCol* __restrict__ cols, 
       const double J1 = d1 * P.dbin1 - 0.5 * (af[0] * cr[0] + af[d12 - 1] * cr[d12 - 1]) * P.dbin1;
       const double J2 = d2 * P.dbin2 - 0.5 * (af[d12] * cr[d12] + af[nbin - 1] * cr[nbin - 1]) * P.dbin2;
       const double J = J1 + J2;
-      S.J[P.off_l[c] * P.n_br + (int64_t)br * col.nz + z] = J;
-      if (k && P.branch_active[br]) k[col.off_k + (int64_t)P.branch_re_idx[br] * col.nz + z] = J * P.f_diurnal;
+      S.J_T[P.off_l[c] * P.n_brT + (int64_t)br * nz + z] = J;
+      if (k && P.branch_T_active[br])
+        k[col.off_k + (int64_t)P.branch_T_re_idx[br] * nz + z] = J * P.f_diurnal;
     }
   }
-  (void)ends;
 }
 
 __global__ void k_photo_reset_change(int ncols, vk_photo_state S) {
@@ -229,9 +344,15 @@ __global__ void k_photo_reset_change(int ncols, vk_photo_state S) {
 int vk_launch_photo(int parts, const VkCol* cols, int ncols, int max_nz,
                     const vk_photo_setup& P, const vk_photo_state& S, const double* y, double* k,
                     hipStream_t st) {
-  if (P.n_abs > MAX_SP || P.n_scat > MAX_SP) {
-    vk_set_error("photolysis: at most %d absorbers and %d scatterers", MAX_SP, MAX_SP);
+  if (P.n_abs > MAX_SP || P.n_scat > MAX_SP || P.n_absT > MAX_SP) {
+    vk_set_error("photolysis: at most %d absorbers, %d T-dependent absorbers and %d scatterers",
+                 MAX_SP, MAX_SP, MAX_SP);
     return VK_ERR_UNSUPPORTED;
+  }
+  if (P.n_absT < 0 || P.n_brT < 0 || (P.n_absT > 0 && (!P.absp_T_cross || !P.absp_T_idx)) ||
+      (P.n_brT > 0 && (!P.cross_J_T || !P.branch_T_re_idx || !P.branch_T_active || !S.J_T))) {
+    vk_set_error("photolysis: inconsistent temperature-dependent cross sections");
+    return VK_ERR_ARG;
   }
   if (P.din12 < 1 || P.din12 >= P.nbin) {
     vk_set_error("photolysis: din12_indx=%d must lie in 1..nbin-1 (nbin=%d)", P.din12, P.nbin);
@@ -249,8 +370,17 @@ int vk_launch_photo(int parts, const VkCol* cols, int ncols, int max_nz,
     VK_CHECK_HIP(hipGetLastError());
   }
   if (parts & 4) {
-    hipLaunchKernelGGL(k_photo_J, dim3(max_nz, ncols), dim3(NT), 0, st, cols, ncols, P, S, k);
-    VK_CHECK_HIP(hipGetLastError());
+    if (P.n_br > 0) {
+      hipLaunchKernelGGL(k_photo_J, dim3(ncols), dim3(NT), 0, st, cols, ncols, P, S);
+      VK_CHECK_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_photo_J_ends, dim3((P.n_br * max_nz + NT - 1) / NT, ncols), dim3(NT), 0,
+                         st, cols, ncols, P, S, k);
+      VK_CHECK_HIP(hipGetLastError());
+    }
+    if (P.n_brT > 0) {
+      hipLaunchKernelGGL(k_photo_JT, dim3(max_nz, ncols), dim3(NT), 0, st, cols, ncols, P, S, k);
+      VK_CHECK_HIP(hipGetLastError());
+    }
   }
   return VK_OK;
 }

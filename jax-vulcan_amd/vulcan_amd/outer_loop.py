@@ -36,7 +36,9 @@ class SsParams(C.Structure):
             "batch_max_retries", "count_min", "count_max", "na", "zero_bot_row", "photo_frq",
             "update_frq", "conv_mode", "conv_step", "conv_stall_window")] + [
                 (n, C.c_double) for n in ("st_factor", "yconv_cri", "slope_cri", "yconv_min",
-                                          "mtol_conv", "runtime")]
+                                          "mtol_conv", "runtime")] + [
+                    ("photo_frq_final", C.c_int32), ("photo_sw_longdy", C.c_double),
+                    ("photo_sw_longdydt", C.c_double)]
 
 
 class SsState(C.Structure):
@@ -44,7 +46,8 @@ class SsState(C.Structure):
         "y", "y_prev", "ymix", "sol", "delta_arr", "compo", "atom_ini", "atom_loss",
         "atom_loss_prev", "dt", "t", "delta", "rel", "accept", "retry", "nega", "loss",
         "dcount", "status", "attempts", "photo_due", "aflux_change", "refresh_due", "pin",
-        "off_pin", "ring_y", "ring_t", "conv", "hp", "off_hp", "conv_ign", "off_ign")]
+        "off_pin", "ring_y", "ring_t", "conv", "hp", "off_hp", "conv_ign", "off_ign",
+        "photo_cad")]
 
 
 PIN_CHARGE, PIN_ALL_BOT, PIN_SP_BOT, PIN_H2HE = 1, 2, 4, 8
@@ -134,7 +137,10 @@ class RunConfig:
     count_max: int = 5000
     conv_tol: float = 1e-4
     flux_cri: float = 0.1
-    photo_frq: int = 5
+    photo_frq: int = 5           # the reference's ini_update_photo_frq (its default: 100)
+    photo_frq_final: int = 0     # final_update_photo_frq (default 5); 0: no switch
+    photo_switch_longdy: float = -1.0    # < 0: yconv_min * 10 (outer_loop.py:1327-1330)
+    photo_switch_longdydt: float = 1e-6
     update_frq: int = 0          # atmosphere refresh cadence (vulcan_cfg.update_frq); 0: off
     conv_mode: str = "relmax"
     conv_step: int = 500         # vulcan_cfg.py:120-122, 131-147
@@ -165,6 +171,10 @@ class RunConfig:
         p.conv_stall_window = int(self.conv_stall_window)
         for f in ("st_factor", "yconv_cri", "slope_cri", "yconv_min", "mtol_conv", "runtime"):
             setattr(p, f, float(getattr(self, f)))
+        p.photo_frq_final = int(self.photo_frq_final)
+        p.photo_sw_longdy = float(self.photo_switch_longdy if self.photo_switch_longdy >= 0.0
+                                  else 10.0 * self.yconv_min)
+        p.photo_sw_longdydt = float(self.photo_switch_longdydt)
         return p
 
 
@@ -219,7 +229,7 @@ class SteadyStateRunner:
         self.t, self.delta, self.rel = zc(), zc(), zc()
         zi = lambda: torch.zeros(n, dtype=torch.int32, device=dev)
         (self.accept, self.retry, self.nega, self.loss, self.dcount, self.status,
-         self.attempts, self.photo_due) = (zi() for _ in range(8))
+         self.attempts, self.photo_due, self.photo_cad) = (zi() for _ in range(9))
         zero_bot = any(bool(c.atm.use_botflux) for c in cols)
         self.params = cfg.c_params(self.na, zero_bot)
         _lib.check(_lib.load().vk_batch_set_latency_mode(b.handle, int(bool(latency_mode))),
@@ -268,7 +278,8 @@ class SteadyStateRunner:
             else self.photo.state.aflux_change.data_ptr(),
             None if self.refresh is None else self.refresh.due.data_ptr(),
             None if self.pin is None else self.pin.data_ptr(),
-            None if self.off_pin is None else self.off_pin.data_ptr(), *self._conv_ptrs)
+            None if self.off_pin is None else self.off_pin.data_ptr(), *self._conv_ptrs,
+            self.photo_cad.data_ptr())
         self.graph = None
         self._init()
 
@@ -432,7 +443,7 @@ class SteadyStateRunner:
     # outer_loop.py:1795-2013)
     _STATE = ("y", "y_prev", "ymix", "sol", "delta_arr", "atom_ini", "atom_loss",
               "atom_loss_prev", "dt", "t", "delta", "rel", "accept", "retry", "nega", "loss",
-              "dcount", "status", "attempts", "photo_due", "k", "atm")
+              "dcount", "status", "attempts", "photo_due", "photo_cad", "k", "atm")
 
     def snapshot(self) -> dict:
         """Every device array the next attempt reads, copied to the host

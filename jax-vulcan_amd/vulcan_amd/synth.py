@@ -73,6 +73,11 @@ class PhotoSetup:
     ag0: float = 0.0
     hc: float = HC
     f_diurnal: float = 1.0
+    # temperature-dependent absorbers and branches (photo.py:107-111, 334-347);
+    # their per-layer cross sections are per column (Column.photo_T)
+    absp_T_idx: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+    branch_T_re_idx: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+    branch_T_active: np.ndarray = field(default_factory=lambda: np.zeros(0, bool))
 
 
 @dataclass
@@ -90,6 +95,7 @@ class Column:
     dz: np.ndarray            # (nz,)
     photo: Optional[PhotoSetup] = None
     meta: dict = field(default_factory=dict)
+    photo_T: Optional[dict] = None   # absp_T_cross (n_T, nz, nbin), cross_J_T (n_brT, nz, nbin)
 
 
 # --------------------------------------------------------------------------
@@ -394,6 +400,32 @@ def random_photo(rng, ni, parents, rows, nbin=2500):
         cross_J=cross_J, branch_re_idx=rows.astype(np.int64),
         branch_active=np.ones(n_br, dtype=bool),
     )
+
+
+def with_t_dependent_photo(col: "Column", seed: int, n_abs_T: int = 2, n_br_T: int = 3):
+    """The column with n_abs_T of its absorbers and n_br_T of its photolysis
+    branches made temperature-dependent: sigma(lambda) (T_z / 1000 K)^a with
+    a in U[-1.5, 1.5] per species or branch.  A T-dependent branch takes over
+    the k row of the plain branch it replaces (which is deactivated)."""
+    import dataclasses
+    rng = np.random.default_rng(seed)
+    ph = col.photo
+    f = col.atm.Tco / 1000.0
+    sel_a = np.sort(rng.choice(ph.absp_idx.size, size=n_abs_T, replace=False))
+    keep = np.setdiff1d(np.arange(ph.absp_idx.size), sel_a)
+    ea = rng.uniform(-1.5, 1.5, n_abs_T)
+    absp_T_cross = ph.absp_cross[sel_a][:, None, :] * (f[None, :, None] ** ea[:, None, None])
+    sel_b = np.sort(rng.choice(ph.cross_J.shape[0], size=n_br_T, replace=False))
+    eb = rng.uniform(-1.5, 1.5, n_br_T)
+    cross_J_T = ph.cross_J[sel_b][:, None, :] * (f[None, :, None] ** eb[:, None, None])
+    active = ph.branch_active.copy()
+    active[sel_b] = False
+    ph2 = dataclasses.replace(
+        ph, absp_idx=ph.absp_idx[keep], absp_cross=ph.absp_cross[keep],
+        absp_T_idx=ph.absp_idx[sel_a], branch_active=active,
+        branch_T_re_idx=ph.branch_re_idx[sel_b], branch_T_active=np.ones(n_br_T, bool))
+    return dataclasses.replace(col, photo=ph2, photo_T={"absp_T_cross": absp_T_cross,
+                                                       "cross_J_T": cross_J_T})
 
 
 # --------------------------------------------------------------------------

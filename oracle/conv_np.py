@@ -67,8 +67,8 @@ class ConvTracker:
         self.conv_step = int(conv_step)
         self.ring_y = np.zeros((conv_step, nz, ni))
         self.ring_t = np.zeros(conv_step)
-        self.longdy = 0.0
-        self.longdydt = 0.0
+        self.longdy = 1.0      # state.py:706-707
+        self.longdydt = 1.0
         self.seen_min = np.inf
         self.since = 0
         self.p = dict(st_factor=st_factor, yconv_cri=yconv_cri, slope_cri=slope_cri,

@@ -820,6 +820,9 @@ class RunParams(NamedTuple):
     photo_frq: int = 5
     flux_cri: float = 0.1
     flux_atol: float = 1.0
+    photo_frq_final: int = 0          # 0: no ini -> final switch
+    photo_sw_longdy: float = 1.0      # yconv_min * 10 (outer_loop.py:1327-1330)
+    photo_sw_longdydt: float = 1e-6
 
 
 class PhotoState(NamedTuple):
@@ -828,22 +831,30 @@ class PhotoState(NamedTuple):
     dflux_u: np.ndarray
     aflux: np.ndarray
     aflux_change: float = 0.0
+    tcross: dict | None = None   # the column's absp_T_cross / cross_J_T
 
 
-def photo_state(ph, dz, nz):
+def photo_state(ph, dz, nz, tcross=None):
     nbin = ph.bins.shape[0]
-    return PhotoState(ph, dz, np.zeros((nz + 1, nbin)), np.zeros((nz, nbin)), 0.0)
+    return PhotoState(ph, dz, np.zeros((nz + 1, nbin)), np.zeros((nz, nbin)), 0.0, tcross)
 
 
 def photo_update(y, ymix, k, pst: PhotoState, flux_atol=1.0):
     """tau -> flux -> J -> k rows, aflux_change (outer_loop.py:253-352)."""
     ph = pst.ph
-    tau = compute_tau(y, pst.dz, ph.absp_idx, ph.absp_cross, ph.scat_idx, ph.scat_cross)
+    tc = pst.tcross or {}
+    tidx = np.asarray(getattr(ph, "absp_T_idx", np.zeros(0, np.int64)), dtype=np.int64)
+    tkw = dict(absp_T_idx=tidx, absp_T_cross=tc.get("absp_T_cross")) if tidx.size else {}
+    tau = compute_tau(y, pst.dz, ph.absp_idx, ph.absp_cross, ph.scat_idx, ph.scat_cross, **tkw)
     af, sf, dd, du = compute_flux(tau, ph.sflux_top, ymix, ph.absp_idx, ph.absp_cross,
                                   ph.scat_idx, ph.scat_cross, ph.bins, ph.mu_zenith, ph.edd,
-                                  ph.ag0, ph.hc, pst.dflux_u)
+                                  ph.ag0, ph.hc, pst.dflux_u, **tkw)
     J = compute_J(af, ph.cross_J, ph.din12_indx, ph.dbin1, ph.dbin2)
     k = update_k_with_J(k, J, ph.branch_re_idx, ph.branch_active, ph.f_diurnal)
+    bT = np.asarray(getattr(ph, "branch_T_re_idx", np.zeros(0, np.int64)), dtype=np.int64)
+    if bT.size:
+        JT = compute_J_T(af, tc["cross_J_T"], ph.din12_indx, ph.dbin1, ph.dbin2)
+        k = update_k_with_J(k, JT, bT, np.asarray(ph.branch_T_active, bool), ph.f_diurnal)
     mask = af > flux_atol
     ratio = np.where(mask, np.abs(af - pst.aflux) / np.maximum(np.abs(af), 1e-300), 0.0)
     change = float(np.max(ratio)) if np.any(mask) else 0.0
@@ -864,14 +875,25 @@ def steady_state_run(y0, k, atm, net, dt0, p: StepParams = StepParams(),
     attempts = 0
     rel = np.inf
     status = "max_steps"
+    frq = rp.photo_frq
+    rel_carried = 1.0          # longdy's initial value (state.py:706)
+    n_photo = 0
     while True:
-        if photo is not None and s.retry_count == 0 and s.accept_count % rp.photo_frq == 0:
+        if photo is not None and s.retry_count == 0 and s.accept_count % frq == 0:
             k, photo, _ = photo_update(s.y, s.ymix, k, photo, rp.flux_atol)
+            n_photo += 1
         sol, delta = ros2_step(s.y, k, s.dt, atm, net)
         s_new, d = control_update(s, sol, delta, n0, compo, ini, p)
         attempts += 1
         if d["do_accept"]:
             rel = rel_change(s_new.y, s.y_prev, s_new.ymix, p.mtol, p.atol)
+            rel_carried = rel
+        # ini -> final photolysis cadence (outer_loop.py:1040-1049) on the
+        # carried relative change standing in for longdy, over dt for longdydt
+        if (photo is not None and rp.photo_frq_final > 0 and frq != rp.photo_frq_final
+                and rel_carried < rp.photo_sw_longdy
+                and rel_carried / s.dt < rp.photo_sw_longdydt):
+            frq = rp.photo_frq_final
         if trace is not None:
             trace.append(dict(dt=s.dt, accept=d["do_accept"], delta=s_new.delta, rel=rel))
         s = s_new
@@ -885,4 +907,5 @@ def steady_state_run(y0, k, atm, net, dt0, p: StepParams = StepParams(),
         if max_attempts is not None and attempts >= max_attempts:
             status = "max_attempts"
             break
-    return s.y, dict(state=s, attempts=attempts, rel=rel, status=status, k=k)
+    return s.y, dict(state=s, attempts=attempts, rel=rel, status=status, k=k,
+                     photo_updates=n_photo, photo_frq=frq)
