@@ -434,14 +434,16 @@ def with_t_dependent_photo(col: "Column", seed: int, n_abs_T: int = 2, n_br_T: i
 
 def make_column(seed: int, nz: int, ni: int, n_rev: int, n_photo: int = 0,
                 nbin: int = 2500, dt: float = 1e-6, name: str = "column",
-                p_bot: float = 1e9, p_top: float = 1e-2, consistent: bool = False) -> Column:
-    """Draw one column with the workload-1 distributions."""
+                p_bot: float = 1e9, p_top: float = 1e-2, consistent: bool = False,
+                ymix_lo: float = -20.0) -> Column:
+    """Draw one column with the workload-1 distributions (mixing ratios
+    log-uniform 10^ymix_lo .. 1)."""
     rng = np.random.default_rng(seed)
     P = np.logspace(np.log10(p_bot), np.log10(p_top), nz)
     Tco = smooth_temperature(rng, P)
     geo = compute_geometry(P, Tco)
     M = P / (KB * Tco)
-    ymix = 10.0 ** rng.uniform(-20.0, 0.0, size=(nz, ni))
+    ymix = 10.0 ** rng.uniform(ymix_lo, 0.0, size=(nz, ni))
     ymix /= ymix.sum(axis=1, keepdims=True)
     y = M[:, None] * ymix
     net, parents, rows = random_network(rng, ni, n_rev, n_photo)
