@@ -21,6 +21,11 @@ from pathlib import Path
 
 FACTOR_CR = 6    # vk_tune_set(VK_TUNE_FACTOR, sl, VK_FACTOR_CR): cyclic-reduction factor
 VK_TUNE_REFINE, VK_TUNE_SPLIT = 2, 3
+# The synthetic initial states put trace species at 1e-20 of the layer: at the
+# reference's default dt_min (1e-14) the error estimate exceeds rtol and the
+# force-accept rule (which keeps y_prev) freezes a column; the steady-state
+# legs run with dt_min = 1e-30 (a run-configuration value, vulcan_cfg.dt_min).
+SERVE_DT_MIN = 1e-30
 
 ROOT = Path(__file__).resolve().parent
 sys.path[:0] = [str(ROOT), str(ROOT / "jax-vulcan_amd")]
@@ -104,6 +109,70 @@ def bench_rates(B: int, net, atm, reps: int = 20, cpu_cols: int = 20, no_cpu: bo
     return res
 
 
+def bench_config2(photo_batch: int):
+    """One full steady-state solve of the nz=150, ni=93, 2461-row column
+    (independent K_eq) with two-stream photolysis over 2500 bins refreshed in
+    the loop every 5 accepted steps, on the cyclic-reduction factor; and the
+    photolysis update (tau, two-stream, J on MFMA, k rows) of photo_batch
+    such columns in one launch set."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from vulcan_amd import _lib, core, photo, synth
+    from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
+    c = synth.config2()
+    r = SteadyStateRunner([c], RunConfig(dt_min=SERVE_DT_MIN), latency_mode=True)
+    r.run(max_attempts=40, check_every=20, use_graph=False)     # warm the code paths
+    r.reset_to_initial()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r.run(max_attempts=100000, check_every=50, use_graph=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    res = r.results()
+    out = {"workload": "nz=150, ni=93, 1200 reversible (independent K_eq) + 60 photolysis rows, "
+                       "nbin=2500 two-stream refreshed every 5 accepted steps, adaptive Ros2 to "
+                       "relative change < 1e-4 or 5000 accepted steps, dt_min 1e-30",
+           "factor_kernel": "cr", "wall_s": round(wall, 3), "attempts": int(res["attempts"][0]),
+           "accepted": int(res["accept"][0]),
+           "converged": bool(res["status"][0] & 4), "capped": bool(res["status"][0] & 8),
+           "ms_per_attempt": round(wall / max(int(res["attempts"][0]), 1) * 1e3, 3),
+           "t_end_s": float(res["t"][0]), "last_rel_change": float(res["rel"][0])}
+    # batched photolysis update
+    B = photo_batch
+    b = core.uniform_batch(B, c.nz, c.net)
+    ph = photo._ph_dict(c.photo)
+    setup = photo._Setup(ph, [c.nz] * B, [c.dz] * B, branch_re_idx=ph["branch_re_idx"],
+                         branch_active=ph["branch_active"])
+    st = photo._State(setup.plen, setup.llen, setup.n_br, B)
+    cst = st.c()
+    y = torch.as_tensor(np.tile(c.y, (B, 1)), dtype=torch.float64, device="cuda").reshape(-1)
+    k = torch.as_tensor(np.tile(c.k, (B, 1)), dtype=torch.float64, device="cuda").reshape(-1)
+    lib = _lib.load()
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def upd():
+        _lib.check(lib.vk_photo_f64(b.handle, 7, C.byref(setup.c), C.byref(cst), y.data_ptr(),
+                                    k.data_ptr(), stream), "vk_photo_f64")
+    upd()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    e0.record()
+    for _ in range(reps):
+        upd()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbin, n_br, nz = setup.nbin, setup.n_br, c.nz
+    out["photo_update_batch"] = {
+        "columns": B, "ms_per_update": round(ms, 4),
+        "column_updates_per_s": round(B / (ms * 1e-3), 1),
+        "J_contraction_flop": 2 * n_br * nz * nbin * B,
+        "flux_state_bytes": 7 * (nz + 1) * nbin * 8 * B}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,8 +195,16 @@ def main() -> int:
     ap.add_argument("--serve-factor", choices=("cr", "auto"), default="cr",
                     help="block factor of the serving pools: cyclic reduction (latency mode) "
                          "or the throughput kernels (--factor)")
-    ap.add_argument("--serve-alone", type=int, default=2,
+    ap.add_argument("--serve-alone", type=int, default=8,
                     help="requests also solved one at a time, for the comparison")
+    ap.add_argument("--serve-budget-gb", type=float, default=0.0,
+                    help="HBM budget of the serving pools (0: 200 GB scaled by requests / "
+                         "10000, the full workload's budget per request)")
+    ap.add_argument("--config2", action=argparse.BooleanOptionalAction, default=True,
+                    help="time one full steady-state solve of the nz=150, ni=93 column with "
+                         "in-loop photolysis (nbin=2500), and a batched photolysis update")
+    ap.add_argument("--photo-batch", type=int, default=256,
+                    help="columns of the batched photolysis-update timing")
     ap.add_argument("--factor", choices=("auto", "tile", "rows", "blocked", "tile2", "wave", "tile8", "cr"),
                     default="auto",
                     help="block-factor kernel for the nz=150 / ni=93 bucket (auto: the "
@@ -332,28 +409,50 @@ def main() -> int:
         latency = args.serve_factor == "cr"
         if not latency and args.factor == "auto":
             autotune.tune_columns(reqs, verbose=True)
-        eng = ServingEngine(hbm_budget_bytes=200e9, cfg=RunConfig(), chunk=25,
+        budget_gb = args.serve_budget_gb or 200.0 * len(reqs) / 10000.0
+        scfg = RunConfig(dt_min=SERVE_DT_MIN)
+        eng = ServingEngine(hbm_budget_bytes=budget_gb * 1e9, cfg=scfg, chunk=25,
                             latency_mode=latency)
         t0 = time.perf_counter()
         res = eng.run(reqs, arrivals, report_every=30.0)
         wall = time.perf_counter() - t0
         summ = latency_summary(res)
+        waits = np.array([r.start - r.arrival for r in res])
         alone = []
         for i in range(min(args.serve_alone, len(reqs))):
-            r1 = SteadyStateRunner([reqs[i]], RunConfig(), latency_mode=latency)
+            r1 = SteadyStateRunner([reqs[i]], scfg, latency_mode=latency)
             t1 = time.perf_counter()
             r1.run(max_attempts=100000, check_every=25, use_graph=True, report_every=30.0)
             alone.append(time.perf_counter() - t1)
-        serving = {"factor_kernel": args.serve_factor,"requests": len(reqs), "arrival_rate_per_s": args.serve_rate,
-                   "hbm_budget_gb": 200, "slots": {str(k): v for k, v in eng.slot_counts.items()},
+        alone_rate = len(alone) / sum(alone) if alone else None
+        serving = {"factor_kernel": args.serve_factor, "requests": len(reqs),
+                   "arrival_rate_per_s": args.serve_rate, "hbm_budget_gb": round(budget_gb, 2),
+                   "slots": {str(k): v for k, v in eng.slot_counts.items()},
                    "wall_s": round(wall, 3),
                    "p50_latency_s": round(summ["p50_s"], 3), "p99_latency_s": round(summ["p99_s"], 3),
+                   "admission_wait_s": {"p50": round(float(np.percentile(waits, 50)), 3),
+                                        "p99": round(float(np.percentile(waits, 99)), 3),
+                                        "max": round(float(waits.max()), 3)},
                    "throughput_solves_per_s": round(summ["throughput_solves_per_s"], 4),
                    "converged": summ["converged"], "mean_attempts": round(summ["attempts_mean"], 1),
+                   "latency_bound": ("time to steady state" if summ["converged"] == len(res)
+                                     else f"{len(res) - summ['converged']} requests ran to the "
+                                          f"5000-step cap"),
                    "one_at_a_time_solve_s": [round(a, 3) for a in alone],
-                   "sizes": "nz~U{60..200}, ni in {50,69,93}, nr=13*ni; steady state: relative "
-                            "change < 1e-4 or 5000 accepted steps"}
+                   "one_at_a_time_solves_per_s": None if alone_rate is None else round(alone_rate, 4),
+                   "served_over_one_at_a_time": (None if not alone_rate else
+                                                 round(summ["throughput_solves_per_s"] / alone_rate, 2)),
+                   "sizes": "nz~U{60..200}, ni in {50,69,93}, nr=13*ni, consistent K_eq, dt_min "
+                            f"{SERVE_DT_MIN:g}; steady state: relative change < 1e-4 (t > 100 s, "
+                            "120 accepted steps) or 5000 accepted steps; budget 200 GB x "
+                            "requests / 10000"}
         print(f"[bench] serving: {serving}", file=sys.stderr, flush=True)
+
+    # ---- plain steady state with in-loop photolysis (one column) -------------
+    config2 = None
+    if args.config2:
+        config2 = bench_config2(args.photo_batch)
+        print(f"[bench] config 2: {config2}", file=sys.stderr, flush=True)
 
     conv_cost = None
     if args.conv_cost > 0:
@@ -410,7 +509,9 @@ def main() -> int:
     achieved = fac_flops / (fac_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-                "traffic": factor_traffic(B)}
+                "traffic": factor_traffic(B),
+                "pipe": "FP64 VALU FMA (the tile block factor issues vector FMAs; MI355X's "
+                        "FP64 vector and matrix peaks are both 78.6 TFLOP/s)"}
 
     # ---- CPU baseline: C restatement of the reference step -------------------
     cpu = None
@@ -468,6 +569,7 @@ def main() -> int:
         "singular_columns": n_singular,
         "uneven_batch": uneven,
         "serving": serving,
+        "config2_steady_state": config2,
         "stopping_rule_cost": conv_cost,
         "solve_latency": solve_latency,
         "rate_build": rate_build,
