@@ -231,8 +231,11 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
  * (solver.py:91-112).  The block-Thomas factors refine each block solve of
  * the backward sweep, x_j = X_j t_j; x_j += X_j (t_j - A'_j x_j) with A'_j
  * the Schur complement X_j inverts (one extra read of the diagonal-block
- * stack per refined sweep); the cyclic-reduction factor takes one
- * whole-column step (residual, correction solve).  0: none; 1: the stage-1 solve;
+ * stack per refined sweep); a column where some layer's correction exceeded
+ * that layer's solution (an inverse with no correct digit, cond(A'_j) of
+ * 1e12 and more) then takes one whole-column step, residual and correction
+ * solve, which the other columns skip.  The cyclic-reduction factor takes
+ * the whole-column step for every column.  0: none; 1: the stage-1 solve;
  * 2 (default): both stages.  Unrefined, a step leaves up to ~1e-9 relative
  * error on cells that the reference's LU determines to 1e-12. */
 #define VK_TUNE_REFINE 2

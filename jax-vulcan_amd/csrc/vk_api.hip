@@ -93,7 +93,8 @@ int vk_launch_cr_solve(int sl, const VkCol* cols, int ncols, int max_nz, const d
 int vk_launch_bt_solve(int sl, const VkCol* cols, int ncols, const double* inv,
                        const double* sup_d, const double* sub_d, const double* rhs,
                        double* rscratch, double* x, int mode, SolveEpi epi, const int* active,
-                       hipStream_t stream, const double* refine_diag = nullptr);
+                       hipStream_t stream, const double* refine_diag = nullptr,
+                       int* refl = nullptr);
 int vk_launch_tile_to_nat(int sl, const VkCol* cols, int ncols, int max_nz, const double* tile,
                           double* nat, hipStream_t stream);
 int vk_launch_bt_residual(int sl, const VkCol* cols, int ncols, int max_nz, const double* blk,
@@ -845,8 +846,11 @@ static int g_mol_precompute = 1;
 
 // VK_TUNE_REFINE: 0 none; 1 the stage-1 solve is refined; 2 (default) both
 // stages' solves.  The block-Thomas factors refine each block solve of the
-// backward sweep; the cyclic-reduction factor takes one whole-column
-// refinement step (residual, correction solve).
+// backward sweep, and a column where some block's correction exceeded the
+// solution (an explicit inverse with no correct digit) takes one
+// whole-column step on top; the cyclic-reduction factor takes one
+// whole-column refinement step (residual, correction solve) for every
+// column.
 static int g_refine = 2;
 
 // Offset of the inverse stack in the step workspace: the factor writes the
@@ -856,9 +860,14 @@ static size_t ros2_ws_inv_offset(vk_batch b) {
          align_up<double>(b->k_doubles);
 }
 
+// Offset of the per-column whole-column refinement flags (ints).
+static size_t ros2_ws_refl_offset(vk_batch b) {
+  return ros2_ws_inv_offset(b) + align_up<double>(b->block_doubles);
+}
+
 // Bytes before the cyclic-reduction region of the step workspace.
 static size_t ros2_ws_base_bytes(vk_batch b) {
-  return ros2_ws_inv_offset(b) + align_up<double>(b->block_doubles);
+  return ros2_ws_refl_offset(b) + align_up<int>((size_t)b->ncols);
 }
 
 // Whether bucket sl of b uses the cyclic-reduction factor.
@@ -945,6 +954,7 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
   double* inv = g_refine > 0 ? (double*)((char*)ws + ros2_ws_inv_offset(b)) : blk;
   const double* ref1 = g_refine >= 1 ? blk : nullptr;
   const double* ref2 = g_refine >= 2 ? blk : nullptr;
+  int* refl = (int*)((char*)ws + ros2_ws_refl_offset(b));
   const double r = ros2_gamma();
   {
     KernelTimer t(6, st);
@@ -969,6 +979,7 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
     const double* dtc = dt + c0;
     int32_t* stc = status ? status + c0 : nullptr;
     const int* act = active ? active + c0 : nullptr;
+    int* rfl = refl + c0;
     {
       KernelTimer t(0, s);
       if ((rc = vk_launch_chem_layer(sl, M_LHS, cols, nc, b->max_nz, b->max_S,
@@ -1009,12 +1020,26 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
         e1r.base = k2;
         rc = vk_launch_cr_solve(sl, cols, nc, b->max_nz, rmod, sup, sub, crw, k1, e1r,
                                 act, s);
-      } else if (cr)
+      } else if (cr) {
         rc = vk_launch_cr_solve(sl, cols, nc, b->max_nz, rhs_y, sup, sub, crw, k1,
                                 e1, act, s, true);
-      else
+      } else if (ref1) {
+        // the columns rfl flags: residual into rhs2 (free until the stage-2
+        // chemistry), then k1 += the correction, yk2 again
+        if ((rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, fused ? rmod : rhs_y,
+                                     rmod, k1, fused ? 1 : 0, e1, act, s, ref1, rfl)))
+          return rc;
+        if ((rc = vk_launch_bt_residual(sl, cols, nc, b->max_nz, blk, sup, sub, rhs_y,
+                                        k1, rhs2, rfl, s)))
+          return rc;
+        SolveEpi e1r = e1;
+        e1r.base = k1;
+        rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, rhs2, rmod, k1, 0, e1r, rfl, s,
+                                ref1);
+      } else {
         rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, fused ? rmod : rhs_y,
-                                rmod, k1, fused ? 1 : 0, e1, act, s, ref1);
+                                rmod, k1, fused ? 1 : 0, e1, act, s);
+      }
       if (rc) return rc;
     }
     {
@@ -1041,12 +1066,25 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
         e2r.base = rmod;
         rc = vk_launch_cr_solve(sl, cols, nc, b->max_nz, rhs_y, sup, sub, crw, k2, e2r,
                                 act, s);
-      } else if (cr)
+      } else if (cr) {
         rc = vk_launch_cr_solve(sl, cols, nc, b->max_nz, rhs2, sup, sub, crw, k2, e2,
                                 act, s);
-      else
-        rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, rhs2, rmod, k2, 0, e2,
-                                act, s, ref2);
+      } else if (ref2) {
+        // the columns rfl flags: residual into rhs_y (free after stage 1),
+        // then k2 += the correction, sol and delta again
+        if ((rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, rhs2, rmod, k2, 0, e2,
+                                     act, s, ref2, rfl)))
+          return rc;
+        if ((rc = vk_launch_bt_residual(sl, cols, nc, b->max_nz, blk, sup, sub, rhs2,
+                                        k2, rhs_y, rfl, s)))
+          return rc;
+        SolveEpi e2r = e2;
+        e2r.base = k2;
+        rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, rhs_y, rmod, k2, 0, e2r, rfl, s,
+                                ref2);
+      } else {
+        rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, rhs2, rmod, k2, 0, e2, act, s);
+      }
       if (rc) return rc;
     }
     return VK_OK;

@@ -395,8 +395,16 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(TILE_IN &
 // x_j is made in the next step down, where X_{j-1} is loaded anyway and X_j
 // is still in registers, so a refined sweep reads each inverse once and D
 // once.
+//
+// refl (REFINE 1, optional): per column, 0 when the block refinement's
+// correction of some layer exceeded that layer's solution in max norm (an
+// inverse X_j with no correct digit there, cond(A'_j) ~ 1e12 and up), else
+// VK_ST_DONE.  The host gives those columns one whole-column refinement step
+// (k_bt_residual, then a correction solve with epi.base) and passes refl as
+// the active mask of both, so the other columns skip them.
 
 __device__ inline void solve_epilogue(const SolveEpi& epi, int64_t e, double xv, double* x) {
+  if (epi.base) xv = epi.base[e] + xv;
   x[e] = xv;
   if (epi.kind == 1) {
     epi.out1[e] = epi.y[e] + xv / epi.e0;
@@ -409,17 +417,22 @@ __device__ inline void solve_epilogue(const SolveEpi& epi, int64_t e, double xv,
 }
 
 template <int SL, int REFINE>
-__global__ __launch_bounds__(VK_NT) void k_bt_solve(
+__global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ? 2 : 1))) void k_bt_solve(
     const VkCol* __restrict__ cols, int ncols, const double* __restrict__ inv,
     const double* __restrict__ sup_d, const double* __restrict__ sub_d,
-    const double* __restrict__ rhs, double* __restrict__ rscratch, double* __restrict__ x,
-    int mode, SolveEpi epi, const int* __restrict__ active, const double* __restrict__ dstk) {
+    const double* __restrict__ rhs, double* __restrict__ rscratch, double* x,
+    int mode, SolveEpi epi, const int* __restrict__ active, const double* __restrict__ dstk,
+    int* __restrict__ refl) {
   __shared__ double vec[2][VK_G * SL];
   __shared__ double rv[VK_G * SL];  // refinement: x_j, then the residual
   __shared__ double bx[VK_G * SL];  // refinement: b_{j-1} o x_j
+  __shared__ double red[VK_NT / 64][2];  // per wave: max |correction|, max |x| of a layer
   const int c = blockIdx.x;
   if (c >= ncols) return;
-  if (active && (active[c] & VK_ST_DONE)) return;
+  if (active && (active[c] & VK_ST_DONE)) {
+    if (refl && threadIdx.x == 0) refl[c] = VK_ST_DONE;
+    return;
+  }
   const VkCol col = cols[c];
   if (vk_slots(col.ni) != SL) return;
   const int nz = col.nz, ni = col.ni;
@@ -491,6 +504,7 @@ __global__ __launch_bounds__(VK_NT) void k_bt_solve(
     // holds b_j o x0_u, w (registers) x0_u.
     const double* dblk = dstk + col.off_b;
     double xp[SL][SL];
+    bool wide = false;  // thread 0: some layer's correction exceeded its solution
     for (int j = nz - 1; j >= -1; --j) {
       if (j >= 0) load_tile<SL>(a, stk + (int64_t)j * BLK);
       if (j < nz - 1) {
@@ -515,12 +529,15 @@ __global__ __launch_bounds__(VK_NT) void k_bt_solve(
         __syncthreads();
         double d[SL];
         tile_matvec<SL>(xp, rv, d);                          // X_u (t_u - A'_u x0_u)
+        double dmax = 0.0, xmax = 0.0;
         if (tc == 0) {
 #pragma unroll
           for (int s = 0; s < SL; ++s) {
             const int i = s * VK_G + tr;
             if (i < ni) {
               const double xv = w[s] + d[s];
+              dmax = fmax(dmax, fabs(d[s]));
+              xmax = fmax(xmax, fabs(xv));
               solve_epilogue(epi, col.off_s + (int64_t)u * ni + i, xv, x);
               if (j >= 0)
                 vec[j & 1][i] = rm[col.off_s + (int64_t)j * ni + i] -
@@ -530,7 +547,27 @@ __global__ __launch_bounds__(VK_NT) void k_bt_solve(
             }
           }
         }
+        if (refl) {
+          // the tc == 0 lanes of a wave are 0, 16, 32, 48
+          dmax = fmax(dmax, __shfl_xor(dmax, 16));
+          dmax = fmax(dmax, __shfl_xor(dmax, 32));
+          xmax = fmax(xmax, __shfl_xor(xmax, 16));
+          xmax = fmax(xmax, __shfl_xor(xmax, 32));
+          if ((tid & 63) == 0) {
+            red[tid >> 6][0] = dmax;
+            red[tid >> 6][1] = xmax;
+          }
+        }
         __syncthreads();
+        if (refl && tid == 0) {
+          double dm = red[0][0], xm = red[0][1];
+#pragma unroll
+          for (int v = 1; v < VK_NT / 64; ++v) {
+            dm = fmax(dm, red[v][0]);
+            xm = fmax(xm, red[v][1]);
+          }
+          wide = wide || dm > xm;
+        }
       }
       if (j < 0) break;
       tile_matvec<SL>(a, vec[j & 1], w);                     // x0_j
@@ -548,6 +585,7 @@ __global__ __launch_bounds__(VK_NT) void k_bt_solve(
 #pragma unroll
         for (int t = 0; t < SL; ++t) xp[s][t] = a[s][t];
     }
+    if (refl && tid == 0) refl[c] = wide ? 0 : VK_ST_DONE;
   }
 }
 
@@ -1284,15 +1322,15 @@ int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag
 int vk_launch_bt_solve(int sl, const VkCol* cols, int ncols, const double* inv,
                        const double* sup_d, const double* sub_d, const double* rhs,
                        double* rscratch, double* x, int mode, SolveEpi epi, const int* active,
-                       hipStream_t stream, const double* refine_diag) {
+                       hipStream_t stream, const double* refine_diag, int* refl) {
   if (refine_diag) {
     VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_bt_solve<SL_, 1>), dim3(ncols), dim3(VK_NT), 0,
                                           stream, cols, ncols, inv, sup_d, sub_d, rhs, rscratch,
-                                          x, mode, epi, active, refine_diag));
+                                          x, mode, epi, active, refine_diag, refl));
   } else {
     VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_bt_solve<SL_, 0>), dim3(ncols), dim3(VK_NT), 0,
                                           stream, cols, ncols, inv, sup_d, sub_d, rhs, rscratch,
-                                          x, mode, epi, active, nullptr));
+                                          x, mode, epi, active, nullptr, nullptr));
   }
   VK_CHECK_HIP(hipGetLastError());
   return VK_OK;

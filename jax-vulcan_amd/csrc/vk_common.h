@@ -204,8 +204,9 @@ struct SolveEpi {
   const double* yk2;
   double* out1;
   double* out2;
-  // when set (cyclic-reduction solves only): the solution is base + the
-  // solve's result, i.e. the solve computed a refinement correction
+  // when set: the solution is base + the solve's result, i.e. the solve
+  // computed a refinement correction; base may be the output x itself
+  // (each element is read, then written, by one thread)
   const double* base = nullptr;
 };
 

@@ -143,7 +143,7 @@ class SlotPool:
         sl = (cap_ni + 15) // 16
         if latency_mode or _lib.load().vk_tune_get(0, sl) == FACTOR_CR:
             ws += 7 * cap_nz * block_doubles(cap_ni)      # the cyclic-reduction levels
-        return 8 * (6 * cs + ck + ca + ws) + 8 * (3 * VK_MAX_ATOMS + 4) + 4 * 8
+        return 8 * (6 * cs + ck + ca + ws) + 8 * (3 * VK_MAX_ATOMS + 4) + 4 * 9
 
     # ------------------------------------------------------------------
     def place(self, slot: int, col, index: int, now: float):

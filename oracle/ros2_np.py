@@ -350,6 +350,73 @@ def banded_solve(diag, sup, sub, rhs):
     return sla.solve_banded((bw, bw), ab, rhs.reshape(-1), check_finite=False).reshape(nz, ni)
 
 
+def _two_sum(a, b):
+    """s + e = a + b exactly (Knuth's TwoSum)."""
+    s = a + b
+    bb = s - a
+    return s, (a - (s - bb)) + (b - bb)
+
+
+def _two_prod(a, b):
+    """p + e = a * b exactly (Dekker's product with Veltkamp's split)."""
+    p = a * b
+    def split(v):
+        c = 134217729.0 * v           # 2^27 + 1
+        h = c - (c - v)
+        return h, v - h
+    ah, al = split(a)
+    bh, bl = split(b)
+    return p, al * bl - (((p - ah * bh) - al * bh) - ah * bl)
+
+
+def _residual_dd(diag, sup, sub, rhs, xh, xl):
+    """rhs - A (xh + xl) as accurate as if formed in twice FP64's precision
+    and rounded once: the compensated dot product Dot2 of Ogita, Rump and
+    Oishi (SIAM J. Sci. Comput. 26(6), 2005, Algorithm 5.3), the terms of
+    xl taken in plain FP64 (they are second order)."""
+    s, c = rhs.copy(), np.zeros_like(rhs)
+    def add(t_h, t_l):
+        nonlocal s, c
+        p, pe = _two_prod(t_h, t_l[0])
+        s, se = _two_sum(s, -p)
+        c = c + (se - pe) - t_h * t_l[1]
+    ni = rhs.shape[1]
+    for j in range(ni):
+        add(diag[:, :, j], (np.broadcast_to(xh[:, j:j + 1], rhs.shape),
+                            np.broadcast_to(xl[:, j:j + 1], rhs.shape)))
+    if rhs.shape[0] > 1:
+        z = np.zeros((1, ni))
+        add(np.concatenate([sup, z]), (np.concatenate([xh[1:], z]), np.concatenate([xl[1:], z])))
+        add(np.concatenate([z, sub]), (np.concatenate([z, xh[:-1]]), np.concatenate([z, xl[:-1]])))
+    return s + c
+
+
+def bt_solve_exact(diag, sup, sub, rhs, fac: BTFactors | None = None, max_iter: int = 40):
+    """The block-tridiagonal system (diagonal off-diagonal blocks) solved to
+    the exact solution of the FP64 system, rounded to FP64: the per-block LU
+    solve of bt_solve, then iterative refinement with the solution carried
+    as an unevaluated sum of two doubles and the residual formed with the
+    compensated dot product (_residual_dd), until a correction no longer
+    changes it.  Each step shrinks the error by ~cond * 2^-53, so this
+    converges while cond(A) < ~1e15; the limiting accuracy is
+    ~cond * 2^-106.  The yardstick for how close an FP64 solve (the
+    reference's or another) comes to the exact solution; not a restatement
+    of a reference function."""
+    if fac is None:
+        fac = bt_factor(diag, sup, sub)
+    xh = bt_solve(fac, rhs)
+    xl = np.zeros_like(xh)
+    for _ in range(max_iter):
+        d = bt_solve(fac, _residual_dd(diag, sup, sub, rhs, xh, xl))
+        s, e = _two_sum(xh, d)
+        xh, xl = _two_sum(s, e + xl)
+        if not np.all(np.isfinite(xh)):
+            raise FloatingPointError("bt_solve_exact: the refinement diverged")
+        if np.all(np.abs(d) <= 1e-20 * np.abs(xh)):
+            break
+    return xh + xl
+
+
 def chem_term_scale(y, M, k, net):
     """Per (layer, species): the sum of |terms| of chem_rhs (gross
     production + gross loss), the natural scale of its cancellation error."""
@@ -429,6 +496,25 @@ def ros2_step_banded(y, k, dt, atm, net):
     yk2 = y + k1 / r
     rhs2 = total_rhs(yk2, k, atm, net, grav) - (2.0 / (r * dt)) * k1
     k2 = banded_solve(diag, sup_neg, sub_neg, rhs2)
+    sol = y + (3.0 / (2.0 * r)) * k1 + (1.0 / (2.0 * r)) * k2
+    return sol, np.abs(sol - yk2)
+
+
+def ros2_step_exact(y, k, dt, atm, net):
+    """ros2_step with both stage systems solved exactly (bt_solve_exact);
+    the RHS evaluations stay FP64 as in ros2_step (jax_step.py:257-314).
+    What the step would be with error-free linear solves: the yardstick for
+    the solve error of FP64 steps on ill-conditioned columns, where the
+    reference's own per-block LU is off from it by far more than FP64
+    rounding."""
+    r = GAMMA
+    grav = compute_diff_grav(atm)
+    diag, sup_neg, sub_neg, rhs_y = ros2_lhs(y, k, dt, atm, net, grav)
+    fac = bt_factor(diag, sup_neg, sub_neg)
+    k1 = bt_solve_exact(diag, sup_neg, sub_neg, rhs_y, fac)
+    yk2 = y + k1 / r
+    rhs2 = total_rhs(yk2, k, atm, net, grav) - (2.0 / (r * dt)) * k1
+    k2 = bt_solve_exact(diag, sup_neg, sub_neg, rhs2, fac)
     sol = y + (3.0 / (2.0 * r)) * k1 + (1.0 / (2.0 * r)) * k2
     return sol, np.abs(sol - yk2)
 

@@ -84,6 +84,60 @@ def test_banded_solve_matches_dense():
     assert np.allclose(x_sw.reshape(-1), ref_sw, rtol=1e-12)
 
 
+def test_exact_solve_recovers_integer_solution_of_ill_conditioned_system():
+    """bt_solve_exact on a block-tridiagonal system with integer entries and
+    an integer solution, so the right-hand side and the exact solution are
+    representable: each diagonal block holds [[a, a+1], [a-1, a]] (det 1,
+    cond ~ 4a^2 = 4e12 at a = 1e6) beside a well-conditioned 2 x 2 part that
+    the off-diagonals couple.  The per-block LU leaves ~1e-4 relative error
+    on the ill-conditioned species; the refinement with compensated
+    (twice-FP64) residuals recovers the integers.  A residual formed in plain
+    FP64, or with sup and sub swapped, does not (both checked when this pin
+    was written: 7e-3 and 19 off)."""
+    rng = np.random.default_rng(17)
+    nz, ni, a = 6, 4, 1.0e6
+    diag = np.zeros((nz, ni, ni))
+    for j in range(nz):
+        diag[j, :2, :2] = [[a, a + 1.0], [a - 1.0, a]]
+        diag[j, 2:, 2:] = [[5.0, 1.0], [2.0, 4.0]]
+        diag[j, 2, 0] = 1.0
+    sup = np.zeros((nz - 1, ni))
+    sub = np.zeros((nz - 1, ni))
+    sup[:, 2:] = [1.0, -1.0]
+    sub[:, 2:] = [2.0, 1.0]
+    x_int = rng.integers(-50, 50, size=(nz, ni)).astype(np.float64)
+    A = O.assemble_dense(diag, sup, sub)
+    rhs = (A @ x_int.reshape(-1)).reshape(nz, ni)     # exact: integers far below 2^53
+    plain = O.bt_solve(O.bt_factor(diag, sup, sub), rhs)
+    exact = O.bt_solve_exact(diag, sup, sub, rhs)
+    scale = np.abs(x_int).max()
+    assert np.max(np.abs(plain - x_int)) > 1e-8 * scale
+    assert np.max(np.abs(exact - x_int)) <= 1e-12 * scale
+
+
+def test_exact_step_equals_step_when_well_conditioned(cfg1):
+    """At dt = 1e-18 every block is dominated by c0 I: the exactly solved
+    Ros2 step and the per-block LU step agree to roundoff on every cell; at
+    the workload's dt the exact step satisfies the stage systems to
+    extended-precision residuals."""
+    c = cfg1
+    a, _ = O.ros2_step(c.y, c.k, 1e-18, c.atm, c.net)
+    b, _ = O.ros2_step_exact(c.y, c.k, 1e-18, c.atm, c.net)
+    assert np.max(np.abs(a - b) / np.maximum(np.abs(a), 1e-300)) < 1e-13
+    diag, sup, sub, rhs = O.ros2_lhs(c.y, c.k, c.dt, c.atm, c.net)
+    x = O.bt_solve_exact(diag, sup, sub, rhs)
+    L = np.longdouble
+    xl = x.astype(L)
+    r = rhs.astype(L) - np.einsum("zij,zj->zi", diag.astype(L), xl)
+    r[:-1] -= sup.astype(L) * xl[1:]
+    r[1:] -= sub.astype(L) * xl[:-1]
+    mag = (np.einsum("zij,zj->zi", np.abs(diag), np.abs(x)) + np.abs(rhs))
+    mag[:-1] += np.abs(sup) * np.abs(x[1:])
+    mag[1:] += np.abs(sub) * np.abs(x[:-1])
+    # componentwise backward error of the rounded-to-FP64 solution: FP64 level
+    assert float(np.max(np.abs(r.astype(np.float64)) / mag)) < 1e-15
+
+
 def test_ros2_banded_equals_block_thomas_when_well_conditioned(cfg1):
     """At dt = 1e-18 every block is dominated by c0 I: the banded and block
     Thomas Ros2 steps agree to roundoff on every cell."""
