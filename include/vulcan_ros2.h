@@ -159,7 +159,7 @@ int vk_ros2_step_f64(vk_batch b, const double* y, const double* k, const double*
  * reference: it is for callers that cannot allocate, such as code under HIP
  * graph capture and tight serving loops.  The workspace is a file-scope
  * device array of VK_NOALLOC_SCRATCH_BYTES per GPU, held until the process
- * ends (160 MiB: one column of nz = 200, ni <= 96 in latency mode, or four
+ * ends (160 MiB: one column of nz = 200, ni <= 96 in latency mode, or three
  * such columns with the tile factor; vk_ros2_workspace_bytes says what a
  * batch needs).  A batch needing more is refused with VK_ERR_UNSUPPORTED.
  * Same arguments and results as vk_ros2_step_f64, bit for bit; the size
@@ -230,8 +230,10 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
  * backward stable; the refined one is, as the reference's lu_solve is
  * (solver.py:91-112).  The block-Thomas factors refine each block solve of
  * the backward sweep, x_j = X_j t_j; x_j += X_j (t_j - A'_j x_j) with A'_j
- * the Schur complement X_j inverts (one extra read of the diagonal-block
- * stack per refined sweep); a column where some layer's correction exceeded
+ * the Schur complement X_j inverts (the tile factor stores each A'_j beside
+ * its inverse, so a refined sweep reads one more block stack; the other
+ * variants form A'_j x_j from D_j and X_{j-1}); a column where some layer's
+ * correction exceeded
  * that layer's solution (an inverse with no correct digit, cond(A'_j) of
  * 1e12 and more) then takes one whole-column step, residual and correction
  * solve, which the other columns skip.  The cyclic-reduction factor takes

@@ -78,8 +78,9 @@ struct KernelTimer {
 int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag, int diag_tile,
                         int transpose, const double* sup_d, const double* sub_d, double* inv,
                         const double* rhs, double* rmod, int* status, const int* active,
-                        hipStream_t stream);
+                        hipStream_t stream, double* schur = nullptr);
 bool vk_factor_fuses_sweep(int sl);
+bool vk_factor_stores_schur(int sl);
 bool vk_factor_is_cr(int sl);
 int vk_launch_nat_to_tile(int sl, const VkCol* cols, int ncols, int max_nz, const double* nat,
                           double* tile, int transpose, hipStream_t stream);
@@ -94,7 +95,7 @@ int vk_launch_bt_solve(int sl, const VkCol* cols, int ncols, const double* inv,
                        const double* sup_d, const double* sub_d, const double* rhs,
                        double* rscratch, double* x, int mode, SolveEpi epi, const int* active,
                        hipStream_t stream, const double* refine_diag = nullptr,
-                       int* refl = nullptr);
+                       int* refl = nullptr, const double* refine_schur = nullptr);
 int vk_launch_tile_to_nat(int sl, const VkCol* cols, int ncols, int max_nz, const double* tile,
                           double* nat, hipStream_t stream);
 int vk_launch_bt_residual(int sl, const VkCol* cols, int ncols, int max_nz, const double* blk,
@@ -860,9 +861,15 @@ static size_t ros2_ws_inv_offset(vk_batch b) {
          align_up<double>(b->k_doubles);
 }
 
+// Offset of the Schur-complement stack: with refinement the tile factor
+// stores each A'_j there, and the refined solves take their residuals on it.
+static size_t ros2_ws_schur_offset(vk_batch b) {
+  return ros2_ws_inv_offset(b) + align_up<double>(b->block_doubles);
+}
+
 // Offset of the per-column whole-column refinement flags (ints).
 static size_t ros2_ws_refl_offset(vk_batch b) {
-  return ros2_ws_inv_offset(b) + align_up<double>(b->block_doubles);
+  return ros2_ws_schur_offset(b) + align_up<double>(b->block_doubles);
 }
 
 // Bytes before the cyclic-reduction region of the step workspace.
@@ -954,6 +961,7 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
   double* inv = g_refine > 0 ? (double*)((char*)ws + ros2_ws_inv_offset(b)) : blk;
   const double* ref1 = g_refine >= 1 ? blk : nullptr;
   const double* ref2 = g_refine >= 2 ? blk : nullptr;
+  double* schur = (double*)((char*)ws + ros2_ws_schur_offset(b));
   int* refl = (int*)((char*)ws + ros2_ws_refl_offset(b));
   const double r = ros2_gamma();
   {
@@ -992,6 +1000,13 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
     // cyclic-reduction variant has its own factor and solves.
     const bool cr = bucket_is_cr(b, sl);
     const bool fused = !cr && vk_factor_fuses_sweep(sl);
+    // refined block solves on the stored Schur complements (the tile factor)
+    // or on D_j and X_{j-1} (the other variants)
+    const bool sch = !cr && ref1 && vk_factor_stores_schur(sl);
+    const double* rd1 = sch ? nullptr : ref1;
+    const double* rd2 = sch ? nullptr : ref2;
+    const double* rs1 = sch ? schur : nullptr;
+    const double* rs2 = sch && ref2 ? schur : nullptr;
     {
       KernelTimer t(1, s);
       if (cr)  // the stage-1 reduction of rhs_y rides in the update launches
@@ -1000,7 +1015,7 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
       else
         rc = vk_launch_bt_factor(sl, cols, nc, blk, 1, 0, sup, sub, inv,
                                  fused ? rhs_y : nullptr, fused ? rmod : nullptr, (int*)stc,
-                                 act, s);
+                                 act, s, sch ? schur : nullptr);
       if (rc) return rc;
     }
     SolveEpi e1{1, r, 0.0, y, nullptr, nullptr, yk2, nullptr};
@@ -1027,7 +1042,7 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
         // the columns rfl flags: residual into rhs2 (free until the stage-2
         // chemistry), then k1 += the correction, yk2 again
         if ((rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, fused ? rmod : rhs_y,
-                                     rmod, k1, fused ? 1 : 0, e1, act, s, ref1, rfl)))
+                                     rmod, k1, fused ? 1 : 0, e1, act, s, rd1, rfl, rs1)))
           return rc;
         if ((rc = vk_launch_bt_residual(sl, cols, nc, b->max_nz, blk, sup, sub, rhs_y,
                                         k1, rhs2, rfl, s)))
@@ -1035,7 +1050,7 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
         SolveEpi e1r = e1;
         e1r.base = k1;
         rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, rhs2, rmod, k1, 0, e1r, rfl, s,
-                                ref1);
+                                rd1, nullptr, rs1);
       } else {
         rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, fused ? rmod : rhs_y,
                                 rmod, k1, fused ? 1 : 0, e1, act, s);
@@ -1073,7 +1088,7 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
         // the columns rfl flags: residual into rhs_y (free after stage 1),
         // then k2 += the correction, sol and delta again
         if ((rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, rhs2, rmod, k2, 0, e2,
-                                     act, s, ref2, rfl)))
+                                     act, s, rd2, rfl, rs2)))
           return rc;
         if ((rc = vk_launch_bt_residual(sl, cols, nc, b->max_nz, blk, sup, sub, rhs2,
                                         k2, rhs_y, rfl, s)))
@@ -1081,7 +1096,7 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
         SolveEpi e2r = e2;
         e2r.base = k2;
         rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, rhs_y, rmod, k2, 0, e2r, rfl, s,
-                                ref2);
+                                rd2, nullptr, rs2);
       } else {
         rc = vk_launch_bt_solve(sl, cols, nc, inv, sup, sub, rhs2, rmod, k2, 0, e2, act, s);
       }

@@ -285,13 +285,15 @@ __device__ inline double nat_elem(const double* blk, int ni, int i, int q, int t
 // instantiations: the natural path's bounds and transposition arithmetic
 // would otherwise stay live beside the register tile.  The tile-input kernel
 // is held to 128 VGPRs for SL <= 6 (four columns per CU); what it spills is
-// per-layer state, outside the pivot steps.
+// per-layer state, outside the pivot steps.  When schur is non-null (tile
+// layout at off_b) each Schur complement A'_j is stored there before its
+// inversion, for the refined solves.
 template <int SL, bool TILE_IN>
 __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(TILE_IN && SL <= 6 ? 4 : 1))) void k_bt_factor(
     const VkCol* __restrict__ cols, int ncols, const double* diag,
     int transpose, const double* __restrict__ sup_d, const double* __restrict__ sub_d,
     double* inv, const double* __restrict__ rhs, double* __restrict__ rmod,
-    int* __restrict__ status, const int* __restrict__ active) {
+    int* __restrict__ status, const int* __restrict__ active, double* __restrict__ schur) {
   __shared__ BtShared<SL> sh;
   const int c = blockIdx.x;
   if (c >= ncols) return;
@@ -372,6 +374,7 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(TILE_IN &
         __builtin_amdgcn_sched_barrier(0);
       }
     }
+    if (schur) store_tile<SL>(a, schur + col.off_b + (int64_t)j * BLK);
     bad |= gj_invert<SL>(a, ni, sh);
     store_tile<SL>(a, inv + col.off_b + (int64_t)j * BLK);
   }
@@ -396,7 +399,12 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(TILE_IN &
 // is still in registers, so a refined sweep reads each inverse once and D
 // once.
 //
-// refl (REFINE 1, optional): per column, 0 when the block refinement's
+// REFINE 2 (dstk holds the Schur complements A'_j themselves, stored by the
+// factor): the same refinement of each block solve of the backward sweep,
+// with the residual t_j - A'_j x0_j taken on the stored A'_j, so a layer
+// reads X_j and A'_j and needs no second inverse in registers.
+//
+// refl (REFINE 1 and 2, optional): per column, 0 when the block refinement's
 // correction of some layer exceeded that layer's solution in max norm (an
 // inverse X_j with no correct digit there, cond(A'_j) ~ 1e12 and up), else
 // VK_ST_DONE.  The host gives those columns one whole-column refinement step
@@ -417,7 +425,7 @@ __device__ inline void solve_epilogue(const SolveEpi& epi, int64_t e, double xv,
 }
 
 template <int SL, int REFINE>
-__global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ? 2 : 1))) void k_bt_solve(
+__global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ? (REFINE == 1 ? 2 : 4) : 1))) void k_bt_solve(
     const VkCol* __restrict__ cols, int ncols, const double* __restrict__ inv,
     const double* __restrict__ sup_d, const double* __restrict__ sub_d,
     const double* __restrict__ rhs, double* __restrict__ rscratch, double* x,
@@ -498,6 +506,74 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ?
       }
       __syncthreads();
     }
+  } else if constexpr (REFINE == 2) {
+    // per layer: x0 = X_j t_j; r = t_j - A'_j x0; x_j = x0 + X_j r
+    const double* sblk = dstk + col.off_b;
+    bool wide = false;  // thread 0: some layer's correction exceeded its solution
+    for (int j = nz - 1; j >= 0; --j) {
+      load_tile<SL>(a, stk + (int64_t)j * BLK);
+      tile_matvec<SL>(a, vec[j & 1], w);                      // x0_j
+      if (tc == 0) {
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          const int i = s * VK_G + tr;
+          rv[i] = (i < ni) ? w[s] : 0.0;
+        }
+      }
+      __syncthreads();
+      double q[SL];
+      stream_matvec<SL>(sblk + (int64_t)j * BLK, rv, q);      // A'_j x0_j
+      if (tc == 0) {
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          const int i = s * VK_G + tr;
+          bx[i] = (i < ni) ? vec[j & 1][i] - q[s] : 0.0;
+        }
+      }
+      __syncthreads();
+      double d[SL];
+      tile_matvec<SL>(a, bx, d);                              // X_j (t_j - A'_j x0_j)
+      double dmax = 0.0, xmax = 0.0;
+      if (tc == 0) {
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          const int i = s * VK_G + tr;
+          if (i < ni) {
+            const double xv = w[s] + d[s];
+            dmax = fmax(dmax, fabs(d[s]));
+            xmax = fmax(xmax, fabs(xv));
+            solve_epilogue(epi, col.off_s + (int64_t)j * ni + i, xv, x);
+            if (j > 0)
+              vec[(j - 1) & 1][i] = rm[col.off_s + (int64_t)(j - 1) * ni + i] -
+                                    bsup[(int64_t)(j - 1) * ni + i] * xv;
+          } else if (j > 0) {
+            vec[(j - 1) & 1][i] = 0.0;
+          }
+        }
+      }
+      if (refl) {
+        // the tc == 0 lanes of a wave are 0, 16, 32, 48
+        dmax = fmax(dmax, __shfl_xor(dmax, 16));
+        dmax = fmax(dmax, __shfl_xor(dmax, 32));
+        xmax = fmax(xmax, __shfl_xor(xmax, 16));
+        xmax = fmax(xmax, __shfl_xor(xmax, 32));
+        if ((tid & 63) == 0) {
+          red[tid >> 6][0] = dmax;
+          red[tid >> 6][1] = xmax;
+        }
+      }
+      __syncthreads();
+      if (refl && tid == 0) {
+        double dm = red[0][0], xm = red[0][1];
+#pragma unroll
+        for (int v = 1; v < VK_NT / 64; ++v) {
+          dm = fmax(dm, red[v][0]);
+          xm = fmax(xm, red[v][1]);
+        }
+        wide = wide || dm > xm;
+      }
+    }
+    if (refl && tid == 0) refl[c] = wide ? 0 : VK_ST_DONE;
   } else {
     // step j: load X_j; finish layer u = j + 1 (its residual needs X_j, its
     // correction X_u, kept in xp); form x0_j = X_j t_j.  rv holds x0_u, bx
@@ -1278,6 +1354,10 @@ int vk_launch_bt_factor_blk(int sl, const VkCol* cols, int ncols, const double* 
 int g_factor_variant[8] = {VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE,
                            VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE};
 
+bool vk_factor_stores_schur(int sl) {
+  return sl >= 1 && sl <= 7 && g_factor_variant[sl] == VK_FACTOR_TILE;
+}
+
 bool vk_factor_fuses_sweep(int sl) {
   return !(sl >= 1 && sl <= 7 && (g_factor_variant[sl] == VK_FACTOR_BLOCKED ||
                                   g_factor_variant[sl] == VK_FACTOR_CR));
@@ -1286,7 +1366,11 @@ bool vk_factor_fuses_sweep(int sl) {
 int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag, int diag_tile,
                         int transpose, const double* sup_d, const double* sub_d, double* inv,
                         const double* rhs, double* rmod, int* status, const int* active,
-                        hipStream_t stream) {
+                        hipStream_t stream, double* schur) {
+  if (schur && !vk_factor_stores_schur(sl)) {
+    vk_set_error("block factor: the selected variant does not store the Schur complements");
+    return VK_ERR_UNSUPPORTED;
+  }
   if (sl >= 1 && sl <= 7 &&
       (g_factor_variant[sl] == VK_FACTOR_TILE2 || g_factor_variant[sl] == VK_FACTOR_WAVE ||
        g_factor_variant[sl] == VK_FACTOR_TILE8)) {
@@ -1309,11 +1393,11 @@ int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag
   if (diag_tile) {
     VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_bt_factor<SL_, true>), dim3(ncols), dim3(VK_NT), 0,
                                           stream, cols, ncols, diag, transpose, sup_d, sub_d,
-                                          inv, rhs, rmod, status, active));
+                                          inv, rhs, rmod, status, active, schur));
   } else {
     VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_bt_factor<SL_, false>), dim3(ncols), dim3(VK_NT), 0,
                                           stream, cols, ncols, diag, transpose, sup_d, sub_d,
-                                          inv, rhs, rmod, status, active));
+                                          inv, rhs, rmod, status, active, schur));
   }
   VK_CHECK_HIP(hipGetLastError());
   return VK_OK;
@@ -1322,8 +1406,13 @@ int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag
 int vk_launch_bt_solve(int sl, const VkCol* cols, int ncols, const double* inv,
                        const double* sup_d, const double* sub_d, const double* rhs,
                        double* rscratch, double* x, int mode, SolveEpi epi, const int* active,
-                       hipStream_t stream, const double* refine_diag, int* refl) {
-  if (refine_diag) {
+                       hipStream_t stream, const double* refine_diag, int* refl,
+                       const double* refine_schur) {
+  if (refine_schur) {
+    VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_bt_solve<SL_, 2>), dim3(ncols), dim3(VK_NT), 0,
+                                          stream, cols, ncols, inv, sup_d, sub_d, rhs, rscratch,
+                                          x, mode, epi, active, refine_schur, refl));
+  } else if (refine_diag) {
     VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_bt_solve<SL_, 1>), dim3(ncols), dim3(VK_NT), 0,
                                           stream, cols, ncols, inv, sup_d, sub_d, rhs, rscratch,
                                           x, mode, epi, active, refine_diag, refl));

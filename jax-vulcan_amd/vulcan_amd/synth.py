@@ -522,3 +522,172 @@ def poisson_arrivals(seed: int = 1005, n: int = 10000, lam: float = 200.0):
     """Arrival times (s) of a Poisson stream with rate lam."""
     rng = np.random.default_rng(seed)
     return np.cumsum(rng.exponential(1.0 / lam, size=n))
+
+
+# --------------------------------------------------------------------------
+# workload 4: an open column with a unique steady state
+# --------------------------------------------------------------------------
+
+@dataclass
+class OpenColumn:
+    """The differentiation workload: one nz=150 column of 92 trace species
+    and a background gas (ni = 93) under 1200 reactions, open at the bottom.
+
+    Reactions q = 0 .. 1199 (forward row 2q+1, reverse row 2q+2):
+
+    * q = 0: the background gas, made at K_BG * M (three-body row with no
+      reactants) and lost at K_BG, so it settles at M and sets ysum;
+    * q = 1 .. 92: first-order sinks s -> 0 of every trace (irreversible);
+    * q = 93 .. 112: sources 0 -> s of 20 traces at k M (irreversible);
+    * q = 113 .. 612: reversible isomerisations s_a + M <-> s_b + M;
+    * q = 613 .. 1199: reversible 2->2 reactions among the traces, with the
+      workload-1 Arrhenius distributions.
+
+    Both reversible kinds take K_eq from per-species potentials (detailed
+    balance).  Ten traces with no volume source get bottom fluxes
+    (log-uniform 1e5..1e12 cm-2 s-1); every trace deposits at the bottom.
+    The pressure grid and T(z), Kzz and geometry follow workload 1.
+
+    Parameters of the gradient: one log multiplier per reaction (shared by
+    its two rows), log T(z) per layer (entering k(T), M = P/kT, Tco, Ti,
+    Hpi, g and the hydrostatic dzi) and the log bottom fluxes.
+    """
+    nz: int
+    ni: int
+    net: NetworkArrays
+    P: np.ndarray
+    T0: np.ndarray
+    Kzz: np.ndarray
+    kind: np.ndarray           # (nq,) 0 bg, 1 sink, 2 source, 3 isomerisation, 4 two-body
+    A: np.ndarray
+    b: np.ndarray
+    E: np.ndarray
+    logK: np.ndarray
+    flux_sp: np.ndarray        # (10,) species with bottom fluxes
+    logbf0: np.ndarray         # (10,) natural log of the fluxes
+    vdep: np.ndarray           # (ni,) deposition velocities
+    loss_sp: np.ndarray        # (5,) species of the loss
+    bg: int
+    T_ref: float = 1000.0
+    K_BG: float = 1.0
+
+    @property
+    def nq(self) -> int:
+        return int(self.kind.shape[0])
+
+    def leaves(self, T, logm=None, logbf=None):
+        """(k_arr, AtmStatic) at temperatures T, multipliers exp(logm), fluxes exp(logbf)."""
+        T = np.asarray(T, dtype=np.float64)
+        logm = np.zeros(self.nq) if logm is None else np.asarray(logm, dtype=np.float64)
+        logbf = self.logbf0 if logbf is None else np.asarray(logbf, dtype=np.float64)
+        M = self.P / (KB * T)
+        k = np.zeros((2 * self.nq + 1, self.nz))
+        m = np.exp(logm)
+        for q in range(self.nq):
+            kd = self.kind[q]
+            if kd == 0:
+                k[2 * q + 1] = m[q] * self.K_BG
+                k[2 * q + 2] = m[q] * self.K_BG
+                continue
+            kf = m[q] * self.A[q] * (T / self.T_ref) ** self.b[q] * np.exp(-self.E[q] / T)
+            k[2 * q + 1] = kf
+            if kd >= 3:
+                k[2 * q + 2] = kf / 10.0 ** self.logK[q]
+        geo = compute_geometry(self.P, T)
+        bf = np.zeros(self.ni)
+        bf[self.flux_sp] = np.exp(logbf)
+        atm = AtmStatic(
+            Kzz=self.Kzz, Dzz=np.zeros((self.nz - 1, self.ni)), dzi=geo["dzi"],
+            vz=np.zeros(self.nz - 1), Hpi=geo["Hpi"], Ti=geo["Ti"], Tco=T, g=geo["g"],
+            ms=np.full(self.ni, MU_MEAN), alpha=np.zeros(self.ni), M=M,
+            vm=np.zeros((self.nz, self.ni)), vs=np.zeros((self.nz - 1, self.ni)),
+            top_flux=np.zeros(self.ni), bot_flux=bf, bot_vdep=self.vdep,
+            gas_indx_mask=np.ones(self.ni, dtype=bool), use_botflux=True)
+        return k, atm
+
+    def initial_state(self):
+        """A starting state for the steady-state solve: traces at 1e-12 of M."""
+        y = np.empty((self.nz, self.ni))
+        M = self.P / (KB * self.T0)
+        y[:] = 1e-12 * M[:, None]
+        y[:, self.bg] = M
+        return y
+
+
+def config4(seed: int = 1004, nz: int = 150, n_iso: int = 500, n_two: int = 587,
+            n_src: int = 20, n_flux: int = 10) -> OpenColumn:
+    """The differentiation workload (see OpenColumn)."""
+    rng = np.random.default_rng(seed)
+    nt = 92
+    ni, bg = nt + 1, nt
+    P = np.logspace(9.0, -2.0, nz)
+    T0 = smooth_temperature(rng, P)
+    Kzz = 10.0 ** rng.uniform(5.0, 10.0, size=nz - 1)
+    pot = np.concatenate([rng.uniform(-1.25, 1.25, size=nt), [0.0]])
+    perm = rng.permutation(nt)
+    src_sp = np.sort(perm[:n_src])
+    flux_sp = np.sort(perm[n_src:n_src + n_flux])
+    loss_sp = np.sort(rng.choice(nt, size=5, replace=False))
+    nq = 1 + nt + n_src + n_iso + n_two
+    S = 2 * nq + 1
+    PAD = ni
+    r_idx = np.full((S, 2), PAD, dtype=np.int64)
+    p_idx = np.full((S, 2), PAD, dtype=np.int64)
+    r_st = np.zeros((S, 2))
+    p_st = np.zeros((S, 2))
+    is3 = np.zeros(S, dtype=bool)
+    kind = np.zeros(nq, dtype=np.int64)
+    A = np.zeros(nq)
+    b = np.zeros(nq)
+    E = np.zeros(nq)
+    logK = np.zeros(nq)
+
+    def put(q, reac, prod):
+        f, r = 2 * q + 1, 2 * q + 2
+        for s, (sp, st) in enumerate(_collapse(reac).items()):
+            r_idx[f, s], r_st[f, s] = sp, st
+            p_idx[r, s], p_st[r, s] = sp, st
+        for s, (sp, st) in enumerate(_collapse(prod).items()):
+            p_idx[f, s], p_st[f, s] = sp, st
+            r_idx[r, s], r_st[r, s] = sp, st
+
+    q = 0
+    put(q, [], [bg])                     # 0 -> BG at K_BG M; BG -> 0 at K_BG
+    is3[1] = True
+    q += 1
+    for s in range(nt):                  # sinks
+        put(q, [s], [])
+        kind[q], A[q] = 1, 10.0 ** rng.uniform(-5.0, -2.0)
+        b[q], E[q] = rng.uniform(-1.0, 1.0), rng.uniform(0.0, 1000.0)
+        q += 1
+    for s in src_sp:                     # sources at k M
+        put(q, [], [int(s)])
+        is3[2 * q + 1] = True
+        kind[q], A[q] = 2, 10.0 ** rng.uniform(-16.0, -12.0)
+        b[q], E[q] = rng.uniform(-1.0, 1.0), rng.uniform(0.0, 1000.0)
+        q += 1
+    for _ in range(n_iso):               # s_a + M <-> s_b + M
+        a_, b_ = rng.choice(nt, size=2, replace=False)
+        put(q, [int(a_)], [int(b_)])
+        is3[2 * q + 1] = is3[2 * q + 2] = True
+        kind[q], A[q] = 3, 10.0 ** rng.uniform(-22.0, -17.0)
+        b[q], E[q] = rng.uniform(-2.0, 2.0), rng.uniform(0.0, 2.0e4)
+        logK[q] = pot[b_] - pot[a_]
+        q += 1
+    for _ in range(n_two):               # two-body 2 -> 2 among the traces
+        reac = rng.integers(0, nt, size=2).tolist()
+        prod = rng.integers(0, nt, size=2).tolist()
+        put(q, reac, prod)
+        kind[q], A[q] = 4, 10.0 ** rng.uniform(-14.0, -9.0)
+        b[q], E[q] = rng.uniform(-2.0, 2.0), rng.uniform(0.0, 2.0e4)
+        logK[q] = pot[prod].sum() - pot[reac].sum()
+        q += 1
+    assert q == nq
+    net = NetworkArrays(ni=ni, nr=S - 1, reactant_idx=r_idx, product_idx=p_idx,
+                        reactant_stoich=r_st, product_stoich=p_st, is_three_body=is3)
+    vdep = np.full(ni, 1.0)
+    vdep[bg] = 0.0
+    logbf0 = np.log(10.0 ** rng.uniform(5.0, 12.0, size=n_flux))
+    return OpenColumn(nz=nz, ni=ni, net=net, P=P, T0=T0, Kzz=Kzz, kind=kind, A=A, b=b, E=E,
+                      logK=logK, flux_sp=flux_sp, logbf0=logbf0, vdep=vdep, loss_sp=loss_sp,
+                      bg=bg)
