@@ -109,6 +109,57 @@ def bench_rates(B: int, net, atm, reps: int = 20, cpu_cols: int = 20, no_cpu: bo
     return res
 
 
+def bench_config4(reps: int = 5):
+    """The differentiation workload: the steady state of the open nz=150,
+    ni=93, 1200-reaction column (synth.config4) by pseudo-transient
+    continuation on the cyclic-reduction factor, then the implicit gradient
+    of L = sum log ymix of 5 species with respect to every leaf (transposed
+    block Thomas + parameter VJP), timed on both factors."""
+    import numpy as np
+    import torch
+    from vulcan_amd import steady_state_grad as G
+    from vulcan_amd import synth
+    pb = synth.config4()
+    k0, atm0 = pb.leaves(pb.T0)
+    spt = torch.as_tensor(pb.loss_sp, device="cuda")
+
+    def loss(y):
+        ymix = y / y.sum(dim=1, keepdim=True)
+        return ymix[:, spt].log().sum()
+
+    out = {"workload": "open column nz=150, ni=93, 1200 reactions (2401 rate rows), bottom "
+                       "fluxes of 10 species; L = sum over layers of log ymix of 5 species; "
+                       "gradient wrt every k_arr entry and atmosphere leaf (1200 log k, 150 T(z) "
+                       "and 10 bottom fluxes by the chain rule)"}
+    try:
+        G.set_latency_mode(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ys, it, res = G.pseudo_transient_steady_state(pb.initial_state(), k0, atm0, pb.net)
+        torch.cuda.synchronize()
+        out["forward_solve_s"] = round(time.perf_counter() - t0, 3)
+        out["forward_iterations"] = it
+        out["residual_inf"] = res
+        inputs = G.build_steady_state_inputs(k0, atm0)
+        for mode in (False, True):
+            G.set_latency_mode(mode)
+            G.steady_state_value_and_grad(loss, inputs, ys, pb.net, residual_rtol=1e-9)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                _, gi = G.steady_state_value_and_grad(loss, inputs, ys, pb.net,
+                                                      residual_rtol=1e-9)
+            torch.cuda.synchronize()
+            key = "gradient_ms_cyclic_reduction" if mode else "gradient_ms_block_thomas"
+            out[key] = round((time.perf_counter() - t0) / reps * 1e3, 2)
+        gk = gi.k_arr.detach().cpu().numpy()
+        g_m = (k0[1::2] * gk[1::2] + k0[2::2] * gk[2::2]).sum(axis=1)
+        out["max_abs_dL_dlogk"] = float(np.abs(g_m).max())
+    finally:
+        G.set_latency_mode(False)
+    return out
+
+
 def bench_config2(photo_batch: int):
     """One full steady-state solve of the nz=150, ni=93, 2461-row column
     (independent K_eq) with two-stream photolysis over 2500 bins refreshed in
@@ -203,6 +254,8 @@ def main() -> int:
     ap.add_argument("--config2", action=argparse.BooleanOptionalAction, default=True,
                     help="time one full steady-state solve of the nz=150, ni=93 column with "
                          "in-loop photolysis (nbin=2500), and a batched photolysis update")
+    ap.add_argument("--config4", action=argparse.BooleanOptionalAction, default=True,
+                    help="time the differentiation workload (steady state + implicit gradient)")
     ap.add_argument("--photo-batch", type=int, default=256,
                     help="columns of the batched photolysis-update timing")
     ap.add_argument("--factor", choices=("auto", "tile", "rows", "blocked", "tile2", "wave", "tile8", "cr"),
@@ -454,6 +507,11 @@ def main() -> int:
         config2 = bench_config2(args.photo_batch)
         print(f"[bench] config 2: {config2}", file=sys.stderr, flush=True)
 
+    config4 = None
+    if args.config4:
+        config4 = bench_config4()
+        print(f"[bench] config 4: {config4}", file=sys.stderr, flush=True)
+
     conv_cost = None
     if args.conv_cost > 0:
         from vulcan_amd.outer_loop import ColumnConv, RunConfig, SteadyStateRunner
@@ -570,6 +628,7 @@ def main() -> int:
         "uneven_batch": uneven,
         "serving": serving,
         "config2_steady_state": config2,
+        "config4_gradient": config4,
         "stopping_rule_cost": conv_cost,
         "solve_latency": solve_latency,
         "rate_build": rate_build,

@@ -33,7 +33,7 @@ __all__ = [
     "steady_state_residual_inputs", "validate_steady_state_solution",
     "differentiable_steady_state_inputs", "checked_differentiable_steady_state",
     "steady_state_value_and_grad", "differentiable_steady_state", "newton_steady_state",
-    "adjoint_state",
+    "adjoint_state", "pseudo_transient_steady_state",
 ]
 
 ST_SINGULAR = 1
@@ -370,3 +370,49 @@ def newton_steady_state(y0, k_arr, atm: AtmStatic, net: NetworkArrays, tol: floa
         dy, _ = _factor_solve(diag, sup, sub, -res, net, nz, False)
         y = y + dy
     raise RuntimeError(f"Newton did not converge: ||res|| = {nrm:.3e}")
+
+
+def pseudo_transient_steady_state(y0, k_arr, atm: AtmStatic, net: NetworkArrays,
+                                  dt0: float = 1e-6, grow: float = 4.0, shrink: float = 8.0,
+                                  dt_newton: float = 1e30, polish: int = 8,
+                                  max_iter: int = 1000):
+    """Steady state of f(y) = 0 by pseudo-transient continuation on the GPU.
+
+    Each iteration solves (I/dt - J_y) dy = f(y) with the J_y blocks and the
+    block-Thomas factor (a linearly implicit Euler step); a step that keeps
+    every density positive is taken and dt grows by ``grow``, otherwise dt
+    shrinks by ``shrink``.  Once dt reaches ``dt_newton`` the iteration is
+    Newton's, and ``polish`` more Newton steps finish it.  This is the
+    forward solve of the differentiation workload; the reference reaches the
+    same state with its adaptive Ros2 loop (outer_loop.py:492-1112), whose
+    stopping rule leaves a residual the implicit gradient then carries.
+
+    Returns (y, iterations, ||f||_inf).
+    """
+    inputs = build_steady_state_inputs(k_arr, atm)
+    y, k, nz, ni = _prep(y0, k_arr, net)
+    rec = _record(inputs, nz, ni)
+    y = y.clone()
+    dt = float(dt0)
+    left = int(polish)
+    eye = torch.arange(ni, device=y.device)
+    for it in range(max_iter):
+        res = _residual(y, k, rec, net, nz)
+        diag, sup, sub = _jacobian_blocks(y, k, rec, net, nz, ni)
+        newton = dt >= dt_newton
+        if not newton:
+            diag[:, eye, eye] -= 1.0 / dt
+        dy, status = _factor_solve(diag, sup, sub, -res, net, nz, False)
+        yn = y + dy
+        ok = bool(torch.all(yn > 0).item()) and not (int(status.item()) & ST_SINGULAR)
+        if ok:
+            y = yn
+            if newton:
+                left -= 1
+                if left <= 0:
+                    nrm = float(_residual(y, k, rec, net, nz).abs().max().item())
+                    return y, it + 1, nrm
+            dt = min(dt * grow, dt_newton)
+        else:
+            dt /= shrink
+    raise RuntimeError(f"pseudo-transient continuation did not converge in {max_iter} steps")

@@ -575,24 +575,27 @@ class OpenColumn:
     def nq(self) -> int:
         return int(self.kind.shape[0])
 
-    def leaves(self, T, logm=None, logbf=None):
-        """(k_arr, AtmStatic) at temperatures T, multipliers exp(logm), fluxes exp(logbf)."""
+    def leaves(self, T, logm=None, logbf=None, isobaric: bool = False):
+        """(k_arr, AtmStatic) at temperatures T, multipliers exp(logm), fluxes exp(logbf).
+
+        The number density M (and with it the background gas, hence ysum)
+        stays at the reference profile P / (k T0) unless ``isobaric``, when
+        it follows P / (k T).
+        """
         T = np.asarray(T, dtype=np.float64)
         logm = np.zeros(self.nq) if logm is None else np.asarray(logm, dtype=np.float64)
         logbf = self.logbf0 if logbf is None else np.asarray(logbf, dtype=np.float64)
-        M = self.P / (KB * T)
+        M = self.P / (KB * (T if isobaric else self.T0))
         k = np.zeros((2 * self.nq + 1, self.nz))
-        m = np.exp(logm)
-        for q in range(self.nq):
-            kd = self.kind[q]
-            if kd == 0:
-                k[2 * q + 1] = m[q] * self.K_BG
-                k[2 * q + 2] = m[q] * self.K_BG
-                continue
-            kf = m[q] * self.A[q] * (T / self.T_ref) ** self.b[q] * np.exp(-self.E[q] / T)
-            k[2 * q + 1] = kf
-            if kd >= 3:
-                k[2 * q + 2] = kf / 10.0 ** self.logK[q]
+        m = np.exp(logm)[:, None]
+        kf = m * self.A[:, None] * (T[None] / self.T_ref) ** self.b[:, None] \
+            * np.exp(-self.E[:, None] / T[None])
+        kr = np.where((self.kind >= 3)[:, None], kf / 10.0 ** self.logK[:, None], 0.0)
+        bgq = self.kind == 0
+        kf[bgq] = m[bgq] * self.K_BG
+        kr[bgq] = m[bgq] * self.K_BG
+        k[1::2] = kf
+        k[2::2] = kr
         geo = compute_geometry(self.P, T)
         bf = np.zeros(self.ni)
         bf[self.flux_sp] = np.exp(logbf)
@@ -604,6 +607,15 @@ class OpenColumn:
             top_flux=np.zeros(self.ni), bot_flux=bf, bot_vdep=self.vdep,
             gas_indx_mask=np.ones(self.ni, dtype=bool), use_botflux=True)
         return k, atm
+
+    def dlogk_dlogT(self, T):
+        """d log k / d log T per reaction and layer (nq, nz): b + E/T for the
+        Arrhenius rows (a reverse row shares its forward row's, K_eq being
+        independent of T here), 0 for the background pair."""
+        T = np.asarray(T, dtype=np.float64)
+        s = self.b[:, None] + self.E[:, None] / T[None]
+        s[self.kind == 0] = 0.0
+        return s
 
     def initial_state(self):
         """A starting state for the steady-state solve: traces at 1e-12 of M."""
@@ -663,7 +675,7 @@ def config4(seed: int = 1004, nz: int = 150, n_iso: int = 500, n_two: int = 587,
     for s in src_sp:                     # sources at k M
         put(q, [], [int(s)])
         is3[2 * q + 1] = True
-        kind[q], A[q] = 2, 10.0 ** rng.uniform(-16.0, -12.0)
+        kind[q], A[q] = 2, 10.0 ** rng.uniform(-22.0, -18.0)
         b[q], E[q] = rng.uniform(-1.0, 1.0), rng.uniform(0.0, 1000.0)
         q += 1
     for _ in range(n_iso):               # s_a + M <-> s_b + M

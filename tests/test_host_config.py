@@ -1,4 +1,4 @@
-"""CPU: host-side validation of the run configuration (no GPU calls)."""
+"""CPU: host-side checks of the run configuration and the workload generators (no GPU calls)."""
 from __future__ import annotations
 
 import pytest
@@ -37,3 +37,25 @@ def test_vul_end_case_codes():
     assert end_case(100, 1.0, ST_MAXSTEPS | ST_CONVERGED, rm) == 1
     assert end_case(100, 1.0, ST_CONVERGED | 2, rm) == 1
     assert end_case(40, 1.0, 0, rm) == 1
+
+
+def test_config4_rate_tangent_matches_leaf_differences():
+    """The exact d log k / d log T of the differentiation workload's leaf
+    generator against central differences of the generator itself."""
+    import numpy as np
+    from vulcan_amd import synth
+    pb = synth.config4()
+    s = pb.dlogk_dlogT(pb.T0)
+    for z in (0, 57, 149):
+        h = 1e-6
+        Tp, Tm = pb.T0.copy(), pb.T0.copy()
+        Tp[z] *= np.exp(h)
+        Tm[z] *= np.exp(-h)
+        kp, _ = pb.leaves(Tp)
+        km, _ = pb.leaves(Tm)
+        k0, _ = pb.leaves(pb.T0)
+        fd = (kp[:, z] - km[:, z]) / (2 * h)
+        exact = np.concatenate([[0.0], np.repeat(s[:, z], 2)]) * k0[:, z]
+        np.testing.assert_allclose(fd, exact, rtol=1e-6, atol=1e-300)
+        # other layers do not move
+        assert np.array_equal(np.delete(kp, z, axis=1), np.delete(k0, z, axis=1))
