@@ -24,6 +24,8 @@
  *   vk_chem_jac_f64      chem.chem_jac_analytical              chem.py:144-214
  *   vk_bt_factor_f64     solver.factor_block_thomas_diag_offdiag solver.py:33-64
  *   vk_bt_solve_f64      solver.solve_block_thomas_diag_offdiag  solver.py:67-112
+ *   vk_bt_dense_f64      solver.block_thomas                   solver.py:129-176
+ *   vk_diff_grav_f64     jax_step.compute_diff_grav            jax_step.py:67-147
  *   vk_ros2_step_f64     jax_step.jax_ros2_step                jax_step.py:249-315
  *   vk_step_error_f64    integrate.jax_integrate_fixed_dt's delta  integrate.py:30-33
  *   vk_residual_f64      steady_state_grad.steady_state_residual steady_state_grad.py:116-150
@@ -130,6 +132,25 @@ int vk_bt_factor_f64(vk_batch b, const double* diag, const double* sup_d, const 
 /* x = solve with a factored stack; rhs and x at off_s; scratch: state-sized. */
 int vk_bt_solve_f64(vk_batch b, const double* inv, const double* sup_d, const double* sub_d,
                     const double* rhs, double* x, double* scratch, void* stream);
+/* jax_step.compute_diff_grav (jax_step.py:67-147): the y-independent parts
+ * of the molecular-diffusion coefficients from the atmosphere records.
+ * A, B, C are (nz, ni) per column at off_s: rows 1..nz-2 are A/B/C_grav_int,
+ * row 0 holds bdry0_grav (A, B) and row nz-1 bdry_top_grav (A, C); the
+ * entries the reference does not define (C of row 0, B of row nz-1) are 0.
+ * dz_ave = 0.5 (dzi[j-1] + dzi[j]) is left to the caller. */
+int vk_diff_grav_f64(vk_batch b, const double* atm, double* A, double* B, double* C,
+                     void* stream);
+/* Dense block-tridiagonal Thomas: solver.block_thomas (solver.py:129-176),
+ * full ni x ni off-diagonal blocks, one right-hand side, factor and both
+ * sweeps in one launch (one workgroup per column).  diag, sup and sub are
+ * natural (ni, ni) blocks at off_s * ni, nz blocks per column (sup[j] = B_j
+ * couples layer j to j+1, sub[j] = C_{j+1} couples j+1 to j; block nz-1 of
+ * sup and sub is not read); rhs and x at off_s.  inv receives the inverses
+ * of the Schur complements (tile layout, off_b); scratch is state-sized;
+ * status as vk_bt_factor_f64. */
+int vk_bt_dense_f64(vk_batch b, const double* diag, const double* sup, const double* sub,
+                    const double* rhs, double* x, double* inv, double* scratch, int32_t* status,
+                    void* stream);
 /* Latency mode of the two above (no counterpart in the reference; same
  * system as solver.factor_block_thomas_diag_offdiag /
  * solve_block_thomas_diag_offdiag, solver.py:33-112): block cyclic

@@ -98,6 +98,11 @@ int vk_launch_bt_solve(int sl, const VkCol* cols, int ncols, const double* inv,
                        int* refl = nullptr, const double* refine_schur = nullptr);
 int vk_launch_tile_to_nat(int sl, const VkCol* cols, int ncols, int max_nz, const double* tile,
                           double* nat, hipStream_t stream);
+int vk_launch_diff_grav(const VkCol* cols, int ncols, int max_nz, const double* atm, double* A,
+                        double* B, double* C, hipStream_t stream);
+int vk_launch_bt_dense(int sl, const VkCol* cols, int ncols, const double* diag,
+                       const double* sup, const double* sub, const double* rhs, double* x,
+                       double* inv, double* rmod, int* status, hipStream_t stream);
 int vk_launch_bt_residual(int sl, const VkCol* cols, int ncols, int max_nz, const double* blk,
                           const double* sup_d, const double* sub_d, const double* rhs,
                           const double* x, double* res, const int* active, hipStream_t stream);
@@ -791,6 +796,34 @@ int vk_bt_solve_f64(vk_batch b, const double* inv, const double* sup_d, const do
   for (int sl : b->slots) {
     int rc = vk_launch_bt_solve(sl, b->d_cols, b->ncols, inv, sup_d, sub_d, rhs, scratch, x, 0,
                                 epi, nullptr, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return VK_OK;
+}
+
+int vk_diff_grav_f64(vk_batch b, const double* atm, double* A, double* B, double* C,
+                     void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_batch(b)) return rc;
+  if (!atm || !A || !B || !C) {
+    vk_set_error("vk_diff_grav_f64: null pointer");
+    return VK_ERR_ARG;
+  }
+  return vk_launch_diff_grav(b->d_cols, b->ncols, b->max_nz, atm, A, B, C, (hipStream_t)stream);
+}
+
+int vk_bt_dense_f64(vk_batch b, const double* diag, const double* sup, const double* sub,
+                    const double* rhs, double* x, double* inv, double* scratch, int32_t* status,
+                    void* stream) {
+  g_err[0] = 0;
+  if (int rc = check_plain(b, "vk_bt_dense_f64")) return rc;
+  if (!diag || !sup || !sub || !rhs || !x || !inv || !scratch) {
+    vk_set_error("vk_bt_dense_f64: null pointer");
+    return VK_ERR_ARG;
+  }
+  for (int sl : b->slots) {
+    int rc = vk_launch_bt_dense(sl, b->d_cols, b->ncols, diag, sup, sub, rhs, x, inv, scratch,
+                                (int*)status, (hipStream_t)stream);
     if (rc) return rc;
   }
   return VK_OK;

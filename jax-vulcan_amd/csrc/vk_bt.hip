@@ -701,6 +701,174 @@ __global__ __launch_bounds__(VK_NT) void k_bt_residual(
   }
 }
 
+// Dense block-tridiagonal Thomas (solver.py:129-176): full ni x ni
+// off-diagonal blocks, one workgroup per column, one right-hand side.
+//   A'_0 = D_0;  A'_j = D_j - C_j X_{j-1} B_{j-1};  X_j = inv(A'_j)
+//   r'_0 = r_0;  r'_j = r_j - C_j (X_{j-1} r'_{j-1})
+//   x_{nz-1} = X_{nz-1} r'_{nz-1};  x_j = X_j (r'_j - B_j x_{j+1})
+// with B_j = sup[j], C_j = sub[j-1] as in the reference's scan.  diag, sup
+// and sub are natural (ni, ni) blocks at off_s * ni, layer j at j ni^2 (sup
+// and sub padded to nz blocks per column, the last unused); rhs, x and rmod
+// (r'_j) at off_s; inv receives X_j in tile layout at off_b.  The product
+// C_j X_{j-1} B_{j-1} is formed as (C_j X_{j-1}) B_{j-1}, X_{j-1} and then
+// C_j X_{j-1} staged in LDS.
+template <int SL>
+__global__ __launch_bounds__(VK_NT) void k_bt_dense(
+    const VkCol* __restrict__ cols, int ncols, const double* __restrict__ diag,
+    const double* __restrict__ sup, const double* __restrict__ sub,
+    const double* __restrict__ rhs, double* __restrict__ x, double* __restrict__ inv,
+    double* __restrict__ rmod, int* __restrict__ status) {
+  constexpr int N = VK_G * SL;
+  __shared__ BtShared<SL> sh;
+  __shared__ double xs[N * N];   // X_{j-1}, then C_j X_{j-1} (row-major, padded)
+  __shared__ double wv[N];       // X_{j-1} r'_{j-1}, then the backward sweep's vector
+  const int c = blockIdx.x;
+  if (c >= ncols) return;
+  const VkCol col = cols[c];
+  if (vk_slots(col.ni) != SL) return;
+  const int nz = col.nz, ni = col.ni;
+  const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+  constexpr int BLK = vk_blk<SL>();
+  const int64_t nn = (int64_t)ni * ni;
+  const double* D = diag + col.off_s * ni;
+  const double* Bs = sup + col.off_s * ni;
+  const double* Cs = sub + col.off_s * ni;
+  double a[SL][SL];
+  int bad = 0;
+  for (int j = 0; j < nz; ++j) {
+    const double* Dj = D + j * nn;
+#pragma unroll
+    for (int s = 0; s < SL; ++s)
+#pragma unroll
+      for (int t = 0; t < SL; ++t) a[s][t] = nat_elem(Dj, ni, s * VK_G + tr, t * VK_G + tc, 0);
+    if (j == 0) {
+      for (int i = tid; i < N; i += VK_NT) {
+        const double v = (i < ni) ? rhs[col.off_s + i] : 0.0;
+        sh.vec[0][i] = v;
+        if (i < ni) rmod[col.off_s + i] = v;
+      }
+    } else {
+      const double* Cj = Cs + (int64_t)(j - 1) * nn;   // C_j = sub[j-1]
+      const double* Bj = Bs + (int64_t)(j - 1) * nn;   // B_{j-1} = sup[j-1]
+      // r'_j = r_j - C_j w, w = X_{j-1} r'_{j-1} in wv
+      double cw[SL];
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        const int i = s * VK_G + tr;
+        double acc = 0.0;
+#pragma unroll
+        for (int t = 0; t < SL; ++t) acc = fma(nat_elem(Cj, ni, i, t * VK_G + tc, 0),
+                                               wv[t * VK_G + tc], acc);
+        cw[s] = vk_rowsum16(acc);
+      }
+      if (tc == 0) {
+#pragma unroll
+        for (int s = 0; s < SL; ++s) {
+          const int i = s * VK_G + tr;
+          double r = 0.0;
+          if (i < ni) {
+            r = rhs[col.off_s + (int64_t)j * ni + i] - cw[s];
+            rmod[col.off_s + (int64_t)j * ni + i] = r;
+          }
+          sh.vec[j & 1][i] = r;
+        }
+      }
+      // Q = C_j X_{j-1} (X_{j-1} in xs)
+      double q[SL][SL];
+#pragma unroll
+      for (int s = 0; s < SL; ++s)
+#pragma unroll
+        for (int t = 0; t < SL; ++t) q[s][t] = 0.0;
+      for (int k = 0; k < ni; ++k) {
+        double ck[SL], xk[SL];
+#pragma unroll
+        for (int s = 0; s < SL; ++s) ck[s] = nat_elem(Cj, ni, s * VK_G + tr, k, 0);
+#pragma unroll
+        for (int t = 0; t < SL; ++t) xk[t] = xs[k * N + t * VK_G + tc];
+#pragma unroll
+        for (int s = 0; s < SL; ++s)
+#pragma unroll
+          for (int t = 0; t < SL; ++t) q[s][t] = fma(ck[s], xk[t], q[s][t]);
+      }
+      __syncthreads();  // every thread is done with X_{j-1} in xs
+#pragma unroll
+      for (int s = 0; s < SL; ++s)
+#pragma unroll
+        for (int t = 0; t < SL; ++t) xs[(s * VK_G + tr) * N + t * VK_G + tc] = q[s][t];
+      __syncthreads();
+      // A'_j = D_j - Q B_{j-1}
+      for (int k = 0; k < ni; ++k) {
+        double qk[SL], bk[SL];
+#pragma unroll
+        for (int s = 0; s < SL; ++s) qk[s] = xs[(s * VK_G + tr) * N + k];
+#pragma unroll
+        for (int t = 0; t < SL; ++t) bk[t] = nat_elem(Bj, ni, k, t * VK_G + tc, 0);
+#pragma unroll
+        for (int s = 0; s < SL; ++s)
+#pragma unroll
+          for (int t = 0; t < SL; ++t) a[s][t] = fma(-qk[s], bk[t], a[s][t]);
+      }
+    }
+    __syncthreads();
+    bad |= gj_invert<SL>(a, ni, sh);
+    store_tile<SL>(a, inv + col.off_b + (int64_t)j * BLK);
+    // X_j into xs for the next layer; w = X_j r'_j into wv
+    double w[SL];
+    tile_matvec<SL>(a, sh.vec[j & 1], w);
+#pragma unroll
+    for (int s = 0; s < SL; ++s)
+#pragma unroll
+      for (int t = 0; t < SL; ++t) xs[(s * VK_G + tr) * N + t * VK_G + tc] = a[s][t];
+    if (tc == 0) {
+#pragma unroll
+      for (int s = 0; s < SL; ++s) wv[s * VK_G + tr] = (s * VK_G + tr < ni) ? w[s] : 0.0;
+    }
+    __syncthreads();
+  }
+  // backward sweep: x_{nz-1} = X_{nz-1} r'_{nz-1} (= wv now)
+  for (int i = tid; i < ni; i += VK_NT) x[col.off_s + (int64_t)(nz - 1) * ni + i] = wv[i];
+  __syncthreads();
+  for (int j = nz - 2; j >= 0; --j) {
+    // t_j = r'_j - B_j x_{j+1} with x_{j+1} in wv, into sh.vec[0]
+    const double* Bj = Bs + (int64_t)j * nn;
+    double bx[SL];
+#pragma unroll
+    for (int s = 0; s < SL; ++s) {
+      const int i = s * VK_G + tr;
+      double acc = 0.0;
+#pragma unroll
+      for (int t = 0; t < SL; ++t) acc = fma(nat_elem(Bj, ni, i, t * VK_G + tc, 0),
+                                             wv[t * VK_G + tc], acc);
+      bx[s] = vk_rowsum16(acc);
+    }
+    if (tc == 0) {
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        const int i = s * VK_G + tr;
+        sh.vec[0][i] = (i < ni) ? rmod[col.off_s + (int64_t)j * ni + i] - bx[s] : 0.0;
+      }
+    }
+    __syncthreads();
+    double xv[SL];
+    stream_matvec<SL>(inv + col.off_b + (int64_t)j * BLK, sh.vec[0], xv);
+    __syncthreads();  // every thread has read wv and sh.vec[0]
+    if (tc == 0) {
+#pragma unroll
+      for (int s = 0; s < SL; ++s) {
+        const int i = s * VK_G + tr;
+        if (i < ni) {
+          x[col.off_s + (int64_t)j * ni + i] = xv[s];
+          wv[i] = xv[s];
+        } else {
+          wv[i] = 0.0;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (bad && tid == 0 && status) atomicOr(&status[c], VK_ST_SINGULAR);
+}
+
 // Copy a tile-layout inverse stack into natural (nz, ni, ni) blocks.
 template <int SL>
 __global__ __launch_bounds__(VK_NT) void k_tile_to_nat(const VkCol* __restrict__ cols, int ncols,
@@ -1431,6 +1599,16 @@ int vk_launch_bt_residual(int sl, const VkCol* cols, int ncols, int max_nz, cons
   VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_bt_residual<SL_>, dim3(max_nz, ncols), dim3(VK_NT), 0,
                                         stream, cols, ncols, blk, sup_d, sub_d, rhs, x, res,
                                         active));
+  VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}
+
+int vk_launch_bt_dense(int sl, const VkCol* cols, int ncols, const double* diag,
+                       const double* sup, const double* sub, const double* rhs, double* x,
+                       double* inv, double* rmod, int* status, hipStream_t stream) {
+  VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_bt_dense<SL_>, dim3(ncols), dim3(VK_NT), 0, stream,
+                                        cols, ncols, diag, sup, sub, rhs, x, inv, rmod,
+                                        status));
   VK_CHECK_HIP(hipGetLastError());
   return VK_OK;
 }

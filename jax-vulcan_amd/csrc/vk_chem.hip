@@ -651,6 +651,40 @@ __global__ __launch_bounds__(128) void k_mol_static(const VkCol* __restrict__ co
   }
 }
 
+// compute_diff_grav (jax_step.py:67-147): the y-independent gravity /
+// thermal-diffusion, molecular-advection and settling parts of the
+// molecular-diffusion coefficients, the additive terms of mol_static.
+// A, B, C are (nz, ni) per column at off_s: rows 1..nz-2 hold A/B/C_grav_int,
+// row 0 holds bdry0_grav (A, B), row nz-1 bdry_top_grav (A, C); the entries
+// the reference does not define are zero.
+__global__ __launch_bounds__(128) void k_diff_grav(const VkCol* __restrict__ cols, int ncols,
+                                                   const double* __restrict__ atm,
+                                                   double* __restrict__ A, double* __restrict__ B,
+                                                   double* __restrict__ C) {
+  const int c = blockIdx.y, z = blockIdx.x;
+  if (c >= ncols) return;
+  const VkCol col = cols[c];
+  if (z >= col.nz) return;
+  const AtmView av = atm_view(atm + col.off_a, col.nz, col.ni);
+  for (int i = threadIdx.x; i < col.ni; i += blockDim.x) {
+    double m[8];
+    mol_static(z, col.nz, col.ni, i, av, m);
+    const int64_t e = col.off_s + (int64_t)z * col.ni + i;
+    A[e] = m[5];
+    B[e] = m[6];
+    C[e] = m[7];
+  }
+}
+
+int vk_launch_diff_grav(const VkCol* cols, int ncols, int max_nz, const double* atm, double* A,
+                        double* B, double* C, hipStream_t stream) {
+  if (ncols <= 0 || max_nz <= 0) return VK_OK;
+  hipLaunchKernelGGL(k_diff_grav, dim3(max_nz, ncols), dim3(128), 0, stream, cols, ncols, atm, A,
+                     B, C);
+  VK_CHECK_HIP(hipGetLastError());
+  return VK_OK;
+}
+
 int vk_launch_mol_static(const VkCol* cols, int ncols, int max_nz, const double* atm, double* ms,
                          int64_t stride, const int* active, hipStream_t stream) {
   if (ncols <= 0 || max_nz <= 0) return VK_OK;

@@ -40,6 +40,8 @@ _SIGS = {
     "vk_bt_factor_f64": (i32, [p, p, p, p, i32, p, p, p]),
     "vk_bt_solve_f64": (i32, [p, p, p, p, p, p, p, p]),
     "vk_bt_inverse_natural_f64": (i32, [p, p, p, p]),
+    "vk_bt_dense_f64": (i32, [p, p, p, p, p, p, p, p, p, p]),
+    "vk_diff_grav_f64": (i32, [p, p, p, p, p, p]),
     "vk_ros2_workspace_bytes": (sz, [p]),
     "vk_ros2_step_f64": (i32, [p, p, p, p, p, p, p, p, p, sz, p, p]),
     "vk_ros2_step_noalloc_f64": (i32, [p, p, p, p, p, p, p, p, p, p]),

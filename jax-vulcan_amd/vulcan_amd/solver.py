@@ -1,7 +1,7 @@
 """Block-tridiagonal Thomas solver with diagonal off-diagonal blocks (GPU).
 
-Mirrors solver.py of the reference (solver.py:23-126): factor once, solve
-many right-hand sides.  The factors hold the stack of block inverses
+Mirrors solver.py of the reference (solver.py:23-176): factor once, solve
+many right-hand sides; ``block_thomas`` is the dense-off-diagonal form.  The factors hold the stack of block inverses
 ``inv(A'_j)`` (tile layout, see include/vulcan_ros2.h) instead of LU
 factors and pivots; ``diag_inv_natural`` returns them as (nz, ni, ni).
 A leading batch axis on every argument batches independent systems.
@@ -97,6 +97,44 @@ def solve_block_thomas_diag_offdiag(factors: BlockThomasDiagFactors, rhs):
 def block_thomas_diag_offdiag(diag, sup_d, sub_d, rhs):
     """Factor + solve; solver.py:115-126."""
     return solve_block_thomas_diag_offdiag(factor_block_thomas_diag_offdiag(diag, sup_d, sub_d), rhs)
+
+
+def block_thomas(diag, sup, sub, rhs):
+    """Dense block-tridiagonal Thomas solve (solver.py:129-176).
+
+    Shapes: diag (nz, ni, ni), sup/sub (nz-1, ni, ni), rhs (nz, ni) -> (nz, ni);
+    ``sup[j]`` couples layer j to j+1 and ``sub[j]`` layer j+1 to j.  A
+    leading batch axis on every argument batches independent systems.  One
+    launch factors and solves (explicit inverses of the Schur complements,
+    Gauss-Jordan with partial pivoting); a zero pivot leaves the result
+    non-finite, as the reference's LU does.
+    """
+    diag, sup, sub, rhs = _as(diag), _as(sup), _as(sub), _as(rhs)
+    squeeze = diag.dim() == 3
+    if squeeze:
+        diag, sup, sub, rhs = diag[None], sup[None], sub[None], rhs[None]
+    B, nz, ni, _ = diag.shape
+    if nz < 2:
+        raise ValueError("block Thomas needs nz >= 2")
+    if sup.shape != (B, nz - 1, ni, ni) or sub.shape != (B, nz - 1, ni, ni):
+        raise ValueError(f"sup/sub must be ({B}, {nz - 1}, {ni}, {ni})")
+    if rhs.shape != (B, nz, ni):
+        raise ValueError(f"rhs must be ({B}, {nz}, {ni})")
+    b = uniform_batch(B, nz, _solver_network(ni))
+    sup_p = torch.zeros((B, nz, ni, ni), dtype=F64, device=diag.device)
+    sub_p = torch.zeros((B, nz, ni, ni), dtype=F64, device=diag.device)
+    sup_p[:, :-1] = sup
+    sub_p[:, :-1] = sub
+    diag, rhs = diag.contiguous(), rhs.contiguous()
+    x = torch.empty_like(rhs)
+    scratch = torch.empty_like(rhs)
+    inv = torch.empty(b.blk_len, dtype=F64, device=diag.device)
+    status = torch.zeros(B, dtype=torch.int32, device=diag.device)
+    _lib.check(_lib.load().vk_bt_dense_f64(b.handle, diag.data_ptr(), sup_p.data_ptr(),
+                                           sub_p.data_ptr(), rhs.data_ptr(), x.data_ptr(),
+                                           inv.data_ptr(), scratch.data_ptr(), status.data_ptr(),
+                                           _lib.stream_ptr()), "vk_bt_dense_f64")
+    return x[0] if squeeze else x
 
 
 def diag_inv_natural(factors: BlockThomasDiagFactors) -> torch.Tensor:

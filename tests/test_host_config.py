@@ -59,3 +59,38 @@ def test_config4_rate_tangent_matches_leaf_differences():
         np.testing.assert_allclose(fd, exact, rtol=1e-6, atol=1e-300)
         # other layers do not move
         assert np.array_equal(np.delete(kp, z, axis=1), np.delete(k0, z, axis=1))
+
+
+def test_make_atm_static_follows_the_configuration():
+    """jax_step.make_atm_static (jax_step.py:318-351): flags from the run
+    configuration, molecular-diffusion leaves zeroed where it switches them
+    off, the gas mask from gas_indx."""
+    import types
+    import numpy as np
+    from vulcan_amd import jax_step
+    rng = np.random.default_rng(2)
+    nz, ni = 6, 4
+    legacy = types.SimpleNamespace(
+        Kzz=rng.random(nz - 1), Dzz=rng.random((nz - 1, ni)), dzi=rng.random(nz - 1),
+        vz=rng.random(nz - 1), Hpi=rng.random(nz - 1), Ti=rng.random(nz - 1),
+        Tco=rng.random(nz), g=rng.random(nz), ms=rng.random(ni), alpha=rng.random(ni),
+        M=rng.random(nz), vm=rng.random((nz, ni)), vs=rng.random((nz - 1, ni)),
+        top_flux=rng.random(ni), bot_flux=rng.random(ni), bot_vdep=rng.random(ni),
+        gas_indx=[0, 2, 3])
+    a = jax_step.make_atm_static(legacy, ni, nz, cfg=types.SimpleNamespace())
+    assert not (a.use_vm_mol or a.use_settling or a.use_topflux or a.use_botflux)
+    assert np.array_equal(a.Dzz, legacy.Dzz) and not a.vm.any() and not a.vs.any()
+    assert a.gas_indx_mask.tolist() == [True, False, True, True]
+    assert a.vm.shape == (nz, ni) and a.vs.shape == (nz - 1, ni)
+    cfg = types.SimpleNamespace(use_vm_mol=True, use_settling=True, use_topflux=True,
+                                use_botflux=True)
+    a = jax_step.make_atm_static(legacy, ni, nz, cfg=cfg)
+    assert a.use_vm_mol and a.use_settling and a.use_topflux and a.use_botflux
+    assert np.array_equal(a.vm, legacy.vm) and np.array_equal(a.vs, legacy.vs)
+    # molecular diffusion off: Dzz, vm and vs zeroed, vm / settling flags off
+    cfg.use_moldiff = False
+    a = jax_step.make_atm_static(legacy, ni, nz, cfg=cfg)
+    assert not (a.use_vm_mol or a.use_settling) and a.use_topflux and a.use_botflux
+    assert not (a.Dzz.any() or a.vm.any() or a.vs.any())
+    for f in ("Kzz", "dzi", "Tco", "M", "bot_vdep"):
+        assert np.array_equal(getattr(a, f), getattr(legacy, f))
