@@ -94,3 +94,33 @@ def test_make_atm_static_follows_the_configuration():
     assert not (a.Dzz.any() or a.vm.any() or a.vs.any())
     for f in ("Kzz", "dzi", "Tco", "M", "bot_vdep"):
         assert np.array_equal(getattr(a, f), getattr(legacy, f))
+
+
+def test_photo_data_from_static_selects_rows_and_indices():
+    """photo.photo_data_from_static (photo.py:35-81): absorbers with a
+    temperature-dependent cross section leave the plain table, the T table
+    passes through, indices follow species_list."""
+    import types
+    import numpy as np
+    from vulcan_amd import photo
+    species = ["H2", "H2O", "CH4", "CO", "CO2", "NH3", "N2", "He"]
+    nbin, nz = 7, 3
+    rng = np.random.default_rng(1)
+    static = types.SimpleNamespace(
+        nbin=nbin, absp_sp=["H2O", "CH4", "CO2", "NH3"], absp_cross=rng.random((4, nbin)),
+        absp_T_sp=["CO2"], absp_T_cross=rng.random((1, nz, nbin)),
+        scat_sp=["H2", "He"], scat_cross=rng.random((2, nbin)))
+    pd = photo.photo_data_from_static(static, species)
+    assert pd.absp_idx.tolist() == [1, 2, 5]
+    assert np.array_equal(pd.absp_cross, static.absp_cross[[0, 1, 3]])
+    assert pd.absp_T_idx.tolist() == [4]
+    assert np.array_equal(pd.absp_T_cross, static.absp_T_cross)
+    assert pd.scat_idx.tolist() == [0, 7]
+    assert np.array_equal(pd.scat_cross, static.scat_cross)
+    empty = types.SimpleNamespace(nbin=nbin, absp_sp=[], absp_cross=np.zeros((0, nbin)),
+                                  absp_T_sp=[], absp_T_cross=np.zeros((0, nz, nbin)),
+                                  scat_sp=[], scat_cross=np.zeros((0, nbin)))
+    pd = photo.photo_data_from_static(empty, species)
+    assert pd.absp_idx.size == 0 and pd.absp_cross.shape == (0, nbin)
+    assert pd.absp_T_idx.size == 0 and pd.absp_T_cross.shape == (0, nz, nbin)
+    assert pd.scat_idx.size == 0 and pd.scat_cross.shape == (0, nbin)
