@@ -269,6 +269,11 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
  * chemistry and sweeps; measured on MI355X the factor then slows more than
  * the overlap gains.  Results are bitwise independent of the split. */
 #define VK_TUNE_SPLIT 3
+/* Knob VK_TUNE_KDIRECT (sl ignored): 1 has vk_ros2_step_f64 and the
+ * steady-state attempt read the rate tables k in their (S, nz) layout from
+ * the chemistry launches; 0 (default) first copies them layer-major (one
+ * contiguous row per layer).  Results are bitwise equal. */
+#define VK_TUNE_KDIRECT 4
 int vk_tune_set(int knob, int sl, int value);
 int vk_tune_get(int knob, int sl); /* -1 for an unknown knob / bucket */
 

@@ -968,6 +968,10 @@ static bool ensure_aux(vk_batch b, size_t n) {
 static int g_split = 1;
 constexpr int kMinChunk = 128;
 
+// VK_TUNE_KDIRECT: 1 has the chemistry read the rate tables in their (S, nz)
+// layout; 0 (default) copies them layer-major once per step first.
+static int g_k_direct = 0;
+
 static int ros2_step_impl(vk_batch b, const double* y, const double* k, const double* atm,
                           const double* dt, const uint8_t* fix_mask, double* sol, double* delta,
                           void* ws, int32_t* status, const int* active, hipStream_t st,
@@ -996,12 +1000,16 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
   const double* ref2 = g_refine >= 2 ? blk : nullptr;
   double* schur = (double*)((char*)ws + ros2_ws_schur_offset(b));
   int* refl = (int*)((char*)ws + ros2_ws_refl_offset(b));
+  const double* kchem = g_k_direct ? k : kT;  // the rate tables the chemistry reads
+  const int klm = g_k_direct ? 0 : 1;
   const double r = ros2_gamma();
   {
     KernelTimer t(6, st);
-    if (int rc = vk_launch_rates_layer_major(b->d_cols, b->ncols, b->max_S, b->max_nz, k, kT,
-                                             active, st))
-      return rc;
+    if (!g_k_direct) {
+      if (int rc = vk_launch_rates_layer_major(b->d_cols, b->ncols, b->max_S, b->max_nz, k, kT,
+                                               active, st))
+        return rc;
+    }
     if (g_mol_precompute) {
       if (int rc = vk_launch_mol_static(b->d_cols, b->ncols, b->max_nz, atm, mst, mstride,
                                         active, st))
@@ -1024,8 +1032,8 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
     {
       KernelTimer t(0, s);
       if ((rc = vk_launch_chem_layer(sl, M_LHS, cols, nc, b->max_nz, b->max_S,
-                                     b->max_T, b->d_nets, y, nullptr, atm, kT, dtc, fix_mask,
-                                     nullptr, rhs_y, blk, sup, sub, act, 1, mst, mstride, s)))
+                                     b->max_T, b->d_nets, y, nullptr, atm, kchem, dtc, fix_mask,
+                                     nullptr, rhs_y, blk, sup, sub, act, klm, mst, mstride, s)))
         return rc;
     }
     // The factor fuses the stage-1 forward sweep unless the blocked variant
@@ -1093,8 +1101,9 @@ static int ros2_step_impl(vk_batch b, const double* y, const double* k, const do
     {
       KernelTimer t(3, s);
       if ((rc = vk_launch_chem_layer(sl, M_RHS2, cols, nc, b->max_nz, b->max_S,
-                                     b->max_T, b->d_nets, yk2, nullptr, atm, kT, dtc, fix_mask, k1,
-                                     rhs2, nullptr, nullptr, nullptr, act, 1, mst, mstride, s)))
+                                     b->max_T, b->d_nets, yk2, nullptr, atm, kchem, dtc, fix_mask,
+                                     k1, rhs2, nullptr, nullptr, nullptr, act, klm, mst, mstride,
+                                     s)))
         return rc;
     }
     SolveEpi e2{2, 3.0 / (2.0 * r), 1.0 / (2.0 * r), y, k1, yk2, sol, delta};
@@ -1385,6 +1394,14 @@ int vk_tune_set(int knob, int sl, int value) {
     g_refine = value;
     return VK_OK;
   }
+  if (knob == VK_TUNE_KDIRECT) {
+    if (value != 0 && value != 1) {
+      vk_set_error("vk_tune_set: VK_TUNE_KDIRECT takes 0 or 1, not %d", value);
+      return VK_ERR_ARG;
+    }
+    g_k_direct = value;
+    return VK_OK;
+  }
   if (knob == VK_TUNE_MOLPRE) {
     if (value != 0 && value != 1) {
       vk_set_error("vk_tune_set: VK_TUNE_MOLPRE takes 0 or 1, not %d", value);
@@ -1408,6 +1425,7 @@ int vk_tune_set(int knob, int sl, int value) {
 }
 int vk_tune_get(int knob, int sl) {
   if (knob == VK_TUNE_MOLPRE) return g_mol_precompute;
+  if (knob == VK_TUNE_KDIRECT) return g_k_direct;
   if (knob == VK_TUNE_REFINE) return g_refine;
   if (knob == VK_TUNE_SPLIT) return g_split;
   if (knob != VK_TUNE_FACTOR || sl < 1 || sl > 7) return -1;

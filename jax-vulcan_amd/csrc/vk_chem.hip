@@ -272,8 +272,19 @@ __global__ __launch_bounds__(NT) void k_chem_layer(LayerArgs L) {
   __shared__ double dd_s[VK_G * SL];  // A+Ai (+ bottom vdep for M_LHS) per species
   __shared__ double bt_s[VK_G * SL];  // bottom vdep term, kept apart for M_JY
   __shared__ int fix_s[VK_G * SL];    // M_LHS: row i of layer z is fixed
-  const int c = blockIdx.y;
-  const int z = blockIdx.x;
+  // Workgroups go to the 8 XCDs round-robin by linear id; the mapping below
+  // gives each XCD a contiguous run of (column, layer) work items, so
+  // neighbouring layers of a column share one L2 (the y rows of z-1 .. z+1,
+  // and the cache lines of a rate table read in its (S, nz) layout).
+  int z, c;
+  {
+    const int64_t W = (int64_t)gridDim.x * gridDim.y;
+    const int64_t lin = blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
+    const int64_t per = W / 8;
+    const int64_t g = (lin < 8 * per) ? (lin % 8) * per + lin / 8 : lin;
+    z = (int)(g % gridDim.x);
+    c = (int)(g / gridDim.x);
+  }
   if (c >= L.ncols) return;
   if (L.active && (L.active[c] & VK_ST_DONE)) return;
   const VkCol col = L.cols[c];
