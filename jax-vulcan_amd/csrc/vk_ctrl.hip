@@ -11,8 +11,9 @@
 // accept_count > count_min (the "ready" gate of cond_fn, :706-711), or
 // count_max accepted steps.
 //
-// One 256-thread workgroup per column; layer-parallel passes with fixed
-// reduction trees, so every column's result is independent of the batch.
+// One 256-thread workgroup per column, one layer per wave at a time, with
+// fixed reduction trees, so every column's result is independent of the
+// batch.
 #include "vk_common.h"
 
 namespace {
@@ -45,6 +46,13 @@ __device__ inline double nan_clip(double x, double lo, double hi) {
 }
 struct OrI { __device__ int operator()(int a, int b) const { return a | b; } };
 
+// Sum over the 64 lanes of a wave by a fixed butterfly; every lane returns
+// lane 0's total, so the result is one rounding sequence for the wave.
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return __shfl(v, 0, 64);
+}
+
 }  // namespace
 
 __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ cols, int ncols,
@@ -68,7 +76,15 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
   const double* dar = s.delta_arr + col.off_s;
   const double* compo = s.compo + col.off_s;  // (ni, na)
 
-  // ---- pass 1: clip, ymix_new, delta, sign flags, atom sums ----------------
+  // Every pass takes one layer per wave (layers wave, wave + 4, ...) and
+  // species lane, lane + 64 of it, so loads are coalesced along the row and a
+  // layer's sums are one fixed shuffle tree; the same thread owns the same
+  // (layer, species) cells in every pass.
+  const int lane = tid & 63, wave = tid >> 6;
+  constexpr int NW = NT / 64;
+  constexpr int PER = 2;   // species per lane: ni <= 128
+
+  // ---- pass 1: clip, delta, sign flags, atom sums ---------------------------
   double dmax = 0.0;
   // 1: a value after the clip that is not >= 0 (NaN included: all(sol_clip >= 0)
   //    fails on NaN, outer_loop.py:747), 2: a non-finite value,
@@ -76,22 +92,20 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
   int flags = 0;
   double at_clip[VK_MAX_ATOMS];
   for (int a = 0; a < VK_MAX_ATOMS; ++a) at_clip[a] = 0.0;
-  for (int z = tid; z < nz; z += NT) {
-    double ys = 0.0;
-    for (int i = 0; i < ni; ++i) {
+  for (int z = wave; z < nz; z += NW) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = lane + 64 * u;
+      if (i >= ni) continue;
       const int64_t e = (int64_t)z * ni + i;
-      const double sc = clip_val(sol[e], ymix[e], p);
-      if (gas[i] != 0.0) ys += sc;
-    }
-    for (int i = 0; i < ni; ++i) {
-      const int64_t e = (int64_t)z * ni + i;
-      const double sc = clip_val(sol[e], ymix[e], p);
+      const double sv = sol[e], mo = ymix[e];
+      const double sc = clip_val(sv, mo, p);
       if (!(sc >= 0.0)) flags |= 1;
       if (sc < 0.0) flags |= 4;
-      if (!isfinite(sol[e])) flags |= 2;
+      if (!isfinite(sv)) flags |= 2;
       for (int a = 0; a < na; ++a) at_clip[a] += compo[i * na + a] * sc;
       // aggregated delta (outer_loop.py:238-248) with the pre-step ymix
-      double m = (ymix[e] < p.mtol) ? 0.0 : dar[e];
+      double m = (mo < p.mtol) ? 0.0 : dar[e];
       if (sc < p.atol) m = 0.0;
       if (p.zero_bot_row && z == 0) m = 0.0;
       if (sc > 0.0) dmax = nan_max(dmax, m / fmax(fabs(sc), 1e-300));
@@ -129,9 +143,10 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
   double* pin = s.pin ? s.pin + s.off_pin[c] : nullptr;
   const int pflags = pin ? (int)pin[0] : 0;
   const double t_old = s.t[c];
-  for (int z = tid; z < nz; z += NT) {
+  for (int z = wave; z < nz; z += NW) {
     // H2/He pin (outer_loop.py:877-882): the snapshot trips on any attempt
-    // with t > pin_time, from the bottom layer's pre-step ymix
+    // with t > pin_time, from the bottom layer's pre-step ymix (every lane
+    // forms the same values; lane 0 stores them)
     bool h2he_apply = false;
     double h2m = 0.0, hem = 0.0;
     int h2 = -1, he = -1;
@@ -144,59 +159,83 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
         const bool trip = !pinned && t_old > hp[2];
         h2m = trip ? ymix[h2] : hp[4];
         hem = trip ? ymix[he] : hp[5];
-        hp[3] = (pinned || trip) ? 1.0 : 0.0;
-        hp[4] = h2m;
-        hp[5] = hem;
+        if (lane == 0) {
+          hp[3] = (pinned || trip) ? 1.0 : 0.0;
+          hp[4] = h2m;
+          hp[5] = hem;
+        }
         h2he_apply = pinned || trip;
       }
     }
+    double sc[PER], yv[PER];
     double ys = 0.0;
-    for (int i = 0; i < ni; ++i) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = lane + 64 * u;
+      sc[u] = 0.0;
+      if (i >= ni) continue;
       const int64_t e = (int64_t)z * ni + i;
-      const double sc = clip_val(sol[e], ymix[e], p);
-      if (gas[i] != 0.0) ys += sc;
+      sc[u] = clip_val(sol[e], ymix[e], p);
+      if (gas[i] != 0.0) ys += sc[u];
     }
     double ysb = 0.0;
     if (accept) {
+      ys = wave_sum(ys);
       // hydrostatic balance of the gas species (outer_loop.py:847-853); the
-      // other species keep their clipped value.  Written to y first so the
-      // constraints can act on the balanced layer.
-      for (int i = 0; i < ni; ++i) {
-        const int64_t e = (int64_t)z * ni + i;
-        const double sc = clip_val(sol[e], ymix[e], p);
-        y[e] = (gas[i] != 0.0) ? n0[z] * (sc / ys) : sc;
+      // other species keep their clipped value, then the constraints act on
+      // the balanced layer
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int i = lane + 64 * u;
+        yv[u] = (i < ni && gas[i] != 0.0) ? n0[z] * (sc[u] / ys) : sc[u];
       }
       if (pflags) {
-        const int64_t row = (int64_t)z * ni;
         if (pflags & VK_PIN_CHARGE) {   // e = -sum_i charge_i y_i (outer_loop.py:855-860)
           const int eidx = (int)pin[1];
           double q = 0.0;
-          for (int i = 0; i < ni; ++i) q = q + y[row + i] * pin[2 + i];
-          if (eidx >= 0 && eidx < ni) y[row + eidx] = -q;
-        }
-        if (z == 0 && (pflags & VK_PIN_ALL_BOT))   // outer_loop.py:864-865
-          for (int i = 0; i < ni; ++i) y[i] = pin[2 + ni + i];
-        if (z == 0 && (pflags & VK_PIN_SP_BOT))    // outer_loop.py:866-869
-          for (int i = 0; i < ni; ++i) {
-            const double m = pin[2 + 2 * ni + i];
-            if (m == m) y[i] = m * n0[0];
+#pragma unroll
+          for (int u = 0; u < PER; ++u) {
+            const int i = lane + 64 * u;
+            if (i < ni) q = q + yv[u] * pin[2 + i];
           }
-        if (h2he_apply) {                            // outer_loop.py:884-892
-          y[h2] = h2m * n0[0];
-          y[he] = hem * n0[0];
+          q = wave_sum(q);
+#pragma unroll
+          for (int u = 0; u < PER; ++u)
+            if (lane + 64 * u == eidx) yv[u] = -q;
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+          const int i = lane + 64 * u;
+          if (i >= ni || z != 0) continue;
+          if (pflags & VK_PIN_ALL_BOT) yv[u] = pin[2 + ni + i];    // outer_loop.py:864-865
+          if (pflags & VK_PIN_SP_BOT) {                             // outer_loop.py:866-869
+            const double m = pin[2 + 2 * ni + i];
+            if (m == m) yv[u] = m * n0[0];
+          }
+          if (h2he_apply) {                                         // outer_loop.py:884-892
+            if (i == h2) yv[u] = h2m * n0[0];
+            if (i == he) yv[u] = hem * n0[0];
+          }
         }
       }
-      for (int i = 0; i < ni; ++i)
-        if (gas[i] != 0.0) ysb += y[(int64_t)z * ni + i];
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int i = lane + 64 * u;
+        if (i < ni && gas[i] != 0.0) ysb += yv[u];
+      }
+      ysb = wave_sum(ysb);
     }
-    for (int i = 0; i < ni; ++i) {
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = lane + 64 * u;
+      if (i >= ni) continue;
       const int64_t e = (int64_t)z * ni + i;
       const double yprev = yp[e];
       double ynew, mixnew = ymix[e];
       if (force) {
         ynew = yprev < 0.0 ? 0.0 : yprev;
       } else if (accept) {
-        ynew = y[e];
+        ynew = yv[u];
         mixnew = ynew / ysb;
         for (int a = 0; a < na; ++a) at_b[a] += compo[i * na + a] * ynew;
       } else {
@@ -259,8 +298,11 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
     const double* yold = ry + (int64_t)indx * nz * ni;
     double* ynew_slot = ry + (int64_t)slot * nz * ni;
     double lmax = 0.0, smin = INFINITY;
-    for (int z = tid; z < nz; z += NT) {
-      for (int i = 0; i < ni; ++i) {
+    for (int z = wave; z < nz; z += NW) {
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int i = lane + 64 * u;
+        if (i >= ni) continue;
         const int64_t e = (int64_t)z * ni + i;
         const double yv = y[e], mv = ymix[e];
         double ld = fabs((yv - yold[e]) / n0[z]);
@@ -269,7 +311,7 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
         lmax = r > lmax ? r : lmax;
         ynew_slot[e] = yv;
       }
-      if (z + 1 < nz) {
+      if (lane == 0 && z + 1 < nz) {
         const double h = 0.1 * Hp[z];
         const double q = Kzz[z] / (h * h);
         smin = q < smin ? q : smin;

@@ -13,8 +13,9 @@
 //   dflux_u (photo.py:141-145, 210-217); the up sweep recomputes the layer
 //   coefficients from tau and w0 and forms the actinic flux.
 // k_photo_J: the contraction J = cross_J (n_br x nbin) . aflux^T (nbin x nz)
-//   on FP64 MFMA, one workgroup per column: each wave owns 16-layer tiles and
-//   computes every 16-branch tile of a 64-branch group over all bins.  The
+//   on FP64 MFMA, one workgroup per (16-layer tile, column): its waves split
+//   the bins and every 16-branch tile of a 64-branch group; the slab sums
+//   are added in a fixed order.  The
 //   two trapezoid spacings ride in the flux operand (af * dbin), so one
 //   accumulator per tile holds both partial sums; the half end-point terms
 //   are subtracted after.  J goes to the state and, times f_diurnal, into
@@ -202,65 +203,86 @@ __global__ __launch_bounds__(NT) void k_photo_flux(const VkCol* __restrict__ col
 // element (lr + 4v, lc).  Here A = cross_J rows (branches), B = the flux
 // weighted by the bin spacing, k = bins, columns = layers.
 typedef double ph_d4 __attribute__((ext_vector_type(4)));
-constexpr int J_TILES = 2;   // 16-layer tiles per wave and pass
 
+// One workgroup per (16-layer tile, column): its four waves take four
+// contiguous slabs of the bin chunks for every 16-branch tile of a 64-branch
+// group, two chunks in flight per iteration, and the slab sums are added in
+// wave order through LDS.
 __global__ __launch_bounds__(NT) void k_photo_J(const VkCol* __restrict__ cols, int ncols,
                                                 vk_photo_setup P, vk_photo_state S) {
-  const int c = blockIdx.x;
+  __shared__ ph_d4 part[NT / 64][4][64];
+  const int c = blockIdx.y, t0 = blockIdx.x;
   if (c >= ncols) return;
   if (S.due && !S.due[c]) return;
   if (S.status && (S.status[c] & VK_ST_DONE)) return;
   const VkCol col = cols[c];
   const int nz = col.nz, nbin = P.nbin, d12 = P.din12, n_br = P.n_br;
+  if (t0 * 16 >= nz) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane >> 4, lc = lane & 15;
   const double* af = S.aflux + P.off_p[c];
   const double* cr = P.cross_J;
-  const int ntile = (nz + 15) / 16;
   const int nchunk = (nbin + 3) / 4;
+  const int ch0 = (int)((int64_t)nchunk * wave / (NT / 64));
+  const int ch1 = (int)((int64_t)nchunk * (wave + 1) / (NT / 64));
+  const int zrow = min(t0 * 16 + lc, nz - 1);
+  const bool okz = t0 * 16 + lc < nz;
   for (int m0 = 0; m0 < n_br; m0 += 64) {
-    for (int t0 = wave; t0 < ntile; t0 += 4 * J_TILES) {
-      ph_d4 acc[J_TILES][4];
+    ph_d4 acc[4];
 #pragma unroll
-      for (int u = 0; u < J_TILES; ++u)
+    for (int mt = 0; mt < 4; ++mt) acc[mt] = ph_d4{0.0, 0.0, 0.0, 0.0};
+    // this lane's rows: branches m0 + 16 mt + lc (A) and layer t0 * 16 + lc (B)
+    int arow[4];
+    bool okb[4];
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt) acc[u][mt] = ph_d4{0.0, 0.0, 0.0, 0.0};
-      // this lane's rows: branches m0 + 16 mt + lc (A) and layers of each tile (B)
-      int arow[4], zrow[J_TILES];
+    for (int mt = 0; mt < 4; ++mt) {
+      arow[mt] = min(m0 + 16 * mt + lc, n_br - 1);
+      okb[mt] = m0 + 16 * mt + lc < n_br;
+    }
+    auto chunk = [&](int ch, double (&av)[4], double& bw) {
+      const int bb = 4 * ch + lr;
+      const bool inb = bb < nbin;
+      const int b = inb ? bb : nbin - 1;
+      const double wgt = inb ? (b < d12 ? P.dbin1 : P.dbin2) : 0.0;
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt) arow[mt] = min(m0 + 16 * mt + lc, n_br - 1);
-#pragma unroll
-      for (int u = 0; u < J_TILES; ++u) zrow[u] = min((t0 + 4 * u) * 16 + lc, nz - 1);
+      for (int mt = 0; mt < 4; ++mt) av[mt] = okb[mt] ? cr[(int64_t)arow[mt] * nbin + b] : 0.0;
+      bw = okz ? af[(int64_t)zrow * nbin + b] * wgt : 0.0;
+    };
+    int ch = ch0;
 #pragma unroll 1
-      for (int ch = 0; ch < nchunk; ++ch) {
-        const int bb = 4 * ch + lr;
-        const bool inb = bb < nbin;
-        const int b = inb ? bb : nbin - 1;
-        const double wgt = inb ? (b < d12 ? P.dbin1 : P.dbin2) : 0.0;
-        double av[4];
+    for (; ch + 1 < ch1; ch += 2) {
+      double a0[4], a1[4], b0, b1;
+      chunk(ch, a0, b0);
+      chunk(ch + 1, a1, b1);
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-          const bool ok = m0 + 16 * mt + lc < n_br;
-          av[mt] = ok ? cr[(int64_t)arow[mt] * nbin + b] : 0.0;
+      for (int mt = 0; mt < 4; ++mt)
+        acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[mt], b0, acc[mt], 0, 0, 0);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+        acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[mt], b1, acc[mt], 0, 0, 0);
+    }
+    if (ch < ch1) {
+      double a0[4], b0;
+      chunk(ch, a0, b0);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+        acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[mt], b0, acc[mt], 0, 0, 0);
+    }
+    __syncthreads();   // the previous group's sums are read
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) part[wave][mt][lane] = acc[mt];
+    __syncthreads();
+    if (wave == 0) {
+      const int z = t0 * 16 + lc;
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        ph_d4 tot = part[0][mt][lane];
+#pragma unroll
+        for (int w = 1; w < NT / 64; ++w) tot = tot + part[w][mt][lane];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int br = m0 + 16 * mt + lr + 4 * v;
+          if (z < nz && br < n_br) S.J[P.off_l[c] * n_br + (int64_t)br * nz + z] = tot[v];
         }
-#pragma unroll
-        for (int u = 0; u < J_TILES; ++u) {
-          const bool okz = (t0 + 4 * u) * 16 + lc < nz;
-          const double bw = okz ? af[(int64_t)zrow[u] * nbin + b] * wgt : 0.0;
-#pragma unroll
-          for (int mt = 0; mt < 4; ++mt)
-            acc[u][mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mt], bw, acc[u][mt], 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < J_TILES; ++u) {
-        const int z = (t0 + 4 * u) * 16 + lc;
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int br = m0 + 16 * mt + lr + 4 * v;
-            if (z < nz && br < n_br) S.J[P.off_l[c] * n_br + (int64_t)br * nz + z] = acc[u][mt][v];
-          }
       }
     }
   }
@@ -371,7 +393,8 @@ int vk_launch_photo(int parts, const VkCol* cols, int ncols, int max_nz,
   }
   if (parts & 4) {
     if (P.n_br > 0) {
-      hipLaunchKernelGGL(k_photo_J, dim3(ncols), dim3(NT), 0, st, cols, ncols, P, S);
+      hipLaunchKernelGGL(k_photo_J, dim3((max_nz + 15) / 16, ncols), dim3(NT), 0, st, cols,
+                         ncols, P, S);
       VK_CHECK_HIP(hipGetLastError());
       hipLaunchKernelGGL(k_photo_J_ends, dim3((P.n_br * max_nz + NT - 1) / NT, ncols), dim3(NT), 0,
                          st, cols, ncols, P, S, k);
