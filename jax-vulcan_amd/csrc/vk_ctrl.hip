@@ -11,7 +11,7 @@
 // accept_count > count_min (the "ready" gate of cond_fn, :706-711), or
 // count_max accepted steps.
 //
-// One 256-thread workgroup per column, one layer per wave at a time, with
+// One 1024-thread workgroup per column, one layer per wave at a time, with
 // fixed reduction trees, so every column's result is independent of the
 // batch.
 #include "vk_common.h"
@@ -24,14 +24,17 @@ __device__ inline double clip_val(double v, double ymix_old, const vk_ss_params&
   return vk_clip_val(v, ymix_old, p);
 }
 
-template <typename T, typename Op>
+// Control kernel workgroup: 16 waves, one layer per wave at a time.
+constexpr int NTC = 1024;
+
+template <int NTB = NT, typename T, typename Op>
 __device__ inline T block_reduce(T v, T* red, Op op) {
   for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   T r = red[0];
-  for (int w = 1; w < NT / 64; ++w) r = op(r, red[w]);
+  for (int w = 1; w < NTB / 64; ++w) r = op(r, red[w]);
   return r;
 }
 
@@ -55,11 +58,11 @@ __device__ inline double wave_sum(double v) {
 
 }  // namespace
 
-__global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ cols, int ncols,
+__global__ __launch_bounds__(NTC) void k_ss_control(const VkCol* __restrict__ cols, int ncols,
                                                     const double* __restrict__ atm,
                                                     vk_ss_params p, vk_ss_state s) {
-  __shared__ double redd[NT / 64];
-  __shared__ int redi[NT / 64];
+  __shared__ double redd[NTC / 64];
+  __shared__ int redi[NTC / 64];
   const int c = blockIdx.x;
   if (c >= ncols) return;
   if (s.status[c] & VK_ST_DONE) return;
@@ -81,7 +84,7 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
   // layer's sums are one fixed shuffle tree; the same thread owns the same
   // (layer, species) cells in every pass.
   const int lane = tid & 63, wave = tid >> 6;
-  constexpr int NW = NT / 64;
+  constexpr int NW = NTC / 64;
   constexpr int PER = 2;   // species per lane: ni <= 128
 
   // ---- pass 1: clip, delta, sign flags, atom sums ---------------------------
@@ -111,9 +114,9 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
       if (sc > 0.0) dmax = nan_max(dmax, m / fmax(fabs(sc), 1e-300));
     }
   }
-  dmax = block_reduce(dmax, redd, NanMaxD());
-  flags = block_reduce(flags, redi, OrI());
-  for (int a = 0; a < na; ++a) at_clip[a] = block_reduce(at_clip[a], redd, AddD());
+  dmax = block_reduce<NTC>(dmax, redd, NanMaxD());
+  flags = block_reduce<NTC>(flags, redi, OrI());
+  for (int a = 0; a < na; ++a) at_clip[a] = block_reduce<NTC>(at_clip[a], redd, AddD());
 
   // ---- accept / reject (outer_loop.py:741-768) -------------------------------
   double* al = s.atom_loss + (int64_t)c * VK_MAX_ATOMS;
@@ -256,13 +259,13 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
       ymix[e] = mixnew;
     }
   }
-  relmax = block_reduce(relmax, redd, MaxD());
+  relmax = block_reduce<NTC>(relmax, redd, MaxD());
   if (accept)
-    for (int a = 0; a < na; ++a) at_b[a] = block_reduce(at_b[a], redd, AddD());
+    for (int a = 0; a < na; ++a) at_b[a] = block_reduce<NTC>(at_b[a], redd, AddD());
 
   // ---- pass 3 (conv_mode 1): the reference's longdy (outer_loop.py:636-670)
-  __shared__ double cv_d[NT];
-  __shared__ int cv_i[NT];
+  __shared__ double cv_d[NTC / 64];
+  __shared__ int cv_i[NTC / 64];
   double longdy = 0.0, slope_min = 0.0;
   int indx = 0, slot = 0;
   const bool ring = p.conv_mode == 1 && do_accept;
@@ -275,17 +278,25 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
     const double target = t_new * p.st_factor;
     double bd = INFINITY;
     int bi = 0x7fffffff;
-    for (int i = tid; i < p.conv_step; i += NT) {
+    for (int i = tid; i < p.conv_step; i += NTC) {
       const double d = (i == slot) ? INFINITY : fabs(rt[i] - target);
       if (d < bd || (d == bd && i < bi)) { bd = d; bi = i; }
     }
-    cv_d[tid] = bd;
-    cv_i[tid] = bi;
+    // (distance, index) minimum, smallest index on ties: per wave, then waves
+    for (int off = 32; off > 0; off >>= 1) {
+      const double od = __shfl_xor(bd, off, 64);
+      const int oi = __shfl_xor(bi, off, 64);
+      if (od < bd || (od == bd && oi < bi)) { bd = od; bi = oi; }
+    }
+    if (lane == 0) {
+      cv_d[wave] = bd;
+      cv_i[wave] = bi;
+    }
     __syncthreads();
     if (tid == 0) {
       double d0 = cv_d[0];
       int i0 = cv_i[0];
-      for (int w = 1; w < NT; ++w)
+      for (int w = 1; w < NW; ++w)
         if (cv_d[w] < d0 || (cv_d[w] == d0 && cv_i[w] < i0)) { d0 = cv_d[w]; i0 = cv_i[w]; }
       cv_i[0] = (i0 == 0x7fffffff) ? 0 : i0;
     }
@@ -317,8 +328,8 @@ __global__ __launch_bounds__(NT) void k_ss_control(const VkCol* __restrict__ col
         smin = q < smin ? q : smin;
       }
     }
-    longdy = block_reduce(lmax, redd, MaxD());
-    smin = -block_reduce(-smin, redd, MaxD());
+    longdy = block_reduce<NTC>(lmax, redd, MaxD());
+    smin = -block_reduce<NTC>(-smin, redd, MaxD());
     slope_min = fmax(fmin(smin, 1e-8), 1e-10);   // outer_loop.py:679-683
   }
 
@@ -491,7 +502,7 @@ __global__ __launch_bounds__(NT) void k_ss_init(const VkCol* __restrict__ cols, 
 
 int vk_launch_ss_control(const VkCol* cols, int ncols, const double* atm, const vk_ss_params& p,
                          const vk_ss_state& s, hipStream_t st) {
-  hipLaunchKernelGGL(k_ss_control, dim3(ncols), dim3(NT), 0, st, cols, ncols, atm, p, s);
+  hipLaunchKernelGGL(k_ss_control, dim3(ncols), dim3(NTC), 0, st, cols, ncols, atm, p, s);
   VK_CHECK_HIP(hipGetLastError());
   return VK_OK;
 }
