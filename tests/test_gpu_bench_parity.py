@@ -104,3 +104,18 @@ def test_ros2_step_unrefined_block_solves(bench_cols, refs):
     for b, r in enumerate(refs):
         print(r.judge(sol[b], f"column {b}, unrefined")[3])
         assert layer_err(sol[b], r.np).max() <= 10.0 * r.layer_spread.max() + 1e-10
+
+
+def test_rate_table_layouts_bitwise_equal(bench_cols):
+    """VK_TUNE_KDIRECT: the chemistry reading k in its (S, nz) layout gives
+    the same step, bit for bit, as reading the layer-major copy."""
+    from vulcan_amd import _lib
+    lib = _lib.load()
+    VK_TUNE_KDIRECT = 4
+    base = _step(bench_cols, FACTOR_TILE, 2)
+    _lib.check(lib.vk_tune_set(VK_TUNE_KDIRECT, 0, 1), "vk_tune_set")
+    try:
+        direct = _step(bench_cols, FACTOR_TILE, 2)
+    finally:
+        lib.vk_tune_set(VK_TUNE_KDIRECT, 0, 0)
+    assert np.array_equal(base[0], direct[0]) and np.array_equal(base[1], direct[1])
