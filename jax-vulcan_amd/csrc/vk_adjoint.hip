@@ -18,6 +18,10 @@
 //    (jax_step.py:67-246) in dual numbers seeded at e and sums lam * tangent.
 //    Integer-valued leaves (gas mask, mode flags) get a zero cotangent, as
 //    the reference's float0 leaves do.  cot_atm has the record layout.
+//    The "wide" elements, whose rows are many (a per-layer scalar reaches
+//    every species of three layers, ms and alpha every layer of a species),
+//    are left to k_diff_vjp_wide: one workgroup per element, the rows spread
+//    over its threads and summed by a fixed tree.
 #include "vk_common.h"
 
 namespace {
@@ -126,51 +130,22 @@ __device__ Dl diff_row_d(const Seeded& A, const VkAtmOff& o, int nz, int ni, int
 
 }  // namespace
 
-__global__ __launch_bounds__(VK_NT) void k_diff_vjp(const VkCol* __restrict__ cols, int ncols,
-                                                    const double* __restrict__ y,
-                                                    const double* __restrict__ atm,
-                                                    const double* __restrict__ lam,
-                                                    double* __restrict__ cot_atm) {
-  extern __shared__ double ysum[];
-  const int c = blockIdx.y;
-  if (c >= ncols) return;
-  const VkCol col = cols[c];
-  const int nz = col.nz, ni = col.ni;
-  const VkAtmOff o = vk_atm_offsets(nz, ni);
-  const double* rec = atm + col.off_a;
-  const double* yc = y + col.off_s;
-  const double* lc = lam + col.off_s;
-  // ysum over gas species (jax_step.py:165-167), one layer per thread
-  for (int z = threadIdx.x; z < nz; z += VK_NT) {
-    double s = 0.0;
-    for (int i = 0; i < ni; ++i)
-      if (rec[o.gas + i] != 0.0) s = s + yc[(int64_t)z * ni + i];
-    ysum[z] = s;
-  }
-  __syncthreads();
-  const int64_t e = (int64_t)blockIdx.x * VK_NT + threadIdx.x;
-  if (e >= o.total) return;
-  double* out = cot_atm + col.off_a;
-  if (e >= o.M && e < o.M + nz) return;  // M enters the chemistry only: k_chem_vjp writes it
-  if ((e >= o.gas && e < o.gas + ni) || e >= o.flags) {
-    out[e] = 0.0;
-    return;
-  }
-  const double use_vm = rec[o.flags + 0] != 0.0 ? 1.0 : 0.0;
-  const double use_set = rec[o.flags + 1] != 0.0 ? 1.0 : 0.0;
-  const bool use_top = rec[o.flags + 2] != 0.0, use_bot = rec[o.flags + 3] != 0.0;
-  const Seeded A{rec, e};
-  // the layer window and species range element e can reach
-  int z_lo, z_hi, i_lo, i_hi;
+// The layers and species element e of the record reaches (z in [z_lo, z_hi),
+// i in [i_lo, i_hi)); true for the wide elements (see above).
+__device__ inline bool diff_window(const VkAtmOff& o, int nz, int ni, int64_t e, int& z_lo,
+                                   int& z_hi, int& i_lo, int& i_hi) {
+  bool wide = false;
   if (e < o.Tco + nz || (e >= o.g && e < o.g + nz)) {  // per-layer scalars
     int m;
     if (e < o.Tco) m = (int)((e - o.Kzz) % (nz - 1));
     else if (e < o.g) m = (int)(e - o.Tco);
     else m = (int)(e - o.g);
     z_lo = m - 1; z_hi = m + 2; i_lo = 0; i_hi = ni;
+    wide = true;
   } else if (e < o.top_flux) {  // ms, alpha: every layer of species i
     const int i = (int)((e - o.ms) % ni);
     z_lo = 0; z_hi = nz; i_lo = i; i_hi = i + 1;
+    wide = true;
   } else if (e < o.bot_flux) {  // top_flux
     const int i = (int)(e - o.top_flux);
     z_lo = nz - 1; z_hi = nz; i_lo = i; i_hi = i + 1;
@@ -187,6 +162,69 @@ __global__ __launch_bounds__(VK_NT) void k_diff_vjp(const VkCol* __restrict__ co
   }
   z_lo = z_lo < 0 ? 0 : z_lo;
   z_hi = z_hi > nz ? nz : z_hi;
+  return wide;
+}
+
+// ysum over the gas species (jax_step.py:165-167) of layers [z0, z1), one
+// layer per thread, summed in species order as the chemistry kernels do.
+__device__ inline void ysum_layers(const double* rec, const VkAtmOff& o, const double* yc, int ni,
+                                   int z0, int z1, double* ysum) {
+  for (int z = z0 + (int)threadIdx.x; z < z1; z += VK_NT) {
+    double s = 0.0;
+    for (int i = 0; i < ni; ++i)
+      if (rec[o.gas + i] != 0.0) s = s + yc[(int64_t)z * ni + i];
+    ysum[z] = s;
+  }
+}
+
+__global__ __launch_bounds__(VK_NT) void k_diff_vjp(const VkCol* __restrict__ cols, int ncols,
+                                                    const double* __restrict__ y,
+                                                    const double* __restrict__ atm,
+                                                    const double* __restrict__ lam,
+                                                    double* __restrict__ cot_atm) {
+  extern __shared__ double ysum[];
+  const int c = blockIdx.y;
+  if (c >= ncols) return;
+  const VkCol col = cols[c];
+  const int nz = col.nz, ni = col.ni;
+  const VkAtmOff o = vk_atm_offsets(nz, ni);
+  const double* rec = atm + col.off_a;
+  const double* yc = y + col.off_s;
+  const double* lc = lam + col.off_s;
+  // the ysum of the layers this workgroup's elements reach
+  {
+    const int64_t e0 = (int64_t)blockIdx.x * VK_NT;
+    int z0 = 0, z1 = nz;
+    if (e0 >= o.Dzz) {   // Dzz, vm, vs rows: layers around the block's rows
+      const int64_t r0 = e0 < o.vm ? e0 - o.Dzz : (e0 < o.vs ? e0 - o.vm : e0 - o.vs);
+      const int64_t r1 = r0 + VK_NT;
+      z0 = (int)(r0 / ni) - 2;
+      z1 = (int)(r1 / ni) + 3;
+      if ((e0 < o.vm && e0 + VK_NT > o.vm) || (e0 < o.vs && e0 + VK_NT > o.vs)) {
+        z0 = 0;   // the block straddles two fields
+        z1 = nz;
+      }
+      z0 = z0 < 0 ? 0 : z0;
+      z1 = z1 > nz ? nz : z1;
+    }
+    ysum_layers(rec, o, yc, ni, z0, z1, ysum);
+  }
+  __syncthreads();
+  const int64_t e = (int64_t)blockIdx.x * VK_NT + threadIdx.x;
+  if (e >= o.total) return;
+  double* out = cot_atm + col.off_a;
+  if (e >= o.M && e < o.M + nz) return;  // M enters the chemistry only: k_chem_vjp writes it
+  if ((e >= o.gas && e < o.gas + ni) || e >= o.flags) {
+    out[e] = 0.0;
+    return;
+  }
+  const double use_vm = rec[o.flags + 0] != 0.0 ? 1.0 : 0.0;
+  const double use_set = rec[o.flags + 1] != 0.0 ? 1.0 : 0.0;
+  const bool use_top = rec[o.flags + 2] != 0.0, use_bot = rec[o.flags + 3] != 0.0;
+  const Seeded A{rec, e};
+  // the layer window and species range element e can reach
+  int z_lo, z_hi, i_lo, i_hi;
+  if (diff_window(o, nz, ni, e, z_lo, z_hi, i_lo, i_hi)) return;  // k_diff_vjp_wide
   double acc = 0.0;
   for (int z = z_lo; z < z_hi; ++z)
     for (int i = i_lo; i < i_hi; ++i) {
@@ -194,6 +232,48 @@ __global__ __launch_bounds__(VK_NT) void k_diff_vjp(const VkCol* __restrict__ co
       acc = acc + lc[(int64_t)z * ni + i] * d.d;
     }
   out[e] = -acc;
+}
+
+// One workgroup per wide element: the Kzz .. g block and the ms, alpha
+// block of the record, in that order along blockIdx.x.
+__global__ __launch_bounds__(VK_NT) void k_diff_vjp_wide(const VkCol* __restrict__ cols,
+                                                         int ncols, const double* __restrict__ y,
+                                                         const double* __restrict__ atm,
+                                                         const double* __restrict__ lam,
+                                                         double* __restrict__ cot_atm) {
+  extern __shared__ double ysum[];
+  __shared__ double red[VK_NT / 64];
+  const int c = blockIdx.y;
+  if (c >= ncols) return;
+  const VkCol col = cols[c];
+  const int nz = col.nz, ni = col.ni;
+  const VkAtmOff o = vk_atm_offsets(nz, ni);
+  const int64_t n1 = o.M - o.Kzz, n2 = o.top_flux - o.ms;
+  const int64_t w = blockIdx.x;
+  if (w >= n1 + n2) return;
+  const int64_t e = w < n1 ? o.Kzz + w : o.ms + (w - n1);
+  const double* rec = atm + col.off_a;
+  const double* yc = y + col.off_s;
+  const double* lc = lam + col.off_s;
+  int z_lo, z_hi, i_lo, i_hi;
+  diff_window(o, nz, ni, e, z_lo, z_hi, i_lo, i_hi);
+  ysum_layers(rec, o, yc, ni, z_lo > 0 ? z_lo - 1 : 0, z_hi < nz ? z_hi + 1 : nz, ysum);
+  __syncthreads();
+  const double use_vm = rec[o.flags + 0] != 0.0 ? 1.0 : 0.0;
+  const double use_set = rec[o.flags + 1] != 0.0 ? 1.0 : 0.0;
+  const bool use_top = rec[o.flags + 2] != 0.0, use_bot = rec[o.flags + 3] != 0.0;
+  const Seeded A{rec, e};
+  const int nsp = i_hi - i_lo, nrow = (z_hi - z_lo) * nsp;
+  double acc = 0.0;
+  for (int q = threadIdx.x; q < nrow; q += VK_NT) {
+    const int z = z_lo + q / nsp, i = i_lo + q % nsp;
+    const Dl d = diff_row_d(A, o, nz, ni, z, i, yc, ysum, use_vm, use_set, use_top, use_bot);
+    acc = acc + lc[(int64_t)z * ni + i] * d.d;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) cot_atm[col.off_a + e] = -((red[0] + red[1]) + (red[2] + red[3]));
 }
 
 __global__ __launch_bounds__(VK_NT) void k_chem_vjp(const VkCol* __restrict__ cols, int ncols,
@@ -252,6 +332,11 @@ int vk_launch_ss_vjp(const VkCol* cols, int ncols, int max_nz, int max_ni, int64
     const int gx = (int)((max_rec + VK_NT - 1) / VK_NT);
     hipLaunchKernelGGL(k_diff_vjp, dim3(gx, ncols), dim3(VK_NT), sizeof(double) * (size_t)max_nz,
                        st, cols, ncols, y, atm, lam, cot_atm);
+    VK_CHECK_HIP(hipGetLastError());
+    // wide elements: Kzz, dzi, vz, Hpi, Ti (nz-1 each), Tco, g (nz), ms, alpha (ni)
+    const int gw = 5 * (max_nz - 1) + 2 * max_nz + 2 * max_ni;
+    hipLaunchKernelGGL(k_diff_vjp_wide, dim3(gw, ncols), dim3(VK_NT),
+                       sizeof(double) * (size_t)max_nz, st, cols, ncols, y, atm, lam, cot_atm);
     VK_CHECK_HIP(hipGetLastError());
   }
   const size_t shmem = sizeof(double) * 2 * (size_t)(max_ni + 1);
