@@ -3,7 +3,7 @@
 * solver.block_thomas (solver.py:129-176), the dense-off-diagonal block
   Thomas: the reference's seeded system (test_block_thomas.py:22-50, seed
   42, nz=8, ni=5) against a dense LAPACK solve at its bar 1e-9, and larger
-  systems (ni 21 .. 93, one needing pivoting) against the oracle's LU block
+  systems (ni 21 .. 100, one needing pivoting) against the oracle's LU block
   Thomas and LAPACK; a batch equals its columns alone bit for bit; the
   diagonal-off-diagonal system through the dense path matches
   block_thomas_diag_offdiag.
@@ -46,7 +46,8 @@ def test_block_thomas_dense_reference_seed42():
     assert e < 1e-9
 
 
-@pytest.mark.parametrize("nz,ni,scale", [(30, 21, 4.0), (40, 69, 4.0), (24, 93, 4.0), (20, 37, 0.0)])
+@pytest.mark.parametrize("nz,ni,scale", [(30, 21, 4.0), (40, 69, 4.0), (24, 93, 4.0), (20, 37, 0.0),
+                                         (12, 100, 4.0)])
 def test_block_thomas_dense_vs_oracle_and_lapack(nz, ni, scale):
     """scale 0: no diagonal shift, so the blocks need row exchanges."""
     from oracle import ros2_np as O
