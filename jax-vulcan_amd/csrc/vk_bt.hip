@@ -99,15 +99,24 @@ __device__ __forceinline__ bool gj_step(double (&a)[SL][SL], GjCtx<SL>& c) {
   if (lane == 0) { sh.cand_val[buf][wave] = m; sh.cand_idx[buf][wave] = brow; }
   __builtin_amdgcn_s_setprio(0);
   __syncthreads();
-  // Global pivot: best of the wave candidates.
-  double pval = sh.cand_val[buf][0];
-  int p = sh.cand_idx[buf][0];
+  // Global pivot: best of the wave candidates.  All candidates are read
+  // first and combined with selects, so the LDS reads are one round trip.
+  double cv[NW];
+  int ci[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    cv[w] = sh.cand_val[buf][w];
+    ci[w] = sh.cand_idx[buf][w];
+  }
+  double pval = cv[0];
+  int p = ci[0];
   int pw = 0;
 #pragma unroll
   for (int w = 1; w < NW; ++w) {
-    const double v = sh.cand_val[buf][w];
-    const int r = sh.cand_idx[buf][w];
-    if (v > pval || (v == pval && r < p)) { pval = v; p = r; pw = w; }
+    const bool take = cv[w] > pval || (cv[w] == pval && ci[w] < p);
+    pval = take ? cv[w] : pval;
+    p = take ? ci[w] : p;
+    pw = take ? w : pw;
   }
   p = __builtin_amdgcn_readfirstlane(p);
   pw = __builtin_amdgcn_readfirstlane(pw);
