@@ -65,13 +65,21 @@ __device__ __forceinline__ bool gj_step(double (&a)[SL][SL], GjCtx<SL>& c) {
   __builtin_amdgcn_s_setprio(2);
   // Column k of this thread's rows and their pivot ranks.
   double colv[SL], rank[SL];
-  double best = -INFINITY;
 #pragma unroll
   for (int s = 0; s < SL; ++s) {
     colv[s] = row_bcast<KC>(a[s][0]);
     rank[s] = fabs(colv[s]) + c.bias[s];
-    best = fmax(best, rank[s]);
   }
+  // the largest rank by a pairwise tree (fmax is exact and passes over
+  // NaN in any order, so the tree gives the chain's value)
+  double tmax[SL];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) tmax[s] = rank[s];
+#pragma unroll
+  for (int h = 1; h < SL; h *= 2)
+#pragma unroll
+    for (int s = 0; s + h < SL; s += 2 * h) tmax[s] = fmax(tmax[s], tmax[s + h]);
+  const double best = fmax(-INFINITY, tmax[0]);
   const double m = rows_max(best);
   // First row of the wave attaining m: smallest tile row s, then smallest tr.
   int brow = 0x7fffffff;
@@ -124,10 +132,11 @@ __device__ __forceinline__ bool gj_step(double (&a)[SL][SL], GjCtx<SL>& c) {
     p = k;
     c.bad = 1;
   }
+  // the pivot first, so its reciprocal can start while the row arrives
+  const double piv = sh.cand_row[buf][pw][KC];
   double u[SL];
 #pragma unroll
   for (int t = 0; t < SL; ++t) u[t] = sh.cand_row[buf][pw][t * VK_G + tc];
-  const double piv = sh.cand_row[buf][pw][KC];
   if (!(piv != 0.0) || !isfinite(piv)) c.bad = 1;
   const double rpiv = 1.0 / piv;
   if (tid == 0) sh.perm[k] = p;
