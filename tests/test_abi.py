@@ -115,3 +115,36 @@ def test_ctypes_structs_match_the_header(tmp_path):
         assert got[(cname, "size")] == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_network_create_rejects_bad_tables_without_gpu():
+    """vk_network_create validates sizes, indices and stoichiometry before it
+    touches the device: each bad table is refused with its own message."""
+    import ctypes as C
+    import numpy as np
+    from vulcan_amd import _lib
+    lib = _lib.load()
+
+    def create(ni, r_idx, p_idx, r_st, p_st, T=2):
+        nr = r_idx.shape[0] - 1
+        out = C.c_void_p()
+        args = [np.ascontiguousarray(a) for a in (r_idx, p_idx, r_st, p_st)]
+        rc = lib.vk_network_create(ni, nr, T, *(a.ctypes.data for a in args), None, C.byref(out))
+        return rc, lib.vk_last_error().decode()
+
+    ni = 4
+    ok_r = np.array([[ni, ni], [0, 1], [2, 3]], np.int64)
+    ok_p = np.array([[ni, ni], [2, 3], [0, 1]], np.int64)
+    ok_s = np.array([[0.0, 0.0], [1.0, 1.0], [1.0, 1.0]])
+    bad_idx = ok_r.copy()
+    bad_idx[1, 0] = ni + 1
+    rc, msg = create(ni, bad_idx, ok_p, ok_s, ok_s)
+    assert rc != 0 and "out of range" in msg
+    frac = ok_s.copy()
+    frac[2, 1] = 0.5
+    rc, msg = create(ni, ok_r, ok_p, frac, ok_s)
+    assert rc != 0 and "stoichiometry" in msg
+    rc, msg = create(lib.vk_max_species() + 1, ok_r, ok_p, ok_s, ok_s)
+    assert rc != 0 and "unsupported sizes" in msg
+    rc, msg = create(ni, ok_r, ok_p, ok_s, ok_s, T=5)
+    assert rc != 0 and "unsupported sizes" in msg
