@@ -254,6 +254,8 @@ def main() -> int:
     ap.add_argument("--config2", action=argparse.BooleanOptionalAction, default=True,
                     help="time one full steady-state solve of the nz=150, ni=93 column with "
                          "in-loop photolysis (nbin=2500), and a batched photolysis update")
+    ap.add_argument("--metrics", default=None, metavar="PATH",
+                    help="append JSON Lines records (legs, serving requests, result) to PATH")
     ap.add_argument("--config4", action=argparse.BooleanOptionalAction, default=True,
                     help="time the differentiation workload (steady state + implicit gradient)")
     ap.add_argument("--photo-batch", type=int, default=256,
@@ -296,6 +298,8 @@ def main() -> int:
     import numpy as np
     import torch
     from vulcan_amd import _lib, core, synth
+    from vulcan_amd.metrics import open_log
+    mlog = open_log(args.metrics)
 
     if args.copy_probe > 0.0:
         n = int(args.copy_probe * (1 << 30)) // 8
@@ -472,7 +476,7 @@ def main() -> int:
         eng = ServingEngine(hbm_budget_bytes=budget_gb * 1e9, cfg=scfg, chunk=25,
                             latency_mode=latency)
         t0 = time.perf_counter()
-        res = eng.run(reqs, arrivals, report_every=30.0)
+        res = eng.run(reqs, arrivals, report_every=30.0, metrics=mlog)
         wall = time.perf_counter() - t0
         summ = latency_summary(res)
         waits = np.array([r.start - r.arrival for r in res])
@@ -640,6 +644,14 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if mlog is not None:
+        for leg in ("uneven_batch", "serving", "config2_steady_state", "config4_gradient",
+                    "rate_build", "roofline", "cpu_baseline"):
+            if out.get(leg) is not None:
+                mlog.write("bench_leg", leg=leg, **out[leg])
+        mlog.write("bench_result", **{k: v for k, v in out.items()
+                                      if not isinstance(v, dict) or k == "config"})
+        mlog.close()
     print(json.dumps(out))
     return 0
 

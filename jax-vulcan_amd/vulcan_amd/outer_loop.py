@@ -411,10 +411,14 @@ class SteadyStateRunner:
         return bool(((self.status & ST_DONE) != 0).all().item())
 
     def run(self, max_attempts: int = 100000, check_every: int = 50, use_graph: bool = False,
-            report_every: float = 0.0):
+            report_every: float = 0.0, metrics=None):
         """Attempt until every column is done or max_attempts; returns attempts made.
-        report_every > 0 prints progress after the check that crosses each interval (s)."""
+        report_every > 0 prints progress after the check that crosses each interval (s).
+        ``metrics`` (a path, stream or metrics.JsonlLog) receives a
+        ``steady_check`` record at every check."""
         import time
+        from .metrics import open_log
+        log = open_log(metrics)
         t0 = time.perf_counter()
         next_report = report_every
         done_attempts = 0
@@ -428,7 +432,10 @@ class SteadyStateRunner:
                 for _ in range(check_every):
                     self.attempt()
                 done_attempts += check_every
-            if self.done():
+            finished = self.done()
+            if log is not None:
+                self._log_check(log, done_attempts, time.perf_counter() - t0)
+            if finished:
                 break
             if report_every > 0 and time.perf_counter() - t0 >= next_report:
                 next_report += report_every
@@ -437,6 +444,20 @@ class SteadyStateRunner:
                       f"done={n_done}/{self.batch.ncols}", flush=True)
         torch.cuda.synchronize()
         return done_attempts
+
+    def _log_check(self, log, attempts_done: int, wall: float):
+        st = self.status.cpu().numpy()
+        acc = self.accept.cpu().numpy()
+        att = self.attempts.cpu().numpy()
+        log.write("steady_check", columns=int(st.size), attempts=int(attempts_done),
+                  wall_s=round(wall, 6), done=int(((st & ST_DONE) != 0).sum()),
+                  converged=int(((st & ST_CONVERGED) != 0).sum()),
+                  capped=int(((st & ST_MAXSTEPS) != 0).sum()),
+                  accepted_total=int(acc.sum()), attempts_total=int(att.sum()),
+                  column_attempts_per_s=round(float(att.sum()) / wall, 3) if wall > 0 else None,
+                  rejects={"delta": int(self.dcount.sum().item()),
+                           "nega": int(self.nega.sum().item()),
+                           "loss": int(self.loss.sum().item())})
 
     # ------------------------------------------------------------------
     # checkpoint / resume (the run state the reference packs and unpacks,

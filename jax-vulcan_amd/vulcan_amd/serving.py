@@ -280,13 +280,17 @@ class ServingEngine:
 
     def run(self, requests: Sequence, arrivals: Optional[Sequence[float]] = None,
             realtime: bool = True, max_rounds: int = 10 ** 9,
-            report_every: float = 0.0) -> List[Result]:
+            report_every: float = 0.0, metrics=None) -> List[Result]:
         """Serve the requests; arrival times in seconds from the start (all 0 if None).
 
         With realtime=True a request is admitted only once the wall clock has
         reached its arrival time; latency = finish - arrival.  report_every > 0
         prints a progress line (completed / in flight / queued) after the
-        chunk that crosses each such interval."""
+        chunk that crosses each such interval.  ``metrics`` (a path, stream or
+        metrics.JsonlLog) receives a ``request_done`` record per finished
+        request and a ``serving_summary`` record at the end."""
+        from .metrics import open_log
+        log = open_log(metrics)
         reqs = list(requests)
         arr = np.zeros(len(reqs)) if arrivals is None else np.asarray(arrivals, dtype=np.float64)
         for i, r in enumerate(reqs):
@@ -324,6 +328,12 @@ class ServingEngine:
                 for s in done:
                     res = p.harvest(s, t_done)
                     results[res.index] = res
+                    if log is not None:
+                        log.write("request_done", index=res.index, bucket=_bucket(reqs[res.index].ni),
+                                  latency_s=round(res.latency, 6),
+                                  wait_s=round(res.start - res.arrival, 6),
+                                  attempts=res.attempts, accepted=res.accept,
+                                  status=res.status, converged=res.converged)
             if report_every > 0 and time.perf_counter() - t0 >= next_report:
                 next_report += report_every
                 n_done = sum(r is not None for r in results)
@@ -333,7 +343,10 @@ class ServingEngine:
                       f"waiting={n_q} chunks={rounds}", flush=True)
         for p in self.pools.values():
             p.stream.synchronize()
-        return [r for r in results if r is not None]
+        out = [r for r in results if r is not None]
+        if log is not None:
+            log.write("serving_summary", slots=dict(self.slot_counts), **latency_summary(out))
+        return out
 
 
 def latency_summary(results: Sequence[Result]) -> dict:

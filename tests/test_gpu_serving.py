@@ -109,3 +109,33 @@ def test_requests_carrying_rate_sources():
         print(f"req {r.index}: attempts {r.attempts} (k given: {att0}), status {r.status} ({st0})")
         assert r.attempts == att0 and r.status == st0
         assert np.array_equal(r.y, y0)
+
+
+def test_metrics_records_of_runs_and_serving(tmp_path):
+    """The JSON Lines records agree with the results they describe."""
+    import json
+    from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
+    from vulcan_amd.serving import ServingEngine
+    reqs = _requests()
+    path = tmp_path / "run.jsonl"
+    r = SteadyStateRunner(reqs[:2], RunConfig(count_max=300))
+    r.run(max_attempts=600, check_every=100, metrics=path)
+    recs = [json.loads(line) for line in path.read_text().splitlines()]
+    res = r.results()
+    assert recs and all(x["event"] == "steady_check" for x in recs)
+    last = recs[-1]
+    assert last["accepted_total"] == int(res["accept"].sum())
+    assert last["attempts_total"] == int(res["attempts"].sum())
+    assert last["done"] == 2 and last["columns"] == 2
+    assert last["rejects"]["delta"] == int(res["counts"]["delta"].sum())
+    path2 = tmp_path / "serve.jsonl"
+    eng = ServingEngine(hbm_budget_bytes=1e9, max_slots=2, cfg=RunConfig(count_max=300), chunk=25)
+    out = eng.run(reqs, metrics=path2)
+    recs = [json.loads(line) for line in path2.read_text().splitlines()]
+    done = [x for x in recs if x["event"] == "request_done"]
+    summ = [x for x in recs if x["event"] == "serving_summary"]
+    assert sorted(x["index"] for x in done) == sorted(o.index for o in out)
+    assert len(summ) == 1 and summ[0]["n"] == len(out)
+    by_idx = {o.index: o for o in out}
+    for x in done:
+        assert x["attempts"] == by_idx[x["index"]].attempts
