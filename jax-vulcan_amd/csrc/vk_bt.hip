@@ -118,20 +118,19 @@ __device__ __forceinline__ bool gj_step(double (&a)[SL][SL], GjCtx<SL>& c) {
   }
   double pval = cv[0];
   int p = ci[0];
-  int pw = 0;
 #pragma unroll
   for (int w = 1; w < NW; ++w) {
     const bool take = cv[w] > pval || (cv[w] == pval && ci[w] < p);
     pval = take ? cv[w] : pval;
     p = take ? ci[w] : p;
-    pw = take ? w : pw;
   }
   p = __builtin_amdgcn_readfirstlane(p);
-  pw = __builtin_amdgcn_readfirstlane(pw);
   if ((unsigned)p >= (unsigned)c.n) {  // no candidate left: keep indices in range
     p = k;
     c.bad = 1;
   }
+  // row 16 s + tr lives in wave tr / 4, which published it
+  const int pw = (p >> 2) & (NW - 1);
   // the pivot first, so its reciprocal can start while the row arrives
   const double piv = sh.cand_row[buf][pw][KC];
   double u[SL];
@@ -152,17 +151,17 @@ __device__ __forceinline__ bool gj_step(double (&a)[SL][SL], GjCtx<SL>& c) {
 #pragma unroll
     for (int s = 0; s < SL; ++s) a[s][0] = -l[s];
   }
+  // Row p becomes u / piv, with 1 / piv in column k; the products are
+  // formed in the branch of p's tile row, straight into the tile.
   const unsigned psm = __builtin_amdgcn_readfirstlane(1u << (p >> 4));
   if (tr == (p & 15)) {
-    double pr[SL];
-#pragma unroll
-    for (int t = 0; t < SL; ++t) pr[t] = u[t] * rpiv;
-    if (tc == KC) pr[0] = rpiv;
+    const double u0 = tc == KC ? 1.0 : u[0];
 #pragma unroll
     for (int s = 0; s < SL; ++s) {
       if (psm & (1u << s)) {
+        a[s][0] = u0 * rpiv;
 #pragma unroll
-        for (int t = 0; t < SL; ++t) a[s][t] = pr[t];
+        for (int t = 1; t < SL; ++t) a[s][t] = u[t] * rpiv;
         c.bias[s] = -INFINITY;
       }
     }

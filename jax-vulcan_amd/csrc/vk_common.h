@@ -247,12 +247,14 @@ __device__ inline double vk_rowsum16(double v) {
 }
 
 // Lane KC of every 16-lane row, broadcast to the whole row (DPP row_newbcast).
-// Every lane is written, so the DPP move needs no old value.
+// Every lane is written, so the DPP move needs no old value; on the 64-bit
+// value this is one v_mov_b64 with DPP (row_newbcast is a DPP64 control).
 template <int KC>
 __device__ __forceinline__ double row_bcast(double v) {
-  const int lo = __double2loint(v), hi = __double2hiint(v);
-  return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, 0x150 + KC, 0xF, 0xF, false),
-                          __builtin_amdgcn_mov_dpp(lo, 0x150 + KC, 0xF, 0xF, false));
+  static_assert(sizeof(long) == sizeof(double), "64-bit long");
+  const long x = __builtin_bit_cast(long, v);
+  const long y = __builtin_amdgcn_mov_dpp(x, 0x150 + KC, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, y);
 }
 
 // v = max(v, v moved by a DPP pattern); rows outside ROW_MASK keep v.
