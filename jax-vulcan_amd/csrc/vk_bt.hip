@@ -428,18 +428,56 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(TILE_IN &
 // (k_bt_residual, then a correction solve with epi.base) and passes refl as
 // the active mask of both, so the other columns skip them.
 
-__device__ inline void solve_epilogue(const SolveEpi& epi, int64_t e, double xv, double* x) {
-  if (epi.base) xv = epi.base[e] + xv;
+// The epilogue with its inputs already read: bv = base[e], yv = y[e],
+// kv = k1[e], zv = yk2[e] (those the epilogue kind uses).
+__device__ inline void solve_epilogue_v(const SolveEpi& epi, int64_t e, double xv, double* x,
+                                        double bv, double yv, double kv, double zv) {
+  if (epi.base) xv = bv + xv;
   x[e] = xv;
   if (epi.kind == 1) {
-    epi.out1[e] = epi.y[e] + xv / epi.e0;
+    epi.out1[e] = yv + xv / epi.e0;
   } else if (epi.kind == 2) {
 #pragma clang fp contract(off)
-    const double sol = epi.y[e] + epi.e0 * epi.k1[e] + epi.e1 * xv;
+    const double sol = yv + epi.e0 * kv + epi.e1 * xv;
     epi.out1[e] = sol;
-    epi.out2[e] = fabs(sol - epi.yk2[e]);
+    epi.out2[e] = fabs(sol - zv);
   }
 }
+
+__device__ inline void solve_epilogue(const SolveEpi& epi, int64_t e, double xv, double* x) {
+  solve_epilogue_v(epi, e, xv, x, epi.base ? epi.base[e] : 0.0, epi.kind ? epi.y[e] : 0.0,
+                   epi.kind == 2 ? epi.k1[e] : 0.0, epi.kind == 2 ? epi.yk2[e] : 0.0);
+}
+
+// Read-ahead of the per-row vectors a layer's sweep step reads besides its
+// block (right-hand sides, off-diagonals, epilogue inputs): NV vectors of
+// VK_G * SL rows, spread over the workgroup's threads, loaded together with
+// the layer's block so their latency hides behind it, then put in LDS
+// (slot v, row i at dst[v * VK_G * SL + i]) for the step that needs them.
+// src[v] points at row 0 of vector v in that layer, or is null (zeros).
+template <int SL, int NV>
+struct RowAhead {
+  static constexpr int N = VK_G * SL, PER = (NV * N + VK_NT - 1) / VK_NT;
+  double v[PER];
+  __device__ __forceinline__ void load(const double* const (&src)[NV], int ni) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int f = threadIdx.x + q * VK_NT;
+      const int w = f / N, i = f - w * N;
+      const double* p = src[0];
+#pragma unroll
+      for (int u = 1; u < NV; ++u) p = w == u ? src[u] : p;
+      v[q] = (f < NV * N && p && i < ni) ? p[i] : 0.0;
+    }
+  }
+  __device__ __forceinline__ void store(double* dst) const {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int f = threadIdx.x + q * VK_NT;
+      if (f < NV * N) dst[f] = v[q];
+    }
+  }
+};
 
 template <int SL, int REFINE>
 __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ? (REFINE == 1 ? 2 : 4) : 1))) void k_bt_solve(
@@ -452,6 +490,10 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ?
   __shared__ double rv[VK_G * SL];  // refinement: x_j, then the residual
   __shared__ double bx[VK_G * SL];  // refinement: b_{j-1} o x_j
   __shared__ double red[VK_NT / 64][2];  // per wave: max |correction|, max |x| of a layer
+  // read-ahead rows, double-buffered by layer: the forward sweep's r and
+  // c_{j-1}; the refined backward sweep's base, y, k1, yk2, r'_{j-1}, b_{j-1}
+  __shared__ double ahead[2][6 * VK_G * SL];
+  constexpr int NR = VK_G * SL;
   const int c = blockIdx.x;
   if (c >= ncols) return;
   if (active && (active[c] & VK_ST_DONE)) {
@@ -475,17 +517,31 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ?
       vec[0][i] = v;
       if (i < ni) rscratch[col.off_s + i] = v;
     }
+    // slot j & 1 holds r_j and c_{j-1} of layer j, read one layer ahead
+    if (nz > 1) {
+      RowAhead<SL, 2> ra;
+      const double* src[2] = {rhs + col.off_s + ni, bsub};
+      ra.load(src, ni);
+      ra.store(ahead[1]);
+    }
     __syncthreads();
     for (int j = 1; j < nz; ++j) {
+      RowAhead<SL, 2> ra;
+      if (j + 1 < nz) {
+        const double* src[2] = {rhs + col.off_s + (int64_t)(j + 1) * ni, bsub + (int64_t)j * ni};
+        ra.load(src, ni);
+      }
       load_tile<SL>(a, stk + (int64_t)(j - 1) * BLK);
       tile_matvec<SL>(a, vec[(j - 1) & 1], w);
+      if (j + 1 < nz) ra.store(ahead[(j + 1) & 1]);
       if (tc == 0) {
+        const double* av = ahead[j & 1];
 #pragma unroll
         for (int s = 0; s < SL; ++s) {
           const int i = s * VK_G + tr;
           double r = 0.0;
           if (i < ni) {
-            r = rhs[col.off_s + (int64_t)j * ni + i] - bsub[(int64_t)(j - 1) * ni + i] * w[s];
+            r = av[i] - av[NR + i] * w[s];
             rscratch[col.off_s + (int64_t)j * ni + i] = r;
           }
           vec[j & 1][i] = r;
@@ -527,9 +583,40 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ?
     // per layer: x0 = X_j t_j; r = t_j - A'_j x0; x_j = x0 + X_j r
     const double* sblk = dstk + col.off_b;
     bool wide = false;  // thread 0: some layer's correction exceeded its solution
+    // slot j & 1: base, y, k1, yk2 of layer j, then r'_{j-1} and b_{j-1}
+    auto ahead_src = [&](int j, const double* (&src)[6]) {
+      const int64_t o = col.off_s + (int64_t)j * ni;
+      src[0] = epi.base ? epi.base + o : nullptr;
+      src[1] = epi.kind ? epi.y + o : nullptr;
+      src[2] = epi.kind == 2 ? epi.k1 + o : nullptr;
+      src[3] = epi.kind == 2 ? epi.yk2 + o : nullptr;
+      src[4] = j > 0 ? rm + o - ni : nullptr;
+      src[5] = j > 0 ? bsup + (int64_t)(j - 1) * ni : nullptr;
+    };
+    {
+      RowAhead<SL, 6> ra;
+      const double* src[6];
+      ahead_src(nz - 1, src);
+      ra.load(src, ni);
+      ra.store(ahead[(nz - 1) & 1]);
+    }
+    __syncthreads();
     for (int j = nz - 1; j >= 0; --j) {
-      load_tile<SL>(a, stk + (int64_t)j * BLK);
-      tile_matvec<SL>(a, vec[j & 1], w);                      // x0_j
+      RowAhead<SL, 6> ra;
+      if (j > 0) {
+        const double* src[6];
+        ahead_src(j - 1, src);
+        ra.load(src, ni);
+      }
+      // No block stays in registers across the layer: each of the three
+      // products loads its whole block first, so all its loads are in
+      // flight at once; X_j is read again for the correction (from cache).
+      {
+        double t0[SL][SL];
+        load_tile<SL>(t0, stk + (int64_t)j * BLK);
+        tile_matvec<SL>(t0, vec[j & 1], w);                   // x0_j
+      }
+      if (j > 0) ra.store(ahead[(j - 1) & 1]);
       if (tc == 0) {
 #pragma unroll
         for (int s = 0; s < SL; ++s) {
@@ -539,7 +626,11 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ?
       }
       __syncthreads();
       double q[SL];
-      stream_matvec<SL>(sblk + (int64_t)j * BLK, rv, q);      // A'_j x0_j
+      {
+        double t1[SL][SL];
+        load_tile<SL>(t1, sblk + (int64_t)j * BLK);
+        tile_matvec<SL>(t1, rv, q);                           // A'_j x0_j
+      }
       if (tc == 0) {
 #pragma unroll
         for (int s = 0; s < SL; ++s) {
@@ -549,20 +640,28 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ?
       }
       __syncthreads();
       double d[SL];
-      tile_matvec<SL>(a, bx, d);                              // X_j (t_j - A'_j x0_j)
+      {
+        // an opaque copy of the address, so the compiler reloads X_j
+        // instead of holding the first load's registers across the layer
+        const double* xj = stk + (int64_t)j * BLK;
+        asm volatile("" : "+s"(xj));
+        double t2[SL][SL];
+        load_tile<SL>(t2, xj);
+        tile_matvec<SL>(t2, bx, d);                           // X_j (t_j - A'_j x0_j)
+      }
       double dmax = 0.0, xmax = 0.0;
       if (tc == 0) {
+        const double* av = ahead[j & 1];
 #pragma unroll
         for (int s = 0; s < SL; ++s) {
           const int i = s * VK_G + tr;
           if (i < ni) {
-            const double xv = w[s] + d[s];
+            const double xv = rv[i] + d[s];  // rv still holds x0_j (w's registers are free)
             dmax = fmax(dmax, fabs(d[s]));
             xmax = fmax(xmax, fabs(xv));
-            solve_epilogue(epi, col.off_s + (int64_t)j * ni + i, xv, x);
-            if (j > 0)
-              vec[(j - 1) & 1][i] = rm[col.off_s + (int64_t)(j - 1) * ni + i] -
-                                    bsup[(int64_t)(j - 1) * ni + i] * xv;
+            solve_epilogue_v(epi, col.off_s + (int64_t)j * ni + i, xv, x, av[i], av[NR + i],
+                             av[2 * NR + i], av[3 * NR + i]);
+            if (j > 0) vec[(j - 1) & 1][i] = av[4 * NR + i] - av[5 * NR + i] * xv;
           } else if (j > 0) {
             vec[(j - 1) & 1][i] = 0.0;
           }
