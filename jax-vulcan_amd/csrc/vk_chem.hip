@@ -604,28 +604,41 @@ int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max
 }
 
 // kT[off_k + z*S + r] = k[off_k + r*nz + z]: the rate tables layer-major,
-// through a 32 x 32 LDS tile so both sides are coalesced.
+// through an LDS tile of 64 rows x 32 layers so both sides are coalesced
+// (256-byte runs of layers read, 512-byte runs of rows written); each thread
+// moves eight elements.
+constexpr int KT_R = 64, KT_Z = 32;
 __global__ __launch_bounds__(256) void k_rates_layer_major(const VkCol* __restrict__ cols, int ncols,
                                                           const double* __restrict__ k,
                                                           double* __restrict__ kT,
                                                           const int* __restrict__ active, int c0) {
-  __shared__ double tile[32][33];
+  __shared__ double tile[KT_R][KT_Z + 1];
   const int c = c0 + (int)blockIdx.z;
   if (c >= ncols) return;
   if (active && (active[c] & VK_ST_DONE)) return;
   const VkCol col = cols[c];
   const int S = col.S, nz = col.nz;
-  const int r0 = blockIdx.x * 32, z0 = blockIdx.y * 32;
+  const int r0 = blockIdx.x * KT_R, z0 = blockIdx.y * KT_Z;
   if (r0 >= S || z0 >= nz) return;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  for (int i = ty; i < 32; i += 8) {
-    const int r = r0 + i, z = z0 + tx;
-    if (r < S && z < nz) tile[i][tx] = k[col.off_k + (int64_t)r * nz + z];
+  const int tid = threadIdx.x;
+  {
+    const int zl = tid & (KT_Z - 1), rl = tid >> 5;  // 32 layers x 8 rows
+    const int z = z0 + zl;
+#pragma unroll
+    for (int i = rl; i < KT_R; i += 8) {
+      const int r = r0 + i;
+      if (r < S && z < nz) tile[i][zl] = k[col.off_k + (int64_t)r * nz + z];
+    }
   }
   __syncthreads();
-  for (int i = ty; i < 32; i += 8) {
-    const int z = z0 + i, r = r0 + tx;
-    if (r < S && z < nz) kT[col.off_k + (int64_t)z * S + r] = tile[tx][i];
+  {
+    const int rl = tid & (KT_R - 1), zl0 = tid >> 6;  // 64 rows x 4 layers
+    const int r = r0 + rl;
+#pragma unroll
+    for (int i = zl0; i < KT_Z; i += 4) {
+      const int z = z0 + i;
+      if (r < S && z < nz) kT[col.off_k + (int64_t)z * S + r] = tile[rl][i];
+    }
   }
 }
 
@@ -633,7 +646,7 @@ int vk_launch_rates_layer_major(const VkCol* cols, int ncols, int max_S, int max
                                 double* kT, const int* active, hipStream_t stream) {
   for (int c0 = 0; c0 < ncols; c0 += 65535) {  // grid z is limited to 65535
     const int nc = ncols - c0 < 65535 ? ncols - c0 : 65535;
-    dim3 grid((max_S + 31) / 32, (max_nz + 31) / 32, nc);
+    dim3 grid((max_S + KT_R - 1) / KT_R, (max_nz + KT_Z - 1) / KT_Z, nc);
     hipLaunchKernelGGL(k_rates_layer_major, grid, dim3(256), 0, stream, cols, ncols, k, kT, active,
                        c0);
     VK_CHECK_HIP(hipGetLastError());
