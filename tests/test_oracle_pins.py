@@ -540,6 +540,50 @@ def test_flux_layer_subdivision_invariance(ag0):
         np.testing.assert_allclose(down, out[0][1], rtol=1e-13)
 
 
+def _two_stream_ode_slab(dtau, w0, g, mu0, mu1):
+    """Diffuse fluxes leaving a homogeneous slab lit by a unit direct beam,
+    from the two-stream equations themselves (Toon et al. 1989, JGR 94,
+    16287, eqs. 1-2; Meador & Weaver 1980, JAS 37, 630), with tau down:
+
+        dF_dn/dtau = -g1 F_dn + g2 F_up + w0 (1 - g3) e^(-tau/mu0)
+        dF_up/dtau = -g2 F_dn + g1 F_up - w0 g3 e^(-tau/mu0)
+
+    g1 = (1 - w0 (1 + g) / 2) / mu1, g2 = w0 (1 - g) / (2 mu1) (the
+    hemispheric closure at mu1) and g3 = (1 - g mu0 / mu1) / 2, the share of
+    the scattered beam sent up.  Solved as a linear boundary-value problem
+    through the matrix exponential of the augmented system (no diffuse light
+    in at the top, a black surface below); returns (F_up at the top, F_dn
+    at the bottom)."""
+    from scipy.linalg import expm
+    g1 = (1.0 - w0 * (1.0 + g) / 2.0) / mu1
+    g2 = w0 * (1.0 - g) / (2.0 * mu1)
+    g3 = (1.0 - g * mu0 / mu1) / 2.0
+    K = np.array([[-g1, g2, w0 * (1.0 - g3)],
+                  [-g2, g1, -w0 * g3],
+                  [0.0, 0.0, -1.0 / mu0]])
+    E = expm(K * dtau)                      # (F_dn, F_up, beam) at dtau from (0, R, 1) at 0
+    R = -E[1, 2] / E[1, 1]                  # F_up(dtau) = 0
+    return R, E[0, 1] * R + E[0, 2]
+
+
+@pytest.mark.parametrize("ag0", [0.0, 0.3])
+def test_flux_single_layer_solves_the_two_stream_equations(ag0):
+    """One homogeneous layer at the lag's fixed point (photo.py:167-236):
+    its emerging diffuse fluxes are the solution of the two-stream
+    boundary-value problem with closure mu1 = edd = 0.5 and mu0 = 0.7,
+    found here by a matrix exponential rather than the reference's
+    eigen-solution.  Optically thin to thick layers, weak to strong
+    scattering, isotropic and forward scattering (ag0 = 0.3): 1e-12.
+    A wrong g_p / g_m, zeta or ll, or the other share of the scattered beam
+    (Eddington's g3 = (2 - 3 g mu0) / 4 misses by 5-13 %), fails it."""
+    for dtau in (0.05, 0.3, 0.8, 3.0):
+        for w0 in (0.2, 0.75, 0.99):
+            af, sf, dd, du = _lagged_fixed_point(_stack([dtau]), w0, ag0)
+            up, dn = _two_stream_ode_slab(dtau, w0, ag0, 0.7, 0.5)
+            np.testing.assert_allclose(du[-1], up, rtol=1e-12)
+            np.testing.assert_allclose(dd[0], dn, rtol=1e-12)
+
+
 @pytest.mark.parametrize("ag0", [0.0, 0.3])
 def test_flux_conservative_scattering_keeps_net_flux(ag0):
     """w0 = 1 - 1e-8 (the reference's clip, photo.py:167-169) over 24
