@@ -375,7 +375,7 @@ def newton_steady_state(y0, k_arr, atm: AtmStatic, net: NetworkArrays, tol: floa
 def pseudo_transient_steady_state(y0, k_arr, atm: AtmStatic, net: NetworkArrays,
                                   dt0: float = 1e-6, grow: float = 4.0, shrink: float = 8.0,
                                   dt_newton: float = 1e30, polish: int = 8,
-                                  max_iter: int = 1000):
+                                  max_iter: int = 1000, floor: float = 0.0):
     """Steady state of f(y) = 0 by pseudo-transient continuation on the GPU.
 
     Each iteration solves (I/dt - J_y) dy = f(y) with the J_y blocks and the
@@ -386,6 +386,12 @@ def pseudo_transient_steady_state(y0, k_arr, atm: AtmStatic, net: NetworkArrays,
     forward solve of the differentiation workload; the reference reaches the
     same state with its adaptive Ros2 loop (outer_loop.py:492-1112), whose
     stopping rule leaves a residual the implicit gradient then carries.
+
+    ``floor`` > 0 takes a step that would push a density below ``floor``
+    times its current value with that density held there instead, and
+    leaves dt unchanged after such a step (a positivity-preserving damping
+    for networks whose trace densities span many decades; the fixed point
+    is the same, since near it no density is held).
 
     Returns (y, iterations, ||f||_inf).
     """
@@ -404,7 +410,15 @@ def pseudo_transient_steady_state(y0, k_arr, atm: AtmStatic, net: NetworkArrays,
             diag[:, eye, eye] -= 1.0 / dt
         dy, status = _factor_solve(diag, sup, sub, -res, net, nz, False)
         yn = y + dy
-        ok = bool(torch.all(yn > 0).item()) and not (int(status.item()) & ST_SINGULAR)
+        held = False
+        if floor > 0.0:
+            low = yn < floor * y
+            held = bool(low.any().item())
+            yn = torch.where(low, floor * y, yn)
+            ok = bool(torch.isfinite(yn).all().item())
+        else:
+            ok = bool(torch.all(yn > 0).item())
+        ok = ok and not (int(status.item()) & ST_SINGULAR)
         if ok:
             y = yn
             if newton:
@@ -412,7 +426,8 @@ def pseudo_transient_steady_state(y0, k_arr, atm: AtmStatic, net: NetworkArrays,
                 if left <= 0:
                     nrm = float(_residual(y, k, rec, net, nz).abs().max().item())
                     return y, it + 1, nrm
-            dt = min(dt * grow, dt_newton)
+            if not held:
+                dt = min(dt * grow, dt_newton)
         else:
             dt /= shrink
     raise RuntimeError(f"pseudo-transient continuation did not converge in {max_iter} steps")
