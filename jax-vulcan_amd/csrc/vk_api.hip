@@ -230,13 +230,16 @@ static std::vector<std::vector<int>> lpt_lanes(const std::vector<int>& cost) {
 }
 
 // Lane-interleaved record table from per-lane record lists, padded with
-// no-op records to the longest lane rounded up to whole load groups.
-// Returns the records per lane.
-static int interleave(const std::vector<std::vector<VkRec>>& per_lane, std::vector<VkRec>& out) {
+// no-op records to the longest lane rounded up to whole load groups.  A
+// record with no term points at zero_item, the table slot the kernel holds
+// at 0.0, so every record adds c * (a table entry).  Returns the records per
+// lane.
+static int interleave(const std::vector<std::vector<VkRec>>& per_lane, std::vector<VkRec>& out,
+                      uint32_t zero_item) {
   size_t k = 1;
   for (auto& v : per_lane) k = std::max(k, v.size());
   k = (k + VK_REC_GROUP - 1) / VK_REC_GROUP * VK_REC_GROUP;  // whole load groups
-  out.assign(k * VK_CHEM_NT, VkRec{0.0f, vk_rec_word(VK_REC_NONE_ITEM, VK_REC_NONE_DEST)});
+  out.assign(k * VK_CHEM_NT, VkRec{0.0f, vk_rec_word(zero_item, VK_REC_NONE_DEST)});
   for (int l = 0; l < VK_CHEM_NT; ++l)
     for (size_t j = 0; j < per_lane[l].size(); ++j) out[j * VK_CHEM_NT + l] = per_lane[l][j];
   return (int)k;
@@ -358,7 +361,7 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
         int i, q;
         tile_elem(idx, i, q);
         if (i >= ni || q >= ni || JL[(size_t)i * ni + q].empty()) {
-          jl[l].push_back({0.0f, vk_rec_word(VK_REC_NONE_ITEM, (uint32_t)idx)});
+          jl[l].push_back({0.0f, vk_rec_word((uint32_t)(S * T), (uint32_t)idx)});
           continue;
         }
         const auto& lst = JL[(size_t)i * ni + q];
@@ -369,7 +372,7 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
       }
   }
   std::vector<VkRec> jrec;
-  const int jrec_k = interleave(jl, jrec);
+  const int jrec_k = interleave(jl, jrec, (uint32_t)(S * T));
   // loss / production chunk records: loss chunks of species 0..ni-1, then
   // production chunks, VK_FCHUNK list entries each
   std::vector<int> lcp(ni + 1), pcp(ni + 1);
@@ -399,7 +402,7 @@ int vk_network_create(int ni, int nr, int T, const int64_t* reactant_idx,
       for (int ch : lanes[l]) fl[l].insert(fl[l].end(), chunks[ch].begin(), chunks[ch].end());
   }
   std::vector<VkRec> frec;
-  const int frec_k = interleave(fl, frec);
+  const int frec_k = interleave(fl, frec, (uint32_t)S);
   // one device allocation for every table
   size_t off = 0;
   auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };

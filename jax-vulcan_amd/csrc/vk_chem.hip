@@ -239,8 +239,14 @@ __device__ inline DiffCoef mol_coef_pre(int z, int nz, const double* m, int64_t 
 // loaded a group at a time, and the next group is requested before the
 // current one is worked through, so the loads overlap the work instead of
 // waiting behind its branches.
-template <class F>
-__device__ __forceinline__ void walk_records(const VkRec* rec, int count, F&& f) {
+//
+// Every record has a term (a term-less one points at a zero slot), so the
+// terms of a whole group, term(rec) = c * table[item], are formed first, all
+// their table reads in flight together, and close(rec, term) then adds each
+// to the running sum in record order and stores closed sums.
+template <class Term, class Close>
+__device__ __forceinline__ void walk_records(const VkRec* rec, int count, Term&& term,
+                                             Close&& close) {
   VkRec cur[VK_REC_GROUP], nxt[VK_REC_GROUP];
 #pragma unroll
   for (int g = 0; g < VK_REC_GROUP; ++g) cur[g] = rec[(int64_t)g * VK_CHEM_NT];
@@ -250,8 +256,11 @@ __device__ __forceinline__ void walk_records(const VkRec* rec, int count, F&& f)
       for (int g = 0; g < VK_REC_GROUP; ++g)
         nxt[g] = rec[(int64_t)(k0 + VK_REC_GROUP + g) * VK_CHEM_NT];
     }
+    double t[VK_REC_GROUP];
 #pragma unroll
-    for (int g = 0; g < VK_REC_GROUP; ++g) f(cur[g]);
+    for (int g = 0; g < VK_REC_GROUP; ++g) t[g] = term(cur[g]);
+#pragma unroll
+    for (int g = 0; g < VK_REC_GROUP; ++g) close(cur[g], t[g]);
 #pragma unroll
     for (int g = 0; g < VK_REC_GROUP; ++g) cur[g] = nxt[g];
   }
@@ -296,9 +305,15 @@ __global__ __launch_bounds__(NT) void k_chem_layer(LayerArgs L) {
   const bool want_jac = (L.mode == M_CHEM_JAC || L.mode == M_LHS || L.mode == M_JY);
   const bool want_diff = (L.mode == M_LHS || L.mode == M_RHS2 || L.mode == M_FTOT || L.mode == M_JY);
   const bool want_rhs = (L.mode != M_CHEM_JAC && L.mode != M_JY);
+  // rate[S] and drate[S * T] hold 0.0: the record streams point their
+  // term-less records there, so every record adds c * (a table entry)
   double* rate = smem;
-  double* drate = smem + S;
-  double* yrow = smem + S + (want_jac ? (int64_t)S * T : 0);  // [3][ni+1]
+  double* drate = smem + S + 1;
+  double* yrow = smem + S + 1 + (want_jac ? (int64_t)S * T + 1 : 0);  // [3][ni+1]
+  if (tid == 0) {
+    rate[S] = 0.0;
+    if (want_jac) drate[S * T] = 0.0;
+  }
   const double* y = L.y + col.off_s;
   const double* atmb = L.atm ? L.atm + col.off_a : nullptr;
   AtmView av;
@@ -405,15 +420,20 @@ __global__ __launch_bounds__(NT) void k_chem_layer(LayerArgs L) {
   if (want_rhs) {
     for (int vl = tid; vl < VK_CHEM_NT; vl += NT) {
       double acc = 0.0;
-      walk_records(net.frec + vl, net.frec_k, [&](const VkRec& rc) {
+      // a term-less record adds c * rate[S] = +-0, which leaves acc (never
+      // -0: it starts at +0) unchanged
+      walk_records(
+          net.frec + vl, net.frec_k,
+          [&](const VkRec& rc) { return (double)rc.c * rate[vk_rec_item(rc.w)]; },
+          [&](const VkRec& rc, double t) {
 #pragma clang fp contract(off)
-        const uint32_t item = vk_rec_item(rc.w), dest = vk_rec_dest(rc.w);
-        if (item != VK_REC_NONE_ITEM) acc = acc + (double)rc.c * rate[item];
-        if (dest != VK_REC_NONE_DEST) {
-          part[dest] = acc;
-          acc = 0.0;
-        }
-      });
+            acc = acc + t;
+            const uint32_t dest = vk_rec_dest(rc.w);
+            if (dest != VK_REC_NONE_DEST) {
+              part[dest] = acc;
+              acc = 0.0;
+            }
+          });
     }
     __syncthreads();
   }
@@ -487,30 +507,40 @@ __global__ __launch_bounds__(NT) void k_chem_layer(LayerArgs L) {
   __syncthreads();
   // Jacobian entries from the entry-major CSR (chem.py:187-208)
   if (L.mode == M_LHS) {
-    // tile-layout D = c0 I - J - diag(dd), fix rows -> c0 e_i; each tile
+    // tile-layout D = c0 I - J - diag(dd), fix rows -> c0 e_i.  Each tile
     // entry is closed by one record of one lane (the jac list of (i, q) in
-    // order), which writes it to its tile slot
+    // order), which writes 0 - J to its tile slot (padding entries have no
+    // terms: +0); then thread i finishes row i's diagonal entry, (c0 + (0 - J))
+    // + (-dd_i), the same bits as (c0 - J) + (-dd_i), or writes the whole row
+    // of a fixed species.
     constexpr int BLK = vk_blk<SL>();
     double* dst = L.out_blk + col.off_b + (int64_t)z * BLK;
     for (int vl = tid; vl < VK_CHEM_NT; vl += NT) {
       double J = 0.0;
-      walk_records(net.jrec + vl, net.jrec_k, [&](const VkRec& rc) {
+      walk_records(
+          net.jrec + vl, net.jrec_k,
+          [&](const VkRec& rc) { return (double)rc.c * drate[vk_rec_item(rc.w)]; },
+          [&](const VkRec& rc, double t) {
 #pragma clang fp contract(off)
-        const uint32_t item = vk_rec_item(rc.w), dest = vk_rec_dest(rc.w);
-        if (item != VK_REC_NONE_ITEM) J = J + (double)rc.c * drate[item];
-        if (dest != VK_REC_NONE_DEST) {
-          const int blk = (int)dest / (VK_G * VK_G), rr = (int)dest % (VK_G * VK_G);
-          const int i = (blk / SL) * VK_G + rr / VK_G, q = (blk % SL) * VK_G + rr % VK_G;
-          double v = 0.0;
-          if (i < ni && q < ni) {
-            v = (i == q ? c0 : 0.0) - J;
-            if (i == q) v = v + (-dd_s[i]);
-            if (fix_s[i]) v = (i == q) ? c0 : 0.0;
-          }
-          dst[dest] = v;
-          J = 0.0;
-        }
-      });
+            J = J + t;  // term-less records add +-0 (see rate[S])
+            const uint32_t dest = vk_rec_dest(rc.w);
+            if (dest != VK_REC_NONE_DEST) {
+              dst[dest] = 0.0 - J;
+              J = 0.0;
+            }
+          });
+    }
+    __syncthreads();
+    for (int i = tid; i < ni; i += NT) {
+#pragma clang fp contract(off)
+      const int a = i / VK_G, r = i % VK_G;
+      if (fix_s[i]) {
+        for (int q = 0; q < ni; ++q)
+          dst[((a * SL + q / VK_G) * VK_G + r) * VK_G + q % VK_G] = (q == i) ? c0 : 0.0;
+      } else {
+        double* e = dst + ((a * SL + a) * VK_G + r) * VK_G + r;
+        *e = (c0 + *e) + (-dd_s[i]);
+      }
     }
   } else {
     double* dst = L.out0 + col.off_s * ni + (int64_t)z * ni * ni;
@@ -564,7 +594,7 @@ int vk_launch_chem_layer(int sl, int mode, const VkCol* cols, int ncols, int max
   const size_t n16 = (size_t)VK_G * (sl > 0 ? sl : 1);
   const size_t nchunk_max = (2 * (size_t)max_S * max_T + VK_FCHUNK - 1) / VK_FCHUNK + 2 * n16;
   const size_t shmem =
-      sizeof(double) * ((size_t)max_S + (want_jac ? (size_t)max_S * max_T : 0) +
+      sizeof(double) * ((size_t)max_S + 1 + (want_jac ? (size_t)max_S * max_T + 1 : 0) +
                         3 * (n16 + 1) + nchunk_max);
   if (shmem > 160 * 1024) {
     vk_set_error("chem layer needs %zu bytes of LDS (S=%d, T=%d)", shmem, max_S, max_T);

@@ -74,8 +74,9 @@ constexpr int VK_CHEM_NT = 512;
 // above, for the VK_CHEM_NT lanes of a chemistry workgroup.  Record k of
 // lane l sits at [k * VK_CHEM_NT + l], so a wave's loads are
 // contiguous, and lanes carry near-equal record counts however uneven the
-// lists are.  A record adds c * src[item] to the lane's running sum (no term
-// when item is VK_REC_NONE_ITEM); a record whose dest is not
+// lists are.  A record adds c * src[item] to the lane's running sum (a
+// record with no term points item at the zero slot after the table: rate[S]
+// for frec, drate[S * T] for jrec); a record whose dest is not
 // VK_REC_NONE_DEST closes the sum into output dest and restarts it at zero.
 //   jrec: one output per tile entry of D (dest = tile index), whose terms
 //         are its jac list in order, so each entry is the same sequential
@@ -91,7 +92,7 @@ struct VkRec {
   float c;
   uint32_t w;
 };
-constexpr uint32_t VK_REC_NONE_ITEM = 0x3FFFFu;  // 18 bits
+constexpr uint32_t VK_REC_NONE_ITEM = 0x3FFFFu;  // 18 bits: S * T must stay below
 constexpr uint32_t VK_REC_NONE_DEST = 0x3FFFu;   // 14 bits
 __host__ __device__ constexpr uint32_t vk_rec_item(uint32_t w) { return w & 0x3FFFFu; }
 __host__ __device__ constexpr uint32_t vk_rec_dest(uint32_t w) { return w >> 18; }
