@@ -20,7 +20,7 @@ import time
 from pathlib import Path
 
 FACTOR_CR = 6    # vk_tune_set(VK_TUNE_FACTOR, sl, VK_FACTOR_CR): cyclic-reduction factor
-VK_TUNE_REFINE, VK_TUNE_SPLIT, VK_TUNE_KDIRECT = 2, 3, 4
+VK_TUNE_REFINE, VK_TUNE_SPLIT, VK_TUNE_KDIRECT, VK_TUNE_HOLDX = 2, 3, 4, 5
 # The synthetic initial states put trace species at 1e-20 of the layer: at the
 # reference's default dt_min (1e-14) the error estimate exceeds rtol and the
 # force-accept rule (which keeps y_prev) freezes a column; the steady-state
@@ -270,6 +270,9 @@ def main() -> int:
     ap.add_argument("--kdirect", type=int, default=-1,
                     help="chemistry reads k in its (S, nz) layout (VK_TUNE_KDIRECT 1) or a "
                          "layer-major copy (0); -1 keeps the library default")
+    ap.add_argument("--holdx", type=int, default=-1,
+                    help="refined sweeps keep X_j in registers for both of its products "
+                         "(VK_TUNE_HOLDX 1) or read it again (0); -1 keeps the library default")
     ap.add_argument("--refine", type=int, default=-1,
                     help="refined block solves (VK_TUNE_REFINE: 0 none, 1 stage 1, 2 both; "
                          "-1: the library default)")
@@ -324,6 +327,8 @@ def main() -> int:
         _lib.check(lib.vk_tune_set(VK_TUNE_REFINE, 0, args.refine), "vk_tune_set")
     if args.kdirect >= 0:
         _lib.check(lib.vk_tune_set(VK_TUNE_KDIRECT, 0, args.kdirect), "vk_tune_set")
+    if args.holdx >= 0:
+        _lib.check(lib.vk_tune_set(VK_TUNE_HOLDX, 0, args.holdx), "vk_tune_set")
     B = args.batch
     t_gen = time.time()
     cols = synth.columns_same_network(1002, B, NZ, NI, NREV, NPHOTO, dt=1e-10, name="bench")

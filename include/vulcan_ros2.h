@@ -274,6 +274,13 @@ int vk_ss_vjp_f64(vk_batch b, const double* y, const double* k, const double* at
  * the chemistry launches; 0 (default) first copies them layer-major (one
  * contiguous row per layer).  Results are bitwise equal. */
 #define VK_TUNE_KDIRECT 4
+/* Knob VK_TUNE_HOLDX (sl ignored): the refined block solves on the Schur
+ * complements the tile factor stores.  1 (default): each layer reads X_j and
+ * A'_j once and keeps X_j in registers for both of its products (two block
+ * reads per layer, two columns per CU at SL 5-6); 0: X_j is read again for
+ * the correction (three block reads, four columns per CU).  Results are
+ * bitwise equal. */
+#define VK_TUNE_HOLDX 5
 int vk_tune_set(int knob, int sl, int value);
 int vk_tune_get(int knob, int sl); /* -1 for an unknown knob / bucket */
 

@@ -81,6 +81,7 @@ int vk_launch_bt_factor(int sl, const VkCol* cols, int ncols, const double* diag
                         hipStream_t stream, double* schur = nullptr);
 bool vk_factor_fuses_sweep(int sl);
 bool vk_factor_stores_schur(int sl);
+extern int g_solve_hold;
 bool vk_factor_is_cr(int sl);
 int vk_launch_nat_to_tile(int sl, const VkCol* cols, int ncols, int max_nz, const double* nat,
                           double* tile, int transpose, hipStream_t stream);
@@ -1405,6 +1406,14 @@ int vk_tune_set(int knob, int sl, int value) {
     g_k_direct = value;
     return VK_OK;
   }
+  if (knob == VK_TUNE_HOLDX) {
+    if (value != 0 && value != 1) {
+      vk_set_error("vk_tune_set: VK_TUNE_HOLDX takes 0 or 1, not %d", value);
+      return VK_ERR_ARG;
+    }
+    g_solve_hold = value;
+    return VK_OK;
+  }
   if (knob == VK_TUNE_MOLPRE) {
     if (value != 0 && value != 1) {
       vk_set_error("vk_tune_set: VK_TUNE_MOLPRE takes 0 or 1, not %d", value);
@@ -1428,6 +1437,7 @@ int vk_tune_set(int knob, int sl, int value) {
 }
 int vk_tune_get(int knob, int sl) {
   if (knob == VK_TUNE_MOLPRE) return g_mol_precompute;
+  if (knob == VK_TUNE_HOLDX) return g_solve_hold;
   if (knob == VK_TUNE_KDIRECT) return g_k_direct;
   if (knob == VK_TUNE_REFINE) return g_refine;
   if (knob == VK_TUNE_SPLIT) return g_split;

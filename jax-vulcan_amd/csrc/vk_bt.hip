@@ -479,8 +479,21 @@ struct RowAhead {
   }
 };
 
+// Waves per SIMD the sweep kernel is compiled for.  REFINE 3 holds two and
+// a half blocks in registers (X_j across the layer, A'_j beside it, part of
+// X_{j-1} read ahead): four columns per CU up to SL 3, three at SL 4, two
+// at SL 5-6.
+constexpr int solve_waves(int sl, int refine) {
+  return sl > 6 ? 1 : refine == 1 ? 2 : refine == 3 ? (sl <= 3 ? 4 : sl == 4 ? 3 : 2) : 4;
+}
+
+// REFINE 3 is REFINE 2 with X_j loaded once per layer and kept in
+// registers for both of its products (two block reads per refined layer
+// instead of three), at fewer columns per CU; the forward sweep reads each
+// block one layer ahead.  The arithmetic is REFINE 2's, so the two give the
+// same bits.
 template <int SL, int REFINE>
-__global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ? (REFINE == 1 ? 2 : 4) : 1))) void k_bt_solve(
+__global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(solve_waves(SL, REFINE)))) void k_bt_solve(
     const VkCol* __restrict__ cols, int ncols, const double* __restrict__ inv,
     const double* __restrict__ sup_d, const double* __restrict__ sub_d,
     const double* __restrict__ rhs, double* __restrict__ rscratch, double* x,
@@ -525,13 +538,23 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ?
       ra.store(ahead[1]);
     }
     __syncthreads();
+    if constexpr (REFINE == 3) {
+      if (nz > 1) load_tile<SL>(a, stk);
+    }
     for (int j = 1; j < nz; ++j) {
       RowAhead<SL, 2> ra;
       if (j + 1 < nz) {
         const double* src[2] = {rhs + col.off_s + (int64_t)(j + 1) * ni, bsub + (int64_t)j * ni};
         ra.load(src, ni);
       }
-      load_tile<SL>(a, stk + (int64_t)(j - 1) * BLK);
+      // REFINE 3: X_{j-1} was read during the previous layer; X_j is read
+      // now, into an, and taken over once this layer is done
+      double an[SL][SL];
+      if constexpr (REFINE == 3) {
+        if (j + 1 < nz) load_tile<SL>(an, stk + (int64_t)j * BLK);
+      } else {
+        load_tile<SL>(a, stk + (int64_t)(j - 1) * BLK);
+      }
       tile_matvec<SL>(a, vec[(j - 1) & 1], w);
       if (j + 1 < nz) ra.store(ahead[(j + 1) & 1]);
       if (tc == 0) {
@@ -548,6 +571,14 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ?
         }
       }
       __syncthreads();
+      if constexpr (REFINE == 3) {
+        if (j + 1 < nz) {
+#pragma unroll
+          for (int s = 0; s < SL; ++s)
+#pragma unroll
+            for (int t = 0; t < SL; ++t) a[s][t] = an[s][t];
+        }
+      }
     }
     rm = rscratch;
     __syncthreads();
@@ -579,7 +610,7 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ?
       }
       __syncthreads();
     }
-  } else if constexpr (REFINE == 2) {
+  } else if constexpr (REFINE == 2 || REFINE == 3) {
     // per layer: x0 = X_j t_j; r = t_j - A'_j x0; x_j = x0 + X_j r
     const double* sblk = dstk + col.off_b;
     bool wide = false;  // thread 0: some layer's correction exceeded its solution
@@ -601,6 +632,19 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ?
       ra.store(ahead[(nz - 1) & 1]);
     }
     __syncthreads();
+    // REFINE 2: no block stays in registers across the layer; each of the
+    // three products loads its whole block first, so all its loads are in
+    // flight at once, and X_j is read again for the correction.
+    // REFINE 3: X_j stays in registers for both of its products, and the
+    // next layer's blocks are read while this one computes: the first
+    // SLH tile rows of X_{j-1} at the start of the layer, A'_{j-1} once
+    // A'_j is used, the other rows of X_{j-1} once X_j is used.
+    constexpr int SLH = (SL + 1) / 2;
+    double xh[SL][SL], sh2[SL][SL];
+    if constexpr (REFINE == 3) {
+      load_tile<SL>(xh, stk + (int64_t)(nz - 1) * BLK);
+      load_tile<SL>(sh2, sblk + (int64_t)(nz - 1) * BLK);
+    }
     for (int j = nz - 1; j >= 0; --j) {
       RowAhead<SL, 6> ra;
       if (j > 0) {
@@ -608,10 +652,17 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ?
         ahead_src(j - 1, src);
         ra.load(src, ni);
       }
-      // No block stays in registers across the layer: each of the three
-      // products loads its whole block first, so all its loads are in
-      // flight at once; X_j is read again for the correction (from cache).
-      {
+      const double* xnext = stk + (int64_t)(j - 1) * BLK;  // X_{j-1} when j > 0
+      double xn[SLH][SL];
+      if constexpr (REFINE == 3) {
+        if (j > 0) {
+#pragma unroll
+          for (int s = 0; s < SLH; ++s)
+#pragma unroll
+            for (int t = 0; t < SL; ++t) xn[s][t] = xnext[(s * SL + t) * VK_NT + tid];
+        }
+        tile_matvec<SL>(xh, vec[j & 1], w);                   // x0_j
+      } else {
         double t0[SL][SL];
         load_tile<SL>(t0, stk + (int64_t)j * BLK);
         tile_matvec<SL>(t0, vec[j & 1], w);                   // x0_j
@@ -626,7 +677,10 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ?
       }
       __syncthreads();
       double q[SL];
-      {
+      if constexpr (REFINE == 3) {
+        tile_matvec<SL>(sh2, rv, q);                          // A'_j x0_j
+        if (j > 0) load_tile<SL>(sh2, sblk + (int64_t)(j - 1) * BLK);
+      } else {
         double t1[SL][SL];
         load_tile<SL>(t1, sblk + (int64_t)j * BLK);
         tile_matvec<SL>(t1, rv, q);                           // A'_j x0_j
@@ -640,7 +694,16 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(SL <= 6 ?
       }
       __syncthreads();
       double d[SL];
-      {
+      if constexpr (REFINE == 3) {
+        tile_matvec<SL>(xh, bx, d);                           // X_j (t_j - A'_j x0_j)
+        if (j > 0) {
+#pragma unroll
+          for (int s = 0; s < SL; ++s)
+#pragma unroll
+            for (int t = 0; t < SL; ++t)
+              xh[s][t] = s < SLH ? xn[s < SLH ? s : 0][t] : xnext[(s * SL + t) * VK_NT + tid];
+        }
+      } else {
         // an opaque copy of the address, so the compiler reloads X_j
         // instead of holding the first load's registers across the layer
         const double* xj = stk + (int64_t)j * BLK;
@@ -1638,6 +1701,11 @@ int vk_launch_bt_factor_blk(int sl, const VkCol* cols, int ncols, const double* 
 int g_factor_variant[8] = {VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE,
                            VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE, VK_FACTOR_TILE};
 
+// Refined sweeps on stored Schur complements: 1 (default) keeps X_j in
+// registers for both of its products (k_bt_solve<SL, 3>), 0 reads it twice
+// (k_bt_solve<SL, 2>).  Set by vk_tune_set (VK_TUNE_HOLDX).
+int g_solve_hold = 1;
+
 bool vk_factor_stores_schur(int sl) {
   return sl >= 1 && sl <= 7 && g_factor_variant[sl] == VK_FACTOR_TILE;
 }
@@ -1692,7 +1760,11 @@ int vk_launch_bt_solve(int sl, const VkCol* cols, int ncols, const double* inv,
                        double* rscratch, double* x, int mode, SolveEpi epi, const int* active,
                        hipStream_t stream, const double* refine_diag, int* refl,
                        const double* refine_schur) {
-  if (refine_schur) {
+  if (refine_schur && g_solve_hold) {
+    VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_bt_solve<SL_, 3>), dim3(ncols), dim3(VK_NT), 0,
+                                          stream, cols, ncols, inv, sup_d, sub_d, rhs, rscratch,
+                                          x, mode, epi, active, refine_schur, refl));
+  } else if (refine_schur) {
     VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_bt_solve<SL_, 2>), dim3(ncols), dim3(VK_NT), 0,
                                           stream, cols, ncols, inv, sup_d, sub_d, rhs, rscratch,
                                           x, mode, epi, active, refine_schur, refl));

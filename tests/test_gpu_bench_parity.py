@@ -119,3 +119,20 @@ def test_rate_table_layouts_bitwise_equal(bench_cols):
     finally:
         lib.vk_tune_set(VK_TUNE_KDIRECT, 0, 0)
     assert np.array_equal(base[0], direct[0]) and np.array_equal(base[1], direct[1])
+
+
+def test_held_inverse_sweeps_bitwise_equal(bench_cols):
+    """VK_TUNE_HOLDX: the refined sweeps keeping X_j in registers for both
+    of its products (two block reads per layer) give the same step, bit for
+    bit, as the sweeps reading X_j again for the correction."""
+    from vulcan_amd import _lib
+    lib = _lib.load()
+    VK_TUNE_HOLDX = 5
+    assert lib.vk_tune_get(VK_TUNE_HOLDX, 0) == 1
+    held = _step(bench_cols, FACTOR_TILE, 2)
+    _lib.check(lib.vk_tune_set(VK_TUNE_HOLDX, 0, 0), "vk_tune_set")
+    try:
+        reread = _step(bench_cols, FACTOR_TILE, 2)
+    finally:
+        lib.vk_tune_set(VK_TUNE_HOLDX, 0, 1)
+    assert np.array_equal(held[0], reread[0]) and np.array_equal(held[1], reread[1])

@@ -55,3 +55,29 @@ def test_mixed_edge_batch_is_bitwise_alone():
         ra = a.results()
         assert np.array_equal(ra["y"][0], res["y"][i]), i
         assert ra["accept"][0] == res["accept"][i]
+
+
+def test_held_inverse_sweeps_bitwise_equal_every_bucket():
+    """VK_TUNE_HOLDX 1 (X_j kept in registers across a refined layer) and 0
+    (X_j read again) give the same attempts, bit for bit, in every tile
+    size bucket of the edge shapes (SL 1, 2 and 7)."""
+    import torch
+    from vulcan_amd import _lib
+    from vulcan_amd.outer_loop import RunConfig, SteadyStateRunner
+    lib = _lib.load()
+    VK_TUNE_HOLDX = 5
+    cols = [_col(nz, ni, n_rev, 5000 + i) for i, (nz, ni, n_rev) in enumerate(SHAPES)]
+    out = []
+    for hold in (1, 0):
+        _lib.check(lib.vk_tune_set(VK_TUNE_HOLDX, 0, hold), "vk_tune_set")
+        try:
+            r = SteadyStateRunner(cols, RunConfig())
+            for _ in range(3):
+                r.attempt()
+            torch.cuda.synchronize()
+            out.append(r.results())
+        finally:
+            lib.vk_tune_set(VK_TUNE_HOLDX, 0, 1)
+    for key in ("y", "dt", "accept"):
+        for i in range(len(cols)):
+            assert np.array_equal(np.asarray(out[0][key][i]), np.asarray(out[1][key][i])), (key, i)
