@@ -1269,16 +1269,30 @@ __device__ inline void cr_epilogue(const SolveEpi& epi, int64_t e, double xv, do
 }  // namespace
 
 // Eliminate the odd rows of one level: inv(D_j), P_j, Q_j.  Grid (h, ncols).
-template <int SL>
-__global__ __launch_bounds__(VK_NT) void k_cr_elim(const VkCol* __restrict__ cols, int ncols,
+// WIDE: compiled for four workgroups per CU (128 VGPRs; what spills lies
+// outside the pivot steps), for levels with many blocks to invert; else
+// unconstrained, for the few-block levels whose time is one inversion's
+// latency.
+template <int SL, bool WIDE>
+__global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(WIDE && SL <= 6 ? 4 : 1))) void k_cr_elim(const VkCol* __restrict__ cols, int ncols,
                                                     const double* __restrict__ blk0,
                                                     const double* __restrict__ sup_d,
                                                     const double* __restrict__ sub_d,
                                                     double* __restrict__ cr, int level,
                                                     int* __restrict__ status,
                                                     const int* __restrict__ active) {
-  __shared__ BtShared<SL> sh;
-  __shared__ double ga[(VK_G + 1) * VK_G * SL], gb[VK_G * VK_G * SL];
+  // The inversion's shared state and the product panels are never live
+  // together (a barrier parts them), so they share their LDS: four
+  // workgroups fit a CU, as the registers allow, instead of three.
+  __shared__ union {
+    BtShared<SL> sh;
+    struct {
+      double a[(VK_G + 1) * VK_G * SL], b[VK_G * VK_G * SL];
+    } g;
+  } lds;
+  BtShared<SL>& sh = lds.sh;
+  double* ga = lds.g.a;
+  double* gb = lds.g.b;
   const int c = blockIdx.y, w = blockIdx.x;
   if (c >= ncols) return;
   if (active && (active[c] & VK_ST_DONE)) return;
@@ -1825,14 +1839,25 @@ bool vk_factor_is_cr(int sl) {
 // into the region cr (7 * block_doubles); one launch pair per level.  With
 // rhs_fused the update launches also reduce that right-hand side, and the
 // solve for it is called with fwd_done.
+// Levels whose launch has at least this many workgroups (one per block to
+// invert, idle ones of other columns included) take the four-per-CU
+// elimination kernel: more than three per CU of the 256 CUs.
+constexpr int64_t kCrWideBlocks = 768;
+
 int vk_launch_cr_factor(int sl, const VkCol* cols, int ncols, int max_nz, const double* blk,
                         const double* sup_d, const double* sub_d, double* cr, int* status,
                         const int* active, hipStream_t stream, const double* rhs_fused) {
   int level = 0;
   for (int n = max_nz; n > 1; n = (n + 1) / 2, ++level) {
-    VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_cr_elim<SL_>, dim3(n / 2, ncols), dim3(VK_NT), 0,
-                                          stream, cols, ncols, blk, sup_d, sub_d, cr, level,
-                                          status, active));
+    if ((int64_t)(n / 2) * ncols >= kCrWideBlocks) {
+      VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_cr_elim<SL_, true>), dim3(n / 2, ncols),
+                                            dim3(VK_NT), 0, stream, cols, ncols, blk, sup_d,
+                                            sub_d, cr, level, status, active));
+    } else {
+      VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_cr_elim<SL_, false>), dim3(n / 2, ncols),
+                                            dim3(VK_NT), 0, stream, cols, ncols, blk, sup_d,
+                                            sub_d, cr, level, status, active));
+    }
     VK_CHECK_HIP(hipGetLastError());
     VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_cr_update<SL_>, dim3((n + 1) / 2, ncols),
                                           dim3(VK_NT), 0, stream, cols, ncols, blk, sup_d, sub_d,
