@@ -1350,8 +1350,11 @@ __global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(WIDE && S
 // Even rows of one level become the rows of the next.  Grid (ceil(n/2), ncols).
 // Level 0 scales by the diffusion diagonals; later levels multiply dense
 // blocks on MFMA.
-template <int SL>
-__global__ __launch_bounds__(VK_NT) void k_cr_update(const VkCol* __restrict__ cols, int ncols,
+// WIDE (levels with many rows, as k_cr_elim): the products load their
+// operands panel by panel through LDS instead of both whole blocks into
+// registers first, which leaves room for two workgroups per CU.
+template <int SL, bool WIDE>
+__global__ __launch_bounds__(VK_NT) __attribute__((amdgpu_waves_per_eu(WIDE && SL <= 6 ? 2 : 1))) void k_cr_update(const VkCol* __restrict__ cols, int ncols,
                                                       const double* __restrict__ blk0,
                                                       const double* __restrict__ sup_d,
                                                       const double* __restrict__ sub_d,
@@ -1381,19 +1384,19 @@ __global__ __launch_bounds__(VK_NT) void k_cr_update(const VkCol* __restrict__ c
     cr_mm_load<SL>(T, cr + v.D + (int64_t)i * BLK);
     if (i >= 1) {
       const double* Qp = cr + v.Q + (int64_t)(w - 1) * BLK;
-      cr_mm<SL, true>(Qp, cr + v.U + (int64_t)(i - 1) * BLK, -1.0, T, ga, gb);
+      cr_mm<SL, !WIDE>(Qp, cr + v.U + (int64_t)(i - 1) * BLK, -1.0, T, ga, gb);
       if (i >= 2) {
         cr_mm_zero<SL>(X);
-        cr_mm<SL, true>(Qp, cr + v.L + (int64_t)(i - 1) * BLK, -1.0, X, ga, gb);
+        cr_mm<SL, !WIDE>(Qp, cr + v.L + (int64_t)(i - 1) * BLK, -1.0, X, ga, gb);
         cr_mm_store<SL>(X, cr + vn.L + (int64_t)w * BLK);
       }
     }
     if (has_p) {
       const double* Pp = cr + v.P + (int64_t)w * BLK;
-      cr_mm<SL, true>(Pp, cr + v.L + (int64_t)(i + 1) * BLK, -1.0, T, ga, gb);
+      cr_mm<SL, !WIDE>(Pp, cr + v.L + (int64_t)(i + 1) * BLK, -1.0, T, ga, gb);
       if (has_u) {
         cr_mm_zero<SL>(X);
-        cr_mm<SL, true>(Pp, cr + v.U + (int64_t)(i + 1) * BLK, -1.0, X, ga, gb);
+        cr_mm<SL, !WIDE>(Pp, cr + v.U + (int64_t)(i + 1) * BLK, -1.0, X, ga, gb);
         cr_mm_store<SL>(X, cr + vn.U + (int64_t)w * BLK);
       }
     }
@@ -1859,9 +1862,15 @@ int vk_launch_cr_factor(int sl, const VkCol* cols, int ncols, int max_nz, const 
                                             sub_d, cr, level, status, active));
     }
     VK_CHECK_HIP(hipGetLastError());
-    VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_cr_update<SL_>, dim3((n + 1) / 2, ncols),
-                                          dim3(VK_NT), 0, stream, cols, ncols, blk, sup_d, sub_d,
-                                          cr, level, rhs_fused, active));
+    if ((int64_t)((n + 1) / 2) * ncols >= kCrWideBlocks) {
+      VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_cr_update<SL_, true>), dim3((n + 1) / 2, ncols),
+                                            dim3(VK_NT), 0, stream, cols, ncols, blk, sup_d,
+                                            sub_d, cr, level, rhs_fused, active));
+    } else {
+      VK_DISPATCH_SL(sl, hipLaunchKernelGGL((k_cr_update<SL_, false>), dim3((n + 1) / 2, ncols),
+                                            dim3(VK_NT), 0, stream, cols, ncols, blk, sup_d,
+                                            sub_d, cr, level, rhs_fused, active));
+    }
     VK_CHECK_HIP(hipGetLastError());
   }
   VK_DISPATCH_SL(sl, hipLaunchKernelGGL(k_cr_top<SL_>, dim3(ncols), dim3(VK_NT), 0, stream, cols,
