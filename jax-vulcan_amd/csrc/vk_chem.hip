@@ -244,14 +244,25 @@ __device__ inline DiffCoef mol_coef_pre(int z, int nz, const double* m, int64_t 
 // terms of a whole group, term(rec) = c * table[item], are formed first, all
 // their table reads in flight together, and close(rec, term) then adds each
 // to the running sum in record order and stores closed sums.
-template <class Term, class Close>
+//
+// EVEN: the last group reads itself again, so every pass issues the same
+// eight loads and the wait for the current group leaves the next group's
+// loads in flight (with a pass that issued none, that wait drains every
+// load).  It pays in the walks whose closes write LDS (the loss/production
+// sums); the Jacobian walk, whose closes store to memory, measured faster
+// without it.
+template <bool EVEN, class Term, class Close>
 __device__ __forceinline__ void walk_records(const VkRec* rec, int count, Term&& term,
                                              Close&& close) {
   VkRec cur[VK_REC_GROUP], nxt[VK_REC_GROUP];
 #pragma unroll
   for (int g = 0; g < VK_REC_GROUP; ++g) cur[g] = rec[(int64_t)g * VK_CHEM_NT];
   for (int k0 = 0; k0 < count; k0 += VK_REC_GROUP) {
-    if (k0 + VK_REC_GROUP < count) {
+    if constexpr (EVEN) {
+      const int kn = k0 + VK_REC_GROUP < count ? k0 + VK_REC_GROUP : k0;
+#pragma unroll
+      for (int g = 0; g < VK_REC_GROUP; ++g) nxt[g] = rec[(int64_t)(kn + g) * VK_CHEM_NT];
+    } else if (k0 + VK_REC_GROUP < count) {
 #pragma unroll
       for (int g = 0; g < VK_REC_GROUP; ++g)
         nxt[g] = rec[(int64_t)(k0 + VK_REC_GROUP + g) * VK_CHEM_NT];
@@ -422,7 +433,7 @@ __global__ __launch_bounds__(NT) void k_chem_layer(LayerArgs L) {
       double acc = 0.0;
       // a term-less record adds c * rate[S] = +-0, which leaves acc (never
       // -0: it starts at +0) unchanged
-      walk_records(
+      walk_records<true>(
           net.frec + vl, net.frec_k,
           [&](const VkRec& rc) { return (double)rc.c * rate[vk_rec_item(rc.w)]; },
           [&](const VkRec& rc, double t) {
@@ -517,7 +528,7 @@ __global__ __launch_bounds__(NT) void k_chem_layer(LayerArgs L) {
     double* dst = L.out_blk + col.off_b + (int64_t)z * BLK;
     for (int vl = tid; vl < VK_CHEM_NT; vl += NT) {
       double J = 0.0;
-      walk_records(
+      walk_records<false>(
           net.jrec + vl, net.jrec_k,
           [&](const VkRec& rc) { return (double)rc.c * drate[vk_rec_item(rc.w)]; },
           [&](const VkRec& rc, double t) {
