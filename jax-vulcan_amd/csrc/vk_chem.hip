@@ -285,8 +285,15 @@ __device__ __forceinline__ void walk_records(const VkRec* rec, int count, Term&&
 // (rates and their derivatives) allows two workgroups per CU, and half that
 // for the others; each thread walks the record streams of lanes tid,
 // tid + NT, ... of the VK_CHEM_NT-lane tables.
+// The 256-thread (right-hand-side) launches are compiled for six waves per
+// SIMD (80 VGPRs; what spills are scalar registers, into vector lanes), so
+// five workgroups fit a CU as their LDS allows, not four: the stage-2 RHS
+// went from 3.79 to 3.33 ms at 1024 columns.  The 512-thread Jacobian
+// launches keep their registers: their LDS allows two per CU either way,
+// and a variant that formed the derivatives after the chunk sums, in their
+// space, to fit three ran slower (10.6 against 9.9 ms).
 template <int SL, int NT>
-__global__ __launch_bounds__(NT) void k_chem_layer(LayerArgs L) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT <= 256 ? 6 : 4))) void k_chem_layer(LayerArgs L) {
   extern __shared__ double smem[];
   __shared__ double ysum3[3];
   __shared__ double dd_s[VK_G * SL];  // A+Ai (+ bottom vdep for M_LHS) per species
