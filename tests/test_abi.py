@@ -148,3 +148,25 @@ def test_network_create_rejects_bad_tables_without_gpu():
     assert rc != 0 and "unsupported sizes" in msg
     rc, msg = create(ni, ok_r, ok_p, ok_s, ok_s, T=5)
     assert rc != 0 and "unsupported sizes" in msg
+
+
+def test_tuning_knobs_without_gpu():
+    """vk_tune_set / vk_tune_get are host-only: defaults, round trips and
+    rejected values for the sweep, refinement and layout knobs."""
+    from vulcan_amd import _lib
+    lib = _lib.load()
+    VK_TUNE_REFINE, VK_TUNE_SPLIT, VK_TUNE_KDIRECT, VK_TUNE_HOLDX = 2, 3, 4, 5
+    assert lib.vk_tune_get(VK_TUNE_HOLDX, 0) == 1
+    assert lib.vk_tune_get(VK_TUNE_REFINE, 0) == 2
+    assert lib.vk_tune_get(VK_TUNE_KDIRECT, 0) == 0
+    assert lib.vk_tune_get(VK_TUNE_SPLIT, 0) == 1
+    try:
+        assert lib.vk_tune_set(VK_TUNE_HOLDX, 0, 0) == 0
+        assert lib.vk_tune_get(VK_TUNE_HOLDX, 0) == 0
+        assert lib.vk_tune_set(VK_TUNE_HOLDX, 0, 2) != 0
+        assert b"VK_TUNE_HOLDX" in lib.vk_last_error()
+        assert lib.vk_tune_get(VK_TUNE_HOLDX, 0) == 0
+    finally:
+        lib.vk_tune_set(VK_TUNE_HOLDX, 0, 1)
+    assert lib.vk_tune_get(VK_TUNE_HOLDX, 0) == 1
+    assert lib.vk_tune_get(99, 0) == -1
