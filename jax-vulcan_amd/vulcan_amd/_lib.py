@@ -107,6 +107,17 @@ def load():
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
             f"g.build()'` (hipcc, gfx950).  There is no CPU fallback.")
+    # PyTorch-ROCm carries its own HIP and HSA runtimes beside the system
+    # ones this library links.  The device tensors the wrappers pass are
+    # torch's, so torch's runtime opens the GPU first: a first allocation
+    # made by this library in a process where torch has not yet touched the
+    # GPU failed (hipMalloc in vk_network_create).
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except ImportError:
+        pass
     _lib = _Bound(C.CDLL(str(LIB_PATH)))
     return _lib
 

@@ -361,3 +361,21 @@ def test_cr_wide_buckets_match_tile(ni):
     xd = np.linalg.solve(A, rhs.reshape(-1)).reshape(nz, ni)
     assert int(s.item()) == 0
     assert np.abs(_np(x) - xd).max() < 1e-10 * np.abs(xd).max()
+
+
+def test_cr_wide_levels_are_bitwise_narrow():
+    """Sixteen nz=150, ni=93 columns: the first cyclic-reduction levels
+    launch 16 x 75 workgroups, enough for the four-per-CU elimination build
+    and the panel-loading update build; deeper levels, and each column
+    alone, take the narrow builds.  The builds perform the same operations,
+    so every column's step equals its step alone, bit for bit."""
+    from vulcan_amd import synth
+    cols = synth.columns_same_network(1002, 16, 150, 93, 1200, 60, dt=1e-10, name="crwide")
+    with factor(VK_FACTOR_CR):
+        batch, st = _batch_step(cols)
+        alone = [_batch_step([c])[0][0] for c in cols[:4]] + [None] * 12
+    assert not st.any()
+    for c in range(4):
+        assert np.array_equal(batch[c], alone[c]), c
+    for c in range(16):
+        assert np.all(np.isfinite(batch[c]))
