@@ -632,6 +632,7 @@ def main() -> int:
         "single_column_steps_per_s": round(single, 3),
         "single_column_cr_steps_per_s": round(single_cr, 3),
         "batched_over_single": round(value / single, 2),
+        "batched_over_single_cr": round(value / single_cr, 2),
         "kernel_ms_per_launch": {"chem_lhs": round(kern_ms[0], 4), "bt_factor": round(kern_ms[1], 4),
                                  "bt_back1": round(kern_ms[2], 4), "rhs2": round(kern_ms[3], 4),
                                  "bt_solve2": round(kern_ms[4], 4),
