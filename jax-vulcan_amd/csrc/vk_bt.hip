@@ -1844,7 +1844,7 @@ bool vk_factor_is_cr(int sl) {
 // solve for it is called with fwd_done.
 // Levels whose launch has at least this many workgroups (one per block to
 // invert, idle ones of other columns included) take the four-per-CU
-// elimination kernel: more than three per CU of the 256 CUs.
+// elimination and update kernels: three or more per CU of the 256 CUs.
 constexpr int64_t kCrWideBlocks = 768;
 
 int vk_launch_cr_factor(int sl, const VkCol* cols, int ncols, int max_nz, const double* blk,
